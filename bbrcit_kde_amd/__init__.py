@@ -1,0 +1,304 @@
+"""bbrcit_kde_amd: MI355X-native direct kernel-density sums.
+
+This package is a thin Python view of the C ABI in ``include/bbrcit_hip.h``
+(the library ``lib/libbbrcit_hip.so``).  The product interface is the C ABI
+and the header-only C++ API under ``include/`` of this package; this module
+exists so tests and benchmarks can drive the same entry points from Python.
+
+:class:`DirectKde` mirrors the reference's ``CudaDirectKde``
+(include/CudaDirectKde.h:28-123): construct it over reference and query
+points, then evaluate closed index ranges.  Range errors raise the Python
+counterparts of the reference's exceptions: ``IndexError`` for
+``std::out_of_range`` and ``ValueError`` for ``std::invalid_argument``.
+
+There is no CPU fallback: if the library is missing, loading it raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+from .build import LIB_PATH
+
+__all__ = [
+    "BK_EPANECHNIKOV", "BK_GAUSSIAN", "BK_EPANECHNIKOV_PRODUCT_2D", "BK_GAUSSIAN_PRODUCT_2D",
+    "BK_EPANECHNIKOV_CONV_1D", "BK_EPANECHNIKOV_PRODUCT_CONV_2D", "BK_GAUSSIAN_CONV_1D",
+    "BK_GAUSSIAN_PRODUCT_CONV_2D", "BK_PREC_F32", "BK_PREC_F64", "BkError", "KernelDesc",
+    "DirectKde", "load_library", "library_path",
+]
+
+# ---- constants (mirror include/bbrcit_hip.h) ---------------------------------
+BK_OK = 0
+BK_ERR_INVALID_ARG = -1
+BK_ERR_OUT_OF_RANGE = -2
+BK_ERR_HIP = -3
+BK_ERR_STATE = -4
+BK_ERR_UNSUPPORTED = -5
+BK_ERR_ALLOC = -6
+
+BK_SET_REF = 0
+BK_SET_QUERY = 1
+BK_PREC_F32 = 0
+BK_PREC_F64 = 1
+
+BK_EPANECHNIKOV = 1
+BK_GAUSSIAN = 2
+BK_EPANECHNIKOV_PRODUCT_2D = 3
+BK_GAUSSIAN_PRODUCT_2D = 4
+BK_EPANECHNIKOV_CONV_1D = 5
+BK_EPANECHNIKOV_PRODUCT_CONV_2D = 6
+BK_GAUSSIAN_CONV_1D = 7
+BK_GAUSSIAN_PRODUCT_CONV_2D = 8
+
+KIND_NAMES = {
+    BK_EPANECHNIKOV: "EpanechnikovKernel",
+    BK_GAUSSIAN: "GaussianKernel",
+    BK_EPANECHNIKOV_PRODUCT_2D: "EpanechnikovProductKernel2d",
+    BK_GAUSSIAN_PRODUCT_2D: "GaussianProductKernel2d",
+    BK_EPANECHNIKOV_CONV_1D: "EpanechnikovConvKernel1d",
+    BK_EPANECHNIKOV_PRODUCT_CONV_2D: "EpanechnikovProductConvKernel2d",
+    BK_GAUSSIAN_CONV_1D: "GaussianConvKernel1d",
+    BK_GAUSSIAN_PRODUCT_CONV_2D: "GaussianProductConvKernel2d",
+}
+
+
+class KernelDesc(ctypes.Structure):
+    """bk_kernel_desc: kernel kind, dimension and bandwidth(s)."""
+
+    _fields_ = [("kind", ctypes.c_int), ("dim", ctypes.c_int), ("hx", ctypes.c_double), ("hy", ctypes.c_double)]
+
+    def __init__(self, kind: int, dim: int, hx: float, hy: Optional[float] = None):
+        super().__init__(int(kind), int(dim), float(hx), float(hx if hy is None else hy))
+
+    def __repr__(self) -> str:
+        return f"KernelDesc({KIND_NAMES.get(self.kind, self.kind)}, dim={self.dim}, hx={self.hx}, hy={self.hy})"
+
+
+class RangePair(ctypes.Structure):
+    """bk_range_pair: closed reference and query index ranges."""
+
+    _fields_ = [("r_i", ctypes.c_size_t), ("r_j", ctypes.c_size_t), ("q_i", ctypes.c_size_t), ("q_j", ctypes.c_size_t)]
+
+
+class BkError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_LIB: Optional[ctypes.CDLL] = None
+
+_DBL_P = ctypes.POINTER(ctypes.c_double)
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libbbrcit_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m bbrcit_kde_amd.build` first")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    c_ctx_p = ctypes.c_void_p
+    sig = {
+        "bk_create": (ctypes.c_int, [ctypes.POINTER(c_ctx_p), ctypes.c_int, ctypes.c_int]),
+        "bk_destroy": (ctypes.c_int, [c_ctx_p]),
+        "bk_last_error": (ctypes.c_char_p, []),
+        "bk_set_stream": (ctypes.c_int, [c_ctx_p, ctypes.c_void_p]),
+        "bk_upload": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, _DBL_P, _DBL_P, _DBL_P]),
+        "bk_alias_queries_to_refs": (ctypes.c_int, [c_ctx_p]),
+        "bk_size": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+        "bk_update_refs": (ctypes.c_int, [c_ctx_p, _DBL_P, _DBL_P]),
+        "bk_eval_range": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.c_size_t, ctypes.c_size_t, _DBL_P, ctypes.c_int]),
+        "bk_eval_range_device": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
+                                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p,
+                                                ctypes.c_int]),
+        "bk_eval_leaf_pairs": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.POINTER(RangePair),
+                                              ctypes.c_size_t, _DBL_P, ctypes.c_int]),
+        "bk_self_eval": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_int, _DBL_P, ctypes.c_int]),
+        "bk_likelihood_cv": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), _DBL_P, _DBL_P]),
+        "bk_lsq_conv_cv": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), _DBL_P, _DBL_P]),
+        "bk_adapt": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_double, _DBL_P, _DBL_P, _DBL_P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status == BK_OK:
+        return
+    msg = load_library().bk_last_error().decode(errors="replace")
+    if status == BK_ERR_OUT_OF_RANGE:
+        raise IndexError(msg)
+    if status == BK_ERR_INVALID_ARG:
+        raise ValueError(msg)
+    raise BkError(status, msg)
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_DBL_P)
+
+
+def _as_points(coords) -> np.ndarray:
+    c = np.ascontiguousarray(np.asarray(coords, dtype=np.float64))
+    if c.ndim == 1:
+        c = c[:, None]
+    if c.ndim != 2:
+        raise ValueError("coordinates must be an (n, D) array")
+    return c
+
+
+def _opt_vec(v, n: int) -> Optional[np.ndarray]:
+    if v is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+    if a.shape != (n,):
+        raise ValueError(f"expected {n} values, got shape {a.shape}")
+    return a
+
+
+class DirectKde:
+    """Direct kernel-density evaluator on one GPU (mirror of CudaDirectKde).
+
+    ``refs`` and ``queries`` are (n, D) coordinate arrays; ``ref_mass`` and
+    ``ref_abw`` give each reference its mass and adaptive bandwidth factor
+    (default 1).  ``queries=None`` makes the query set the reference set
+    itself (needed by the leave-one-out calls).
+    """
+
+    def __init__(self, refs, queries=None, ref_mass=None, ref_abw=None, *, precision: int = BK_PREC_F32,
+                 device: int = 0):
+        lib = load_library()
+        self._lib = lib
+        self._ctx = ctypes.c_void_p()
+        _check(lib.bk_create(ctypes.byref(self._ctx), int(device), int(precision)))
+        self.precision = precision
+        r = _as_points(refs)
+        self.dim = r.shape[1]
+        self._upload(BK_SET_REF, r, _opt_vec(ref_mass, len(r)), _opt_vec(ref_abw, len(r)))
+        if queries is None:
+            _check(lib.bk_alias_queries_to_refs(self._ctx))
+        else:
+            q = _as_points(queries)
+            self._upload(BK_SET_QUERY, q, None, None)
+
+    def _upload(self, which: int, coords: np.ndarray, mass, abw) -> None:
+        _check(self._lib.bk_upload(self._ctx, which, coords.shape[1], coords.shape[0], _dptr(coords), _dptr(mass),
+                                   _dptr(abw)))
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._lib.bk_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- sizes ----------------------------------------------------------------
+    def reference_size(self) -> int:
+        n = ctypes.c_size_t()
+        _check(self._lib.bk_size(self._ctx, BK_SET_REF, ctypes.byref(n)))
+        return n.value
+
+    def query_size(self) -> int:
+        n = ctypes.c_size_t()
+        _check(self._lib.bk_size(self._ctx, BK_SET_QUERY, ctypes.byref(n)))
+        return n.value
+
+    def set_stream(self, stream_handle: int) -> None:
+        _check(self._lib.bk_set_stream(self._ctx, ctypes.c_void_p(int(stream_handle))))
+
+    def update_refs(self, mass=None, abw=None) -> None:
+        n = self.reference_size()
+        _check(self._lib.bk_update_refs(self._ctx, _dptr(_opt_vec(mass, n)), _dptr(_opt_vec(abw, n))))
+
+    # -- evaluation -------------------------------------------------------------
+    def unnormalized_eval(self, desc: KernelDesc, r_i: int, r_j: int, q_i: int, q_j: int) -> np.ndarray:
+        self._check_indices(r_i, r_j, q_i, q_j)
+        out = np.empty(max(q_j - q_i + 1, 0), dtype=np.float64)
+        _check(self._lib.bk_eval_range(self._ctx, ctypes.byref(desc), r_i, r_j, q_i, q_j, _dptr(out), 0))
+        return out
+
+    def eval(self, desc: KernelDesc, r_i: int, r_j: int, q_i: int, q_j: int) -> np.ndarray:
+        self._check_indices(r_i, r_j, q_i, q_j)
+        out = np.empty(max(q_j - q_i + 1, 0), dtype=np.float64)
+        _check(self._lib.bk_eval_range(self._ctx, ctypes.byref(desc), r_i, r_j, q_i, q_j, _dptr(out), 1))
+        return out
+
+    def eval_device(self, desc: KernelDesc, r_i: int, r_j: int, q_i: int, q_j: int, out_dev_ptr: int,
+                    normalize: bool = True) -> None:
+        """Queue an evaluation writing doubles to device memory (no sync)."""
+        _check(self._lib.bk_eval_range_device(self._ctx, ctypes.byref(desc), r_i, r_j, q_i, q_j,
+                                              ctypes.c_void_p(int(out_dev_ptr)), int(bool(normalize))))
+
+    def eval_leaf_pairs(self, desc: KernelDesc, pairs: Sequence[Sequence[int]], normalize: bool = False) -> list:
+        """Evaluate many (r_i, r_j, q_i, q_j) pairs in one launch; one array per pair."""
+        arr = (RangePair * len(pairs))(*[RangePair(*map(int, p)) for p in pairs])
+        sizes = [int(p[3]) - int(p[2]) + 1 for p in pairs]
+        out = np.empty(max(sum(sizes), 0), dtype=np.float64)
+        _check(self._lib.bk_eval_leaf_pairs(self._ctx, ctypes.byref(desc), arr, len(pairs), _dptr(out),
+                                            int(bool(normalize))))
+        res, off = [], 0
+        for s in sizes:
+            res.append(out[off:off + s])
+            off += s
+        return res
+
+    def self_eval(self, desc: KernelDesc, exclude_self: bool = False, normalize: bool = True) -> np.ndarray:
+        out = np.empty(self.reference_size(), dtype=np.float64)
+        _check(self._lib.bk_self_eval(self._ctx, ctypes.byref(desc), int(bool(exclude_self)), _dptr(out),
+                                      int(bool(normalize))))
+        return out
+
+    def likelihood_cv(self, desc: KernelDesc, weights) -> float:
+        w = _opt_vec(weights, self.reference_size())
+        s = ctypes.c_double()
+        _check(self._lib.bk_likelihood_cv(self._ctx, ctypes.byref(desc), _dptr(w), ctypes.byref(s)))
+        return s.value
+
+    def lsq_conv_cv(self, desc: KernelDesc, weights) -> float:
+        w = _opt_vec(weights, self.reference_size())
+        s = ctypes.c_double()
+        _check(self._lib.bk_lsq_conv_cv(self._ctx, ctypes.byref(desc), _dptr(w), ctypes.byref(s)))
+        return s.value
+
+    def adapt(self, desc: KernelDesc, alpha: float, weights):
+        """Adaptive pass; returns (g, lambda) and leaves the refs adapted."""
+        n = self.reference_size()
+        w = _opt_vec(weights, n)
+        lam = np.empty(n, dtype=np.float64)
+        g = ctypes.c_double()
+        _check(self._lib.bk_adapt(self._ctx, ctypes.byref(desc), float(alpha), _dptr(w), ctypes.byref(g), _dptr(lam)))
+        return g.value, lam
+
+    @staticmethod
+    def _check_indices(*idx: int) -> None:
+        for v in idx:
+            if int(v) < 0:
+                raise IndexError("indices must be non-negative")

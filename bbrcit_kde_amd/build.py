@@ -1,0 +1,81 @@
+"""Build the native parts of bbrcit_kde_amd in-tree.
+
+* ``lib/libbbrcit_hip.so``: the HIP kernels and the C ABI declared in
+  ``include/bbrcit_hip.h``, compiled by hipcc for gfx950 only.
+
+Run as ``python -m bbrcit_kde_amd.build`` or call :func:`build_library`.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+REPO_DIR = PKG_DIR.parent
+CSRC = PKG_DIR / "csrc"
+LIB_DIR = PKG_DIR / "lib"
+LIB_PATH = LIB_DIR / "libbbrcit_hip.so"
+
+HIP_SOURCES = ["bk_kernels.hip", "bk_capi.hip"]
+
+# -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(exe).exists():
+        raise RuntimeError("hipcc not found; the HIP library cannot be built")
+    return exe
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        sys.stderr.write(proc.stderr)
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
+
+
+def build_library(verbose: bool = False) -> Path:
+    """Compile every HIP source and link libbbrcit_hip.so (always rebuilds)."""
+    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    build_dir = PKG_DIR / "_build"
+    build_dir.mkdir(exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = build_dir / (Path(src).stem + ".o")
+        cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(REPO_DIR / "include"), "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        objs.append(str(obj))
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def main() -> int:
+    path = build_library(verbose=True)
+    print(f"built {path}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

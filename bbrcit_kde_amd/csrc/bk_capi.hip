@@ -1,0 +1,846 @@
+// C-ABI of the direct kernel-density library (declared in include/bbrcit_hip.h).
+//
+// A context owns the device copies of the reference and query sets, the
+// cached per-reference features of the last kernel descriptor, and staging
+// buffers that grow on demand and are reused across calls.  Every call
+// validates its arguments the way CudaDirectKde::unnormalized_eval does
+// (include/CudaDirectKde.h:181-190), checks every HIP return code, and turns
+// failures into status codes plus a thread-local message.
+#include "../../include/bbrcit_hip.h"
+#include "bk_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct BkError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw BkError{code, msg}; }
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    fail(BK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return BK_OK;
+  } catch (const BkError &e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return BK_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return BK_ERR_STATE;
+  } catch (...) {
+    g_last_error = "unknown failure";
+    return BK_ERR_STATE;
+  }
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t need) {
+    if (need <= bytes) return;
+    if (ptr) {
+      hip_check(hipFree(ptr), "hipFree");
+      ptr = nullptr;
+      bytes = 0;
+    }
+    if (hipMalloc(&ptr, need) != hipSuccess) {
+      ptr = nullptr;
+      fail(BK_ERR_ALLOC, "hipMalloc of " + std::to_string(need) + " bytes failed");
+    }
+    bytes = need;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+struct PointSet {
+  size_t n = 0;
+  int dim = 0;
+  bool uniform_abw = true;  // every abw == 1
+  double m_max = 0.0;       // largest mass (as stored on the device)
+  // FP32 storage: x, y (D <= 2; y == 0 for D == 1) or row-major coords (D > 2)
+  DevBuf x, y, coords, m, abw;
+  // FP64 storage (D <= 2)
+  DevBuf xd, yd, md, abwd;
+  unsigned long long version = 0;
+  void release() {
+    for (DevBuf *b : {&x, &y, &coords, &m, &abw, &xd, &yd, &md, &abwd}) b->release();
+    n = 0;
+    dim = 0;
+  }
+};
+
+}  // namespace
+
+struct bk_ctx {
+  int device = 0;
+  int precision = BK_PREC_F32;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  PointSet refs, queries;
+  bool alias = false;
+  unsigned long long version_counter = 0;
+
+  // feature cache
+  bool feat_valid = false;
+  bk_kernel_desc feat_desc{};
+  unsigned long long feat_version = 0;
+  DevBuf f0, f1;
+
+  // staging
+  DevBuf partial, out, items, work_a, work_b, weights, scratch, scalar;
+  std::vector<double> host_tmp;
+};
+
+namespace {
+
+using bk::EvalParams;
+using bk::WorkItem;
+
+constexpr double kLog2e = 1.4426950408889634074;
+constexpr double kPi = 3.14159265358979323846;
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    hip_check(hipGetDevice(&prev), "hipGetDevice");
+    if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+void require_ctx(const bk_ctx *ctx) {
+  if (!ctx) fail(BK_ERR_INVALID_ARG, "null context");
+}
+
+// Kernel descriptor -> device family and constants.  The constants restate
+// the reference's KernelTraits (include/Kernels/KernelTraits.h): the
+// normalisations C(h) and the exponent constants of the Gaussian kinds,
+// including GaussianProductConv2dTraits<float|double>, whose kernel is
+// exp(-0.5 arg) (KernelTraits.h:432, 445).
+EvalParams params_of(const bk_kernel_desc *d) {
+  if (!d) fail(BK_ERR_INVALID_ARG, "null kernel descriptor");
+  EvalParams p{};
+  p.dim = d->dim;
+  p.mass_fold = 1.0;
+  const double hx = d->hx, hy = d->hy;
+  auto need_pos = [](double h, const char *name) {
+    if (!(h > 0.0) || !std::isfinite(h))
+      fail(BK_ERR_INVALID_ARG, std::string("bandwidth ") + name + " must be positive and finite");
+  };
+  auto need_dim = [&](int lo, int hi) {
+    if (d->dim < lo || d->dim > hi)
+      fail(BK_ERR_UNSUPPORTED, "kernel kind " + std::to_string(d->kind) +
+                                   " does not support dimension " + std::to_string(d->dim));
+  };
+  need_pos(hx, "hx");
+  switch (d->kind) {
+    case BK_GAUSSIAN: {
+      need_dim(1, 16);
+      p.family = bk::FAM_GAUSS;
+      p.c = 0.5;
+      p.gx = p.gy = p.c * kLog2e / (hx * hx);
+      p.iso = 1;
+      p.norm = std::pow(2.0 * kPi, -0.5 * d->dim) / std::pow(hx, d->dim);
+      break;
+    }
+    case BK_EPANECHNIKOV: {
+      need_dim(1, 16);
+      p.family = bk::FAM_EPAN;
+      p.gx = p.gy = 1.0 / (hx * hx);
+      p.iso = 1;
+      const double D = d->dim;
+      p.norm = 0.5 * (D + 2.0) / (std::pow(kPi, 0.5 * D) / std::tgamma(1.0 + 0.5 * D)) /
+               std::pow(hx, D);
+      break;
+    }
+    case BK_GAUSSIAN_PRODUCT_2D:
+    case BK_GAUSSIAN_PRODUCT_CONV_2D: {
+      need_dim(2, 2);
+      need_pos(hy, "hy");
+      p.family = bk::FAM_GAUSS;
+      p.c = 0.5;
+      p.gx = p.c * kLog2e / (hx * hx);
+      p.gy = p.c * kLog2e / (hy * hy);
+      p.iso = hx == hy;
+      p.norm = d->kind == BK_GAUSSIAN_PRODUCT_2D ? 1.0 / (2.0 * kPi * hx * hy)
+                                                 : 1.0 / (4.0 * kPi * hx * hy);
+      break;
+    }
+    case BK_GAUSSIAN_CONV_1D: {
+      need_dim(1, 1);
+      p.family = bk::FAM_GAUSS;
+      p.c = 0.25;
+      p.gx = p.gy = p.c * kLog2e / (hx * hx);
+      p.iso = 1;
+      p.norm = 1.0 / (std::sqrt(4.0 * kPi) * hx);
+      break;
+    }
+    case BK_EPANECHNIKOV_PRODUCT_2D: {
+      need_dim(2, 2);
+      need_pos(hy, "hy");
+      p.family = bk::FAM_EPANPROD;
+      p.gx = 1.0 / (hx * hx);
+      p.gy = 1.0 / (hy * hy);
+      p.iso = 0;
+      p.norm = 0.5625 / (hx * hy);
+      break;
+    }
+    case BK_EPANECHNIKOV_CONV_1D: {
+      need_dim(1, 1);
+      p.family = bk::FAM_EPANCONV;
+      p.ix = 1.0 / hx;
+      p.iy = 1.0;  // y == 0 for every 1-D point: the y factor is k(0) = 1
+      p.gx = p.gy = 1.0;
+      p.mass_fold = 1.0 / 1024.0;
+      p.norm = 0.6 / hx;
+      break;
+    }
+    case BK_EPANECHNIKOV_PRODUCT_CONV_2D: {
+      need_dim(2, 2);
+      need_pos(hy, "hy");
+      p.family = bk::FAM_EPANCONV;
+      p.ix = 1.0 / hx;
+      p.iy = 1.0 / hy;
+      p.gx = p.gy = 1.0;
+      p.mass_fold = 1.0 / 1024.0;
+      p.norm = 0.36 / (hx * hy);
+      break;
+    }
+    default:
+      fail(BK_ERR_UNSUPPORTED, "unknown kernel kind " + std::to_string(d->kind));
+  }
+  return p;
+}
+
+PointSet &set_of(bk_ctx *ctx, int set) {
+  if (set == BK_SET_REF) return ctx->refs;
+  if (set == BK_SET_QUERY) {
+    if (ctx->alias) fail(BK_ERR_STATE, "the query set is an alias of the reference set");
+    return ctx->queries;
+  }
+  fail(BK_ERR_INVALID_ARG, "set must be BK_SET_REF or BK_SET_QUERY");
+}
+
+const PointSet &query_set(const bk_ctx *ctx) { return ctx->alias ? ctx->refs : ctx->queries; }
+
+// Copy host data into a point set; masses/abw default to 1.
+void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coords,
+                const double *mass, const double *abw) {
+  if (dim < 1 || dim > 16) fail(BK_ERR_UNSUPPORTED, "dimension must be between 1 and 16");
+  if (n > 0 && !coords) fail(BK_ERR_INVALID_ARG, "null coordinate array");
+  if (ctx->precision == BK_PREC_F64 && dim > 2)
+    fail(BK_ERR_UNSUPPORTED, "FP64 contexts support dimensions 1 and 2");
+  ps.dim = dim;
+  ps.n = n;
+  ps.uniform_abw = true;
+  ps.m_max = 0.0;
+  ps.version = ++ctx->version_counter;
+  if (n == 0) return;
+  const hipStream_t s = ctx->stream;
+  if (ctx->precision == BK_PREC_F32) {
+    std::vector<float> m(n), a(n);
+    for (size_t i = 0; i < n; ++i) {
+      m[i] = mass ? (float)mass[i] : 1.0f;
+      a[i] = abw ? (float)abw[i] : 1.0f;
+      if (a[i] != 1.0f) ps.uniform_abw = false;
+      if (!(a[i] > 0.0f) || !std::isfinite(a[i]))
+        fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
+      if (!(m[i] >= 0.0f) || !std::isfinite(m[i]))
+        fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
+      ps.m_max = std::max(ps.m_max, (double)m[i]);
+    }
+    ps.m.reserve(n * sizeof(float));
+    ps.abw.reserve(n * sizeof(float));
+    hip_check(hipMemcpyAsync(ps.m.ptr, m.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload masses");
+    hip_check(hipMemcpyAsync(ps.abw.ptr, a.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload abw");
+    if (dim <= 2) {
+      std::vector<float> x(n), y(n, 0.0f);
+      for (size_t i = 0; i < n; ++i) {
+        x[i] = (float)coords[i * dim];
+        if (dim == 2) y[i] = (float)coords[i * dim + 1];
+      }
+      ps.x.reserve(n * sizeof(float));
+      ps.y.reserve(n * sizeof(float));
+      hip_check(hipMemcpyAsync(ps.x.ptr, x.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload x");
+      hip_check(hipMemcpyAsync(ps.y.ptr, y.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload y");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    } else {
+      std::vector<float> c(n * dim);
+      for (size_t i = 0; i < n * dim; ++i) c[i] = (float)coords[i];
+      ps.coords.reserve(n * dim * sizeof(float));
+      hip_check(hipMemcpyAsync(ps.coords.ptr, c.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s),
+                "upload coords");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+  } else {
+    std::vector<double> x(n), y(n, 0.0), m(n), a(n);
+    for (size_t i = 0; i < n; ++i) {
+      x[i] = coords[i * dim];
+      if (dim == 2) y[i] = coords[i * dim + 1];
+      m[i] = mass ? mass[i] : 1.0;
+      a[i] = abw ? abw[i] : 1.0;
+      if (a[i] != 1.0) ps.uniform_abw = false;
+      if (!(a[i] > 0.0) || !std::isfinite(a[i])) fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
+      if (!(m[i] >= 0.0) || !std::isfinite(m[i]))
+        fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
+      ps.m_max = std::max(ps.m_max, m[i]);
+    }
+    for (DevBuf *b : {&ps.xd, &ps.yd, &ps.md, &ps.abwd}) b->reserve(n * sizeof(double));
+    hip_check(hipMemcpyAsync(ps.xd.ptr, x.data(), n * 8, hipMemcpyHostToDevice, s), "upload x");
+    hip_check(hipMemcpyAsync(ps.yd.ptr, y.data(), n * 8, hipMemcpyHostToDevice, s), "upload y");
+    hip_check(hipMemcpyAsync(ps.md.ptr, m.data(), n * 8, hipMemcpyHostToDevice, s), "upload masses");
+    hip_check(hipMemcpyAsync(ps.abwd.ptr, a.data(), n * 8, hipMemcpyHostToDevice, s), "upload abw");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+}
+
+// Replace masses and/or abw of the reference set (same n, same dim).
+void update_refs(bk_ctx *ctx, const double *mass, const double *abw) {
+  PointSet &ps = ctx->refs;
+  const size_t n = ps.n;
+  if (n == 0) return;
+  const hipStream_t s = ctx->stream;
+  ps.version = ++ctx->version_counter;
+  if (ctx->precision == BK_PREC_F32) {
+    std::vector<float> buf(n);
+    if (mass) {
+      ps.m_max = 0.0;
+      for (size_t i = 0; i < n; ++i) {
+        buf[i] = (float)mass[i];
+        if (!(buf[i] >= 0.0f) || !std::isfinite(buf[i]))
+          fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
+        ps.m_max = std::max(ps.m_max, (double)buf[i]);
+      }
+      hip_check(hipMemcpyAsync(ps.m.ptr, buf.data(), n * 4, hipMemcpyHostToDevice, s), "update masses");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    if (abw) {
+      ps.uniform_abw = true;
+      for (size_t i = 0; i < n; ++i) {
+        buf[i] = (float)abw[i];
+        if (!(buf[i] > 0.0f) || !std::isfinite(buf[i])) fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
+        if (buf[i] != 1.0f) ps.uniform_abw = false;
+      }
+      hip_check(hipMemcpyAsync(ps.abw.ptr, buf.data(), n * 4, hipMemcpyHostToDevice, s), "update abw");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+  } else {
+    if (mass) {
+      ps.m_max = 0.0;
+      for (size_t i = 0; i < n; ++i) {
+        if (!(mass[i] >= 0.0) || !std::isfinite(mass[i]))
+          fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
+        ps.m_max = std::max(ps.m_max, mass[i]);
+      }
+      hip_check(hipMemcpyAsync(ps.md.ptr, mass, n * 8, hipMemcpyHostToDevice, s), "update masses");
+    }
+    if (abw) {
+      ps.uniform_abw = true;
+      for (size_t i = 0; i < n; ++i) {
+        if (!(abw[i] > 0.0) || !std::isfinite(abw[i])) fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
+        if (abw[i] != 1.0) ps.uniform_abw = false;
+      }
+      hip_check(hipMemcpyAsync(ps.abwd.ptr, abw, n * 8, hipMemcpyHostToDevice, s), "update abw");
+    }
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+}
+
+double mass_scale(const PointSet &ps) { return ps.m_max > 0.0 ? ps.m_max : 1.0; }
+
+bool same_desc(const bk_kernel_desc &a, const bk_kernel_desc &b) {
+  return a.kind == b.kind && a.dim == b.dim && a.hx == b.hx && a.hy == b.hy;
+}
+
+// Make sure the per-reference features of `desc` are on the device.
+void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &p) {
+  PointSet &r = ctx->refs;
+  if (ctx->feat_valid && same_desc(ctx->feat_desc, *desc) && ctx->feat_version == r.version) return;
+  ctx->feat_valid = false;
+  const size_t n = r.n;
+  if (n == 0) return;
+  const hipStream_t s = ctx->stream;
+  if (ctx->precision == BK_PREC_F64) {
+    ctx->f0.reserve(n * sizeof(double4));
+    ctx->f1.reserve(n * sizeof(double4));
+    hip_check(bk::launch_prep_f64(p, r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
+                                  r.abwd.as<double>(), n, ctx->f0.as<double4>(), ctx->f1.as<double4>(), s),
+              "prep_f64 launch");
+  } else if (r.dim <= 2) {
+    ctx->f0.reserve(n * sizeof(float4));
+    ctx->f1.reserve(n * sizeof(float4));
+    hip_check(bk::launch_prep_f32(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
+                                  mass_scale(r), ctx->f0.as<float4>(), ctx->f1.as<float4>(), s),
+              "prep_f32 launch");
+  } else {
+    ctx->f0.reserve(n * sizeof(float2));
+    hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, mass_scale(r),
+                                      ctx->f0.as<float2>(), s),
+              "prep_general launch");
+  }
+  ctx->feat_desc = *desc;
+  ctx->feat_version = r.version;
+  ctx->feat_valid = true;
+}
+
+void check_compatible(const bk_ctx *ctx, const EvalParams &p) {
+  const PointSet &q = query_set(ctx);
+  if (ctx->refs.n == 0 && q.n == 0) return;
+  if (ctx->refs.dim != p.dim || (q.n > 0 && q.dim != p.dim))
+    fail(BK_ERR_INVALID_ARG, "kernel dimension " + std::to_string(p.dim) +
+                                 " does not match the uploaded point dimension");
+  if (p.dim > 2 && p.family != bk::FAM_GAUSS && p.family != bk::FAM_EPAN)
+    fail(BK_ERR_UNSUPPORTED, "dimensions above 2 support the Gaussian and Epanechnikov kernels only");
+}
+
+void check_range(const bk_ctx *ctx, size_t r_i, size_t r_j, size_t q_i, size_t q_j) {
+  const PointSet &q = query_set(ctx);
+  if (r_j >= ctx->refs.n || q_j >= q.n)
+    fail(BK_ERR_OUT_OF_RANGE, "r_j and q_j must be less than the number of reference and query points");
+  if (r_i > r_j || q_i > q_j) fail(BK_ERR_INVALID_ARG, "must have r_i <= r_j and q_i <= q_j");
+}
+
+int choose_splits(long long nq, long long nr, int queries_per_block, int tile) {
+  const long long qblocks = (nq + queries_per_block - 1) / queries_per_block;
+  const long long target = 32LL * bk::device_cu_count();
+  long long splits = (target + qblocks - 1) / qblocks;
+  const long long max_splits = std::max(1LL, nr / (4LL * tile));
+  splits = std::max(1LL, std::min(splits, max_splits));
+  splits = std::min(splits, 65535LL);
+  return (int)splits;
+}
+
+// Direct sums over one range pair into device memory `out` (nq doubles).
+void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t r_j, size_t q_i,
+                    size_t q_j, double *out, int normalize, int self_excl) {
+  const EvalParams p = params_of(desc);
+  check_compatible(ctx, p);
+  check_range(ctx, r_i, r_j, q_i, q_j);
+  ensure_features(ctx, desc, p);
+  const hipStream_t s = ctx->stream;
+  const PointSet &r = ctx->refs;
+  const PointSet &q = query_set(ctx);
+  const long long nq = (long long)(q_j - q_i + 1);
+  const long long nr = (long long)(r_j - r_i + 1);
+  const long long r_end = (long long)r_j + 1;
+  double scale = normalize ? p.norm : 1.0;
+  int splits, qpb, tile;
+  if (ctx->precision == BK_PREC_F64) {
+    qpb = 256;
+    tile = 256;
+  } else if (p.dim <= 2) {
+    qpb = bk::kQueriesPerBlock;
+    tile = bk::kTile;
+    scale *= mass_scale(r);
+  } else {
+    qpb = 256;
+    tile = 128;
+    scale *= mass_scale(r);
+  }
+  splits = choose_splits(nq, nr, qpb, tile);
+  double *dst = out;
+  if (splits > 1) {
+    ctx->partial.reserve((size_t)splits * nq * sizeof(double));
+    dst = ctx->partial.as<double>();
+  }
+  hipError_t e;
+  if (ctx->precision == BK_PREC_F64) {
+    e = bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
+                              ctx->f1.as<double4>(), nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq,
+                              splits, dst, scale, self_excl, s);
+  } else if (p.dim <= 2) {
+    e = bk::launch_direct_f32_range(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(),
+                                    ctx->f1.as<float4>(), (long long)r_i, r_end, (long long)q_i, nq, splits,
+                                    dst, scale, self_excl, s);
+  } else {
+    e = bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<float2>(),
+                                  nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits, dst, scale,
+                                  self_excl, s);
+  }
+  hip_check(e, "direct kernel launch");
+  if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, s), "reduce launch");
+}
+
+void copy_to_host(bk_ctx *ctx, double *host, const double *dev, size_t n) {
+  if (n == 0) return;
+  hip_check(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "copy results");
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+double *staging_out(bk_ctx *ctx, size_t n) {
+  ctx->out.reserve(std::max<size_t>(n, 1) * sizeof(double));
+  return ctx->out.as<double>();
+}
+
+void require_alias(const bk_ctx *ctx) {
+  if (!ctx->alias) fail(BK_ERR_STATE, "this call needs bk_alias_queries_to_refs()");
+  if (ctx->refs.n == 0) fail(BK_ERR_STATE, "no reference points uploaded");
+}
+
+// Self evaluation into device buffer `dst` (n_ref doubles).
+void self_eval_dev(bk_ctx *ctx, const bk_kernel_desc *desc, int exclude_self, double *dst, int normalize) {
+  require_alias(ctx);
+  eval_range_dev(ctx, desc, 0, ctx->refs.n - 1, 0, ctx->refs.n - 1, dst, normalize, exclude_self ? 1 : 0);
+}
+
+double *upload_weights(bk_ctx *ctx, const double *w) {
+  if (!w) fail(BK_ERR_INVALID_ARG, "null weight array");
+  const size_t n = ctx->refs.n;
+  ctx->weights.reserve(n * sizeof(double));
+  hip_check(hipMemcpyAsync(ctx->weights.ptr, w, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+            "upload weights");
+  return ctx->weights.as<double>();
+}
+
+double reduce_to_host(bk_ctx *ctx, int mode, const double *w, const double *u, const double *v, double a,
+                      double b) {
+  ctx->scratch.reserve(bk::kReduceBlocks * sizeof(double));
+  ctx->scalar.reserve(sizeof(double));
+  hip_check(bk::launch_weighted_reduce(mode, w, u, v, a, b, (long long)ctx->refs.n, ctx->scratch.as<double>(),
+                                       ctx->scalar.as<double>(), ctx->stream),
+            "reduction launch");
+  double r = 0.0;
+  copy_to_host(ctx, &r, ctx->scalar.as<double>(), 1);
+  return r;
+}
+
+}  // namespace
+
+// ---- C ABI -----------------------------------------------------------------
+
+extern "C" {
+
+const char *bk_last_error(void) { return g_last_error.c_str(); }
+
+int bk_create(bk_ctx **ctx_out, int device, int precision) {
+  return guarded([&] {
+    if (!ctx_out) fail(BK_ERR_INVALID_ARG, "null output pointer");
+    *ctx_out = nullptr;
+    if (precision != BK_PREC_F32 && precision != BK_PREC_F64)
+      fail(BK_ERR_INVALID_ARG, "precision must be BK_PREC_F32 or BK_PREC_F64");
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count)
+      fail(BK_ERR_INVALID_ARG, "device " + std::to_string(device) + " not present (" +
+                                   std::to_string(count) + " devices)");
+    DeviceGuard g(device);
+    bk_ctx *c = new bk_ctx();
+    c->device = device;
+    c->precision = precision;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      fail(BK_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    c->stream = c->own_stream;
+    *ctx_out = c;
+  });
+}
+
+int bk_destroy(bk_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    {
+      DeviceGuard g(ctx->device);
+      (void)hipStreamSynchronize(ctx->stream);
+      ctx->refs.release();
+      ctx->queries.release();
+      for (DevBuf *b : {&ctx->f0, &ctx->f1, &ctx->partial, &ctx->out, &ctx->items, &ctx->work_a, &ctx->work_b,
+                        &ctx->weights, &ctx->scratch, &ctx->scalar})
+        b->release();
+      if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    }
+    delete ctx;
+  });
+}
+
+int bk_set_stream(bk_ctx *ctx, void *stream) {
+  return guarded([&] {
+    require_ctx(ctx);
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+  });
+}
+
+int bk_upload(bk_ctx *ctx, int set, int dim, size_t n, const double *coords, const double *mass,
+              const double *abw) {
+  return guarded([&] {
+    require_ctx(ctx);
+    DeviceGuard g(ctx->device);
+    if (set == BK_SET_REF && ctx->alias) {
+      // re-uploading the references keeps the alias
+    }
+    PointSet &ps = set_of(ctx, set);
+    upload_set(ctx, ps, dim, n, coords, mass, abw);
+    if (set == BK_SET_REF) ctx->feat_valid = false;
+  });
+}
+
+int bk_alias_queries_to_refs(bk_ctx *ctx) {
+  return guarded([&] {
+    require_ctx(ctx);
+    DeviceGuard g(ctx->device);
+    ctx->queries.release();
+    ctx->alias = true;
+  });
+}
+
+int bk_size(const bk_ctx *ctx, int set, size_t *n_out) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!n_out) fail(BK_ERR_INVALID_ARG, "null output pointer");
+    if (set == BK_SET_REF)
+      *n_out = ctx->refs.n;
+    else if (set == BK_SET_QUERY)
+      *n_out = query_set(ctx).n;
+    else
+      fail(BK_ERR_INVALID_ARG, "set must be BK_SET_REF or BK_SET_QUERY");
+  });
+}
+
+int bk_update_refs(bk_ctx *ctx, const double *mass, const double *abw) {
+  return guarded([&] {
+    require_ctx(ctx);
+    DeviceGuard g(ctx->device);
+    update_refs(ctx, mass, abw);
+  });
+}
+
+int bk_eval_range(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t r_j, size_t q_i, size_t q_j,
+                  double *out_host, int normalize) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!out_host) fail(BK_ERR_INVALID_ARG, "null output array");
+    DeviceGuard g(ctx->device);
+    check_range(ctx, r_i, r_j, q_i, q_j);
+    const size_t nq = q_j - q_i + 1;
+    double *dev = staging_out(ctx, nq);
+    eval_range_dev(ctx, desc, r_i, r_j, q_i, q_j, dev, normalize, 0);
+    copy_to_host(ctx, out_host, dev, nq);
+  });
+}
+
+int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t r_j, size_t q_i,
+                         size_t q_j, double *out_dev, int normalize) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!out_dev) fail(BK_ERR_INVALID_ARG, "null output array");
+    DeviceGuard g(ctx->device);
+    eval_range_dev(ctx, desc, r_i, r_j, q_i, q_j, out_dev, normalize, 0);
+  });
+}
+
+int bk_eval_leaf_pairs(bk_ctx *ctx, const bk_kernel_desc *desc, const bk_range_pair *pairs, size_t n_pairs,
+                       double *out_host, int normalize) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (n_pairs == 0) return;
+    if (!pairs || !out_host) fail(BK_ERR_INVALID_ARG, "null pair or output array");
+    DeviceGuard g(ctx->device);
+    const EvalParams p = params_of(desc);
+    check_compatible(ctx, p);
+    const int qpb = (ctx->precision == BK_PREC_F32 && p.dim <= 2) ? bk::kQueriesPerBlock : 256;
+    std::vector<WorkItem> items;
+    long long total = 0;
+    for (size_t k = 0; k < n_pairs; ++k) {
+      const bk_range_pair &pr = pairs[k];
+      check_range(ctx, pr.r_i, pr.r_j, pr.q_i, pr.q_j);
+      const long long nq = (long long)(pr.q_j - pr.q_i + 1);
+      for (long long qb = 0; qb < nq; qb += qpb) {
+        WorkItem it{};
+        it.r_begin = (long long)pr.r_i;
+        it.r_end = (long long)pr.r_j + 1;
+        it.q_begin = (long long)pr.q_i + qb;
+        it.q_count = (int)std::min<long long>(qpb, nq - qb);
+        it.out_index = total + qb;
+        items.push_back(it);
+      }
+      total += nq;
+    }
+    if (items.size() > 0x7fffffffULL) fail(BK_ERR_INVALID_ARG, "too many leaf pairs in one call");
+    ensure_features(ctx, desc, p);
+    const hipStream_t s = ctx->stream;
+    ctx->items.reserve(items.size() * sizeof(WorkItem));
+    hip_check(hipMemcpyAsync(ctx->items.ptr, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                             s),
+              "upload work items");
+    double *dev = staging_out(ctx, (size_t)total);
+    double scale = normalize ? p.norm : 1.0;
+    const PointSet &r = ctx->refs;
+    const PointSet &q = query_set(ctx);
+    hipError_t e;
+    if (ctx->precision == BK_PREC_F64) {
+      e = bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
+                                ctx->f1.as<double4>(), ctx->items.as<WorkItem>(), (int)items.size(), 0, 0, 0, 0,
+                                1, dev, scale, 0, s);
+    } else if (p.dim <= 2) {
+      scale *= mass_scale(r);
+      e = bk::launch_direct_f32_list(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(),
+                                     ctx->f1.as<float4>(), ctx->items.as<WorkItem>(), (int)items.size(), dev, scale,
+                                     0, s);
+    } else {
+      scale *= mass_scale(r);
+      e = bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<float2>(),
+                                    ctx->items.as<WorkItem>(), (int)items.size(), 0, 0, 0, 0, 1, dev, scale, 0, s);
+    }
+    hip_check(e, "leaf-pair kernel launch");
+    copy_to_host(ctx, out_host, dev, (size_t)total);
+  });
+}
+
+int bk_self_eval(bk_ctx *ctx, const bk_kernel_desc *desc, int exclude_self, double *out_host, int normalize) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!out_host) fail(BK_ERR_INVALID_ARG, "null output array");
+    DeviceGuard g(ctx->device);
+    require_alias(ctx);
+    const size_t n = ctx->refs.n;
+    double *dev = staging_out(ctx, n);
+    self_eval_dev(ctx, desc, exclude_self, dev, normalize);
+    copy_to_host(ctx, out_host, dev, n);
+  });
+}
+
+int bk_likelihood_cv(bk_ctx *ctx, const bk_kernel_desc *desc, const double *weights, double *score_out) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!score_out) fail(BK_ERR_INVALID_ARG, "null output pointer");
+    DeviceGuard g(ctx->device);
+    require_alias(ctx);
+    const size_t n = ctx->refs.n;
+    const EvalParams p = params_of(desc);
+    double *w = upload_weights(ctx, weights);
+    ctx->work_a.reserve(n * sizeof(double));
+    double *S = ctx->work_a.as<double>();
+    self_eval_dev(ctx, desc, 1, S, 0);
+    *score_out = reduce_to_host(ctx, 0, w, S, nullptr, p.norm, 0.0);
+  });
+}
+
+namespace {
+// Convolution partner of a kernel kind (ConvKernelAssociator,
+// include/Kernels/ConvKernelAssociator.h:17-84).
+bk_kernel_desc conv_partner(const bk_kernel_desc *d) {
+  bk_kernel_desc c = *d;
+  switch (d->kind) {
+    case BK_EPANECHNIKOV:
+      if (d->dim != 1) fail(BK_ERR_UNSUPPORTED, "no convolution kernel for EpanechnikovKernel with D != 1");
+      c.kind = BK_EPANECHNIKOV_CONV_1D;
+      break;
+    case BK_GAUSSIAN:
+      if (d->dim != 1) fail(BK_ERR_UNSUPPORTED, "no convolution kernel for GaussianKernel with D != 1");
+      c.kind = BK_GAUSSIAN_CONV_1D;
+      break;
+    case BK_EPANECHNIKOV_PRODUCT_2D:
+      c.kind = BK_EPANECHNIKOV_PRODUCT_CONV_2D;
+      break;
+    case BK_GAUSSIAN_PRODUCT_2D:
+      c.kind = BK_GAUSSIAN_PRODUCT_CONV_2D;
+      break;
+    default:
+      fail(BK_ERR_UNSUPPORTED, "kernel kind has no convolution kernel");
+  }
+  return c;
+}
+}  // namespace
+
+int bk_lsq_conv_cv(bk_ctx *ctx, const bk_kernel_desc *desc, const double *weights, double *score_out) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!score_out) fail(BK_ERR_INVALID_ARG, "null output pointer");
+    DeviceGuard g(ctx->device);
+    require_alias(ctx);
+    const size_t n = ctx->refs.n;
+    const bk_kernel_desc cdesc = conv_partner(desc);
+    const EvalParams p = params_of(desc);
+    const EvalParams pc = params_of(&cdesc);
+    double *w = upload_weights(ctx, weights);
+    ctx->work_a.reserve(n * sizeof(double));
+    ctx->work_b.reserve(n * sizeof(double));
+    double *S = ctx->work_a.as<double>();
+    double *Sc = ctx->work_b.as<double>();
+    self_eval_dev(ctx, desc, 1, S, 0);
+    self_eval_dev(ctx, &cdesc, 0, Sc, 0);
+    *score_out = reduce_to_host(ctx, 1, w, Sc, S, pc.norm, 2.0 * p.norm);
+  });
+}
+
+int bk_adapt(bk_ctx *ctx, const bk_kernel_desc *desc, double alpha, const double *weights, double *g_out,
+             double *lambda_out) {
+  return guarded([&] {
+    require_ctx(ctx);
+    DeviceGuard g(ctx->device);
+    require_alias(ctx);
+    if (!(alpha >= 0.0 && alpha <= 1.0)) fail(BK_ERR_INVALID_ARG, "adapt_density: alpha must be between [0, 1]");
+    if (!weights) fail(BK_ERR_INVALID_ARG, "null weight array");
+    const size_t n = ctx->refs.n;
+    const EvalParams p = params_of(desc);
+    check_compatible(ctx, p);
+    // non-adaptive state: mass = weight, abw = 1
+    std::vector<double> ones(n, 1.0);
+    update_refs(ctx, weights, ones.data());
+    if (alpha == 0.0) {
+      if (lambda_out) std::fill(lambda_out, lambda_out + n, 1.0);
+      if (g_out) *g_out = std::nan("");
+      return;
+    }
+    double *w = upload_weights(ctx, weights);
+    ctx->work_a.reserve(n * sizeof(double));
+    ctx->work_b.reserve(n * sizeof(double));
+    double *S = ctx->work_a.as<double>();
+    double *lam = ctx->work_b.as<double>();
+    self_eval_dev(ctx, desc, 0, S, 0);
+    ctx->scratch.reserve(bk::kReduceBlocks * sizeof(double));
+    ctx->scalar.reserve(sizeof(double));
+    hip_check(bk::launch_weighted_reduce(0, w, S, nullptr, p.norm, 0.0, (long long)n, ctx->scratch.as<double>(),
+                                         ctx->scalar.as<double>(), ctx->stream),
+              "reduction launch");
+    hip_check(bk::launch_adapt_lambda(S, (long long)n, p.norm, ctx->scalar.as<double>(), alpha, lam, ctx->stream),
+              "lambda launch");
+    std::vector<double> lambda(n);
+    double log_g = 0.0;
+    copy_to_host(ctx, lambda.data(), lam, n);
+    copy_to_host(ctx, &log_g, ctx->scalar.as<double>(), 1);
+    std::vector<double> mass(n);
+    for (size_t i = 0; i < n; ++i) mass[i] = weights[i] * std::pow(lambda[i], -(double)p.dim);
+    update_refs(ctx, mass.data(), lambda.data());
+    if (lambda_out) std::copy(lambda.begin(), lambda.end(), lambda_out);
+    if (g_out) *g_out = std::exp(log_g);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Internal declarations shared by the HIP kernels (bk_kernels.hip) and the
+// C-ABI implementation (bk_capi.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace bk {
+
+// Device evaluation families.  Every reference kernel class maps onto one
+// of them (see family_of() in bk_capi.hip):
+//   FAM_GAUSS     m * exp(-c * sum_d (dx_d / (h_d a))^2)       Gaussian kinds
+//   FAM_EPAN      m * max(1 - sum_d (dx_d / (h a))^2, 0)       EpanechnikovKernel<D>
+//   FAM_EPANPROD  m * prod_d max(1 - (dx_d / (h_d a))^2, 0)    EpanechnikovProductKernel2d
+//   FAM_EPANCONV  m * prod_d k(dx_d / (h_d a)),                 Epanechnikov convolution kinds
+//                 k(u) = (2-|u|)^3 (u^2 + 6|u| + 4) / 32 for |u| < 2
+enum Family : int { FAM_GAUSS = 0, FAM_EPAN = 1, FAM_EPANPROD = 2, FAM_EPANCONV = 3 };
+
+// Host-derived constants of one kernel descriptor.
+struct EvalParams {
+  int family;
+  int dim;          // point dimension
+  int iso;          // 1 if the quadratic form weights both axes equally
+  double gx, gy;    // FAM_GAUSS: c*log2(e)/h_d^2; FAM_EPAN/EPANPROD: 1/h_d^2
+  double ix, iy;    // FAM_EPANCONV: 1/h_d
+  double c;         // FAM_GAUSS: exponent constant (0.5 or 0.25)
+  double mass_fold; // constant folded into every mass (FAM_EPANCONV: 2^-5 per axis)
+  double norm;      // normalisation C(h), in double
+};
+
+// One unit of work of the direct kernels: the references [r_begin, r_end)
+// against the queries [q_begin, q_begin + q_count), written to
+// dst[out_index + k] for the k-th query.
+struct WorkItem {
+  long long r_begin, r_end;
+  long long q_begin;
+  long long out_index;
+  int q_count;
+  int pad;
+};
+
+// Geometry of the FP32 direct kernel.
+constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
+constexpr int kQpt = 4;            // queries held in registers per thread
+constexpr int kQueriesPerBlock = kBlock * kQpt;
+constexpr int kTile = 256;         // references per LDS stage
+
+// FP32 path, D <= 2 ------------------------------------------------------
+// Per-reference features for the FP32 path.  f1 is used by the families
+// that need more than four values per reference.
+hipError_t launch_prep_f32(const EvalParams &p, const float *x, const float *y,
+                           const float *m, const float *abw, size_t n,
+                           double m_scale, float4 *f0, float4 *f1,
+                           hipStream_t s);
+
+// Range mode: grid (query blocks) x (reference splits).  dst holds
+// `splits` consecutive arrays of nq partial sums (one array if splits == 1).
+hipError_t launch_direct_f32_range(const EvalParams &p, const float *qx,
+                                   const float *qy, const float4 *f0,
+                                   const float4 *f1, long long r_i,
+                                   long long r_end, long long q_i,
+                                   long long nq, int splits, double *dst,
+                                   double out_scale, int self_excl,
+                                   hipStream_t s);
+
+// List mode: one workgroup per work item.
+hipError_t launch_direct_f32_list(const EvalParams &p, const float *qx,
+                                  const float *qy, const float4 *f0,
+                                  const float4 *f1, const WorkItem *items,
+                                  int n_items, double *dst, double out_scale,
+                                  int self_excl, hipStream_t s);
+
+// FP64 path, D <= 2 ------------------------------------------------------
+hipError_t launch_prep_f64(const EvalParams &p, const double *x,
+                           const double *y, const double *m,
+                           const double *abw, size_t n, double4 *f0,
+                           double4 *f1, hipStream_t s);
+hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
+                             const double *qy, const double4 *f0,
+                             const double4 *f1, const WorkItem *items,
+                             int n_items, long long r_i, long long r_end,
+                             long long q_i, long long nq, int splits,
+                             double *dst, double out_scale, int self_excl,
+                             hipStream_t s);
+
+// FP32 path, 3 <= D <= 16 (FAM_GAUSS and FAM_EPAN) ------------------------
+// Features: row-major coordinates (n x D) plus (w, mass term) per point.
+hipError_t launch_prep_general(const EvalParams &p, const float *abw,
+                               const float *m, size_t n, double m_scale,
+                               float2 *wm, hipStream_t s);
+hipError_t launch_direct_general(const EvalParams &p, const float *qc,
+                                 const float *rc, const float2 *wm,
+                                 const WorkItem *items, int n_items,
+                                 long long r_i, long long r_end, long long q_i,
+                                 long long nq, int splits, double *dst,
+                                 double out_scale, int self_excl,
+                                 hipStream_t s);
+
+// Reductions -----------------------------------------------------------
+// out[q] = sum_s partial[s * nq + q], fixed order.
+hipError_t launch_reduce_splits(const double *partial, int splits,
+                                long long nq, double *out, hipStream_t s);
+
+// Deterministic FP64 reductions into *result (device):
+//   mode 0: sum_i w_i * log(a * u_i)
+//   mode 1: sum_i w_i * (a * u_i - b * v_i)
+// scratch must hold kReduceBlocks doubles.
+constexpr int kReduceBlocks = 256;
+hipError_t launch_weighted_reduce(int mode, const double *w, const double *u,
+                                  const double *v, double a, double b,
+                                  long long n, double *scratch,
+                                  double *result, hipStream_t s);
+
+// lambda_i = (norm * S_i / g)^(-alpha), with g read from device memory
+// (g_dev holds log(g) on entry: the mode-0 reduction result).
+hipError_t launch_adapt_lambda(const double *S, long long n, double norm,
+                               const double *log_g_dev, double alpha,
+                               double *lambda, hipStream_t s);
+
+// Number of compute units of the current device (cached per device).
+int device_cu_count();
+
+}  // namespace bk

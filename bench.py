@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the direct 2-D kernel-density sum on one MI355X.
+
+Workload (the headline configuration): 1,000,000 reference points from a
+seeded 3-component 2-D Gaussian mixture (weights uniform(0.5, 1.5),
+normalised), 1,000,000 query points uniform on [-4, 4]^2, GaussianKernel
+h = 0.05, FP32 pair terms with block-compensated FP64 accumulation.  The
+references are in kd-tree order, the order KernelDensity hands its data
+points to the device.  One step = one full 1M x 1M evaluation
+(bk_eval_range_device) with points and output resident in HBM.  The
+Epanechnikov product kernel (hx = 0.08, hy = 0.12) of the same
+configuration is timed after it and reported beside it.
+
+Prints one JSON line (see the keys below).  --gpus above 1 is refused: this
+build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N_REF = 1_000_000
+N_QUERY = 1_000_000
+H_GAUSS = 0.05
+H_EPAN = (0.08, 0.12)
+FLOP_PER_PAIR_GAUSS = 9     # dx, dy, dx^2, fma dy^2, scale, exp, fma (exp counted as 1)
+FLOP_PER_PAIR_EPANPROD = 12
+PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 peak (vector and matrix)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
+    ap.add_argument("--kernels", default="gauss,epanprod", help="comma list: gauss, epanprod")
+    return ap.parse_args()
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload():
+    from bbrcit_kde_amd import datasets as ds
+
+    pts, w = ds.gaussian_mixture(N_REF, seed=ds.SEED_MIXTURE_1M)
+    perm = ds.kd_order(pts, 64)
+    pts, w = pts[perm], w[perm]
+    mass = w / w.sum()
+    q = ds.uniform_box(N_QUERY, -4.0, 4.0, ds.SEED_MIXTURE_1M + 1)
+    return pts, mass, q
+
+
+def time_kernel(kde, desc, steps: int, warmup: int):
+    import torch
+
+    out = torch.empty(N_QUERY, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    kde.set_stream(stream.cuda_stream)
+    for _ in range(warmup):
+        kde.eval_device(desc, 0, N_REF - 1, 0, N_QUERY - 1, out.data_ptr(), True)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        kde.eval_device(desc, 0, N_REF - 1, 0, N_QUERY - 1, out.data_ptr(), True)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return out.cpu().numpy(), ms, wall
+
+
+def cpu_baseline(pts, mass, q, gpu_vals, seconds: float):
+    """Time the reference's CPU direct_eval (or the FP64 restatement) on a
+    bounded sample of the same workload; check the GPU values there."""
+    from bbrcit_kde_amd import datasets as ds
+    from oracle.oracle import REF_LIB, Oracle, Reference, default_threads
+
+    P, M = ds.round_f32(pts), ds.round_f32(mass)
+    result = {}
+    # parity sample: 256 queries spread over the whole range
+    idx = np.linspace(0, N_QUERY - 1, 256).astype(np.int64)
+    orc = Oracle()
+    threads = default_threads()
+    t0 = time.perf_counter()
+    want = orc.direct(2, H_GAUSS, H_GAUSS, P, ds.round_f32(q[idx]), mass=M, normalize=True, threads=threads)
+    dt = time.perf_counter() - t0
+    floor = 1e-12 * float(np.abs(want).max())
+    above = np.abs(want) >= floor
+    rel = np.abs(gpu_vals[idx][above] - want[above]) / np.abs(want[above])
+    result["parity"] = {"oracle": "FP64 restatement", "queries": int(len(idx)), "max_rel_err": float(rel.max()),
+                        "floor": floor, "frac_below_floor": float(1.0 - above.mean()), "tolerance": 1e-5}
+    port_rate = len(idx) * N_REF / dt
+    # port on all host cores, sized to about `seconds`
+    nq = int(min(N_QUERY, max(256, port_rate * seconds / N_REF)))
+    t0 = time.perf_counter()
+    orc.direct(2, H_GAUSS, H_GAUSS, P, ds.round_f32(q[:nq]), mass=M, normalize=True, threads=threads)
+    dt = time.perf_counter() - t0
+    result["port"] = {"value": nq * N_REF / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+                      "sample": f"{nq} queries x {N_REF} refs (FP64 restatement, {threads} threads)"}
+    if REF_LIB.exists():
+        ref = Reference()
+        # the reference's own KernelDensity::direct_eval, FP64 Gaussian, one thread
+        nq_ref = 64
+        t0 = time.perf_counter()
+        ref.direct_eval(2, 0, H_GAUSS, H_GAUSS, pts, mass, q[:nq_ref], leaf_nmax=1024)
+        dt_small = time.perf_counter() - t0
+        rate_guess = nq_ref * N_REF / max(dt_small, 1e-9)
+        nq_ref = int(min(20000, max(64, rate_guess * seconds / N_REF)))
+        t0 = time.perf_counter()
+        ref.direct_eval(2, 0, H_GAUSS, H_GAUSS, pts, mass, q[:nq_ref], leaf_nmax=1024)
+        dt = time.perf_counter() - t0
+        result["reference"] = {"value": nq_ref * N_REF / dt, "unit": "pair-evals/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"{nq_ref} queries x {N_REF} refs, KernelDensity<2,GaussianKernel<2,double>>"
+                                         "::direct_eval (includes its kd-tree build), 1 thread"}
+    return result
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        return 2
+    import torch
+
+    import bbrcit_kde_amd as bk
+
+    log(f"host {platform.node()} cpus={os.cpu_count()}; generating 1M x 1M workload")
+    pts, mass, q = make_workload()
+    kinds = [k.strip() for k in args.kernels.split(",") if k.strip()]
+    results = {}
+    with bk.DirectKde(pts, q, mass) as kde:
+        for k in kinds:
+            if k == "gauss":
+                desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS
+            elif k == "epanprod":
+                desc, flop = bk.KernelDesc(bk.BK_EPANECHNIKOV_PRODUCT_2D, 2, *H_EPAN), FLOP_PER_PAIR_EPANPROD
+            else:
+                raise SystemExit(f"unknown kernel {k}")
+            log(f"timing {k}: warmup {args.warmup}, steps {args.steps}")
+            vals, ms, wall = time_kernel(kde, desc, args.steps, args.warmup)
+            avg_ms = float(np.mean(ms))
+            pairs = float(N_REF) * float(N_QUERY)
+            results[k] = {"vals": vals, "ms": ms, "avg_ms": avg_ms, "pair_evals_per_s": pairs / (avg_ms * 1e-3),
+                          "tflops": pairs * flop / (avg_ms * 1e-3) / 1e12, "wall_s": wall}
+            log(f"{k}: {avg_ms:.3f} ms/step, {pairs / (avg_ms * 1e-3):.4e} pair-evals/s")
+
+    head = results.get("gauss") or next(iter(results.values()))
+    line = {
+        "metric": "pair-evals/s 1Mx1M 2D FP32 vs FP64 oracle (GaussianKernel h=0.05)",
+        "value": head["pair_evals_per_s"],
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["avg_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 3-component 2-D Gaussian mixture refs, uniform queries)",
+        "config": {"workload": "direct 2-D KDE, 1,000,000 mixture refs (kd order) x 1,000,000 uniform queries "
+                               "on [-4,4]^2, GaussianKernel h=0.05", "n_ref": N_REF, "n_query": N_QUERY,
+                   "seed": 1002, "accumulation": "FP32 per 256-ref stage, FP64 across stages"},
+        "roofline": {"bound": "mfma", "pipe": "FP32 VALU + v_exp_f32 (compute-bound, FP32 peak)",
+                     "achieved": head["tflops"], "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": head["tflops"] / PEAK_FP32_TFLOPS, "flop_per_pair": FLOP_PER_PAIR_GAUSS,
+                     "traffic": None},
+    }
+    if "epanprod" in results:
+        e = results["epanprod"]
+        line["epanechnikov_product"] = {"pair_evals_per_s": e["pair_evals_per_s"], "ms_per_step": e["avg_ms"],
+                                        "tflops": e["tflops"], "frac": e["tflops"] / PEAK_FP32_TFLOPS,
+                                        "flop_per_pair": FLOP_PER_PAIR_EPANPROD, "hx": H_EPAN[0],
+                                        "hy": H_EPAN[1]}
+    if not args.no_cpu_baseline:
+        log("CPU baseline leg")
+        cb = cpu_baseline(pts, mass, q, head["vals"], args.cpu_seconds)
+        main_cb = cb.get("reference") or cb["port"]
+        line["cpu_baseline"] = main_cb
+        if "reference" in cb:
+            line["cpu_baseline_port"] = cb["port"]
+        line["parity"] = cb["parity"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

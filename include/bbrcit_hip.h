@@ -1,0 +1,212 @@
+/*
+ * bbrcit_hip.h -- C ABI of the MI355X (gfx950) direct kernel-density library.
+ *
+ * This is the drop-in boundary that replaces the reference's device backend,
+ * the class template `CudaDirectKde<D,FloatT,KernelT>`
+ * (include/CudaDirectKde.h:28-123 in dchao34/bbrcit_kde).  Everything here is
+ * plain C: POD structs, plain pointers and sizes, no C++ or torch types and
+ * no exceptions.  Every function returns an int status: BK_OK (0) on
+ * success, a negative BK_ERR_* code on failure; bk_last_error() then returns
+ * a message describing the failure (thread-local, valid until the next call
+ * on the same thread).
+ *
+ * Data model.  A context owns two point sets on one GPU, the references
+ * ("data") and the queries, uploaded once and addressed afterwards by index
+ * ranges, exactly as CudaDirectKde addresses the points it uploaded in its
+ * constructor.  Each point carries D coordinates, a mass and an adaptive
+ * bandwidth correction `abw`.  A kernel evaluation computes, for each query
+ * q in a closed range [q_i, q_j],
+ *
+ *     S_q = sum_{r = r_i .. r_j} mass_r * U_K(x_q - x_r ; h * abw_r)
+ *
+ * (the unnormalised sum), optionally multiplied by the kernel's normalisation
+ * C(h).  U_K and C are those of the reference's kernel families
+ * (include/Kernels/KernelTraits.h), selected by bk_kernel_desc.
+ *
+ * Precision.  BK_PREC_F32 contexts round coordinates, masses and abw to FP32
+ * on upload (as Point2d<float>::operator= does, include/Point2d.h:31-38),
+ * evaluate every pair term in FP32 and accumulate per query with block
+ * compensation into FP64.  BK_PREC_F64 contexts keep everything in FP64.
+ * Results are always returned as double.
+ *
+ * Threading.  A context is not thread-safe; use one context per host thread.
+ * All work of a context runs in order on its HIP stream (bk_set_stream).
+ */
+#ifndef BBRCIT_HIP_H_
+#define BBRCIT_HIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define BK_OK                0
+#define BK_ERR_INVALID_ARG  -1  /* reference: std::invalid_argument (i > j, ...) */
+#define BK_ERR_OUT_OF_RANGE -2  /* reference: std::out_of_range (j >= n)         */
+#define BK_ERR_HIP          -3  /* a HIP runtime call failed                      */
+#define BK_ERR_STATE        -4  /* call not valid in the context's current state  */
+#define BK_ERR_UNSUPPORTED  -5  /* kernel family / dimension not supported        */
+#define BK_ERR_ALLOC        -6  /* device or host allocation failed               */
+
+/* ---- enums ------------------------------------------------------------ */
+/* Point sets of a context. */
+#define BK_SET_REF   0
+#define BK_SET_QUERY 1
+
+/* Arithmetic precision of a context. */
+#define BK_PREC_F32  0
+#define BK_PREC_F64  1
+
+/* Kernel families: one per kernel class template of the reference
+ * (include/Kernels/*.h).  `dim` of the descriptor must be 1 or 2, except for
+ * BK_GAUSSIAN and BK_EPANECHNIKOV, which accept any D from 1 to 64. */
+#define BK_EPANECHNIKOV                 1 /* EpanechnikovKernel<D,T>          */
+#define BK_GAUSSIAN                     2 /* GaussianKernel<D,T>              */
+#define BK_EPANECHNIKOV_PRODUCT_2D      3 /* EpanechnikovProductKernel2d<T>   */
+#define BK_GAUSSIAN_PRODUCT_2D          4 /* GaussianProductKernel2d<T>       */
+#define BK_EPANECHNIKOV_CONV_1D         5 /* EpanechnikovConvKernel1d<T>      */
+#define BK_EPANECHNIKOV_PRODUCT_CONV_2D 6 /* EpanechnikovProductConvKernel2d<T> */
+#define BK_GAUSSIAN_CONV_1D             7 /* GaussianConvKernel1d<T>          */
+#define BK_GAUSSIAN_PRODUCT_CONV_2D     8 /* GaussianProductConvKernel2d<T>   */
+
+/* ---- POD types -------------------------------------------------------- */
+typedef struct bk_ctx bk_ctx;
+
+/* Kernel descriptor: the value of a reference kernel object.  hx is the
+ * bandwidth of the isotropic and 1-D kernels; (hx, hy) are the two
+ * bandwidths of the product kernels.  hy is ignored where it has no
+ * meaning. */
+typedef struct bk_kernel_desc {
+  int    kind;  /* BK_EPANECHNIKOV ... BK_GAUSSIAN_PRODUCT_CONV_2D */
+  int    dim;   /* point dimension D */
+  double hx;
+  double hy;
+} bk_kernel_desc;
+
+/* A closed pair of index ranges: references [r_i, r_j], queries [q_i, q_j]. */
+typedef struct bk_range_pair {
+  size_t r_i, r_j;
+  size_t q_i, q_j;
+} bk_range_pair;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with arithmetic precision
+ * BK_PREC_F32 or BK_PREC_F64.  Replaces CudaDirectKde's default
+ * construction (include/CudaDirectKde.h:42, 313-320). */
+int bk_create(bk_ctx **ctx_out, int device, int precision);
+
+/* Release every device buffer of the context and the context itself.
+ * Replaces ~CudaDirectKde (include/CudaDirectKde.h:376-381). */
+int bk_destroy(bk_ctx *ctx);
+
+/* Message for the last failure on this thread ("" if none). */
+const char *bk_last_error(void);
+
+/* Make the context run its work on `stream` (a hipStream_t); NULL selects
+ * the context's own stream.  The reference uses the default stream only. */
+int bk_set_stream(bk_ctx *ctx, void *stream);
+
+/* Upload point set `set` (BK_SET_REF or BK_SET_QUERY): n points of dimension
+ * dim, coords row-major (coords[i*dim + d]), one mass and one abw per point.
+ * mass and abw may be NULL (mass 1, abw 1).  The library keeps its own
+ * device copy; the host buffers may be freed on return.  Replaces the
+ * constructor CudaDirectKde(ref_pts, query_pts) and its helper
+ * alloc_copy_host_dev_points (include/CudaDirectKde.h:57-59, 330-374). */
+int bk_upload(bk_ctx *ctx, int set, int dim, size_t n, const double *coords,
+              const double *mass, const double *abw);
+
+/* Make the query set the reference set itself (no second copy).  Required
+ * by the leave-one-out entry points, which exclude the pair i == i by index.
+ * The reference reaches the same state by copying its data tree into a
+ * query tree (include/KernelDensityImpl.h:40, 156, 228, 319). */
+int bk_alias_queries_to_refs(bk_ctx *ctx);
+
+/* Number of points in `set`.  Replaces reference_size()/query_size()
+ * (include/CudaDirectKde.h:64-67). */
+int bk_size(const bk_ctx *ctx, int set, size_t *n_out);
+
+/* Update the masses and abw of the reference set in place (same n).  Used
+ * after an adaptive pass; NULL leaves that array unchanged. */
+int bk_update_refs(bk_ctx *ctx, const double *mass, const double *abw);
+
+/* ---- direct sums ----------------------------------------------------- */
+
+/* out_host[k - q_i] = S_k (times C(h) if normalize != 0) for k in [q_i, q_j],
+ * summing references r_i..r_j.  Errors as the reference: BK_ERR_OUT_OF_RANGE
+ * if r_j >= n_ref or q_j >= n_query, BK_ERR_INVALID_ARG if r_i > r_j or
+ * q_i > q_j.  Blocks until the results are on the host.  Replaces
+ * CudaDirectKde::unnormalized_eval and ::eval
+ * (include/CudaDirectKde.h:176-229). */
+int bk_eval_range(bk_ctx *ctx, const bk_kernel_desc *desc,
+                  size_t r_i, size_t r_j, size_t q_i, size_t q_j,
+                  double *out_host, int normalize);
+
+/* As bk_eval_range, but writes to device memory out_dev and returns without
+ * waiting (work is queued on the context's stream).  For callers that keep
+ * results on the GPU and for timing with events on that stream. */
+int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc,
+                         size_t r_i, size_t r_j, size_t q_i, size_t q_j,
+                         double *out_dev, int normalize);
+
+/* Batched leaf-pair evaluation: all n_pairs (reference range, query range)
+ * pairs in one launch.  The results of pair p are written contiguously,
+ * pair after pair: out_host[off_p + (k - q_i)] with off_p the sum of the
+ * query-range lengths of the pairs before p.  Unnormalised unless
+ * normalize != 0.  Replaces the per-leaf-pair synchronous calls
+ * cu_kde.unnormalized_eval(...) of the dual-tree base case
+ * (include/KernelDensityImpl.h:921-983, call at 939-942). */
+int bk_eval_leaf_pairs(bk_ctx *ctx, const bk_kernel_desc *desc,
+                       const bk_range_pair *pairs, size_t n_pairs,
+                       double *out_host, int normalize);
+
+/* ---- self evaluations and cross-validation ---------------------------- */
+
+/* Requires bk_alias_queries_to_refs.  out_host[i] = S_i over all
+ * references; with exclude_self != 0 the pair (i, i) is left out of the sum
+ * (by index, so no cancellation).  Normalised if normalize != 0. */
+int bk_self_eval(bk_ctx *ctx, const bk_kernel_desc *desc, int exclude_self,
+                 double *out_host, int normalize);
+
+/* Leave-one-out log-likelihood score
+ *     LCV = sum_i w_i * log( C * sum_{j != i} m_j U(x_i - x_j) )
+ * with weights w (host array, n_ref values), reduced in FP64 on the GPU.
+ * Same value definition as KernelDensity::likelihood_cross_validate
+ * (include/KernelDensityImpl.h:215-252), whose value minus self term is
+ * computed here by excluding j == i.  Requires bk_alias_queries_to_refs. */
+int bk_likelihood_cv(bk_ctx *ctx, const bk_kernel_desc *desc,
+                     const double *weights, double *score_out);
+
+/* Least-squares cross-validation score with the convolution kernel K* of
+ * desc (the kind's convolution partner, see ConvKernelAssociator):
+ *     LSCV = sum_i w_i C* sum_j m_j U*(x_i - x_j)
+ *            - 2 sum_i w_i C sum_{j != i} m_j U(x_i - x_j)
+ * reduced in FP64 on the GPU.  Same value definition as
+ * KernelDensity::lsq_convolution_cross_validate
+ * (include/KernelDensityImpl.h:142-209), including the reference's
+ * Gaussian-product convolution kernel (KernelTraits.h:423-447).  desc.kind
+ * must be one of BK_EPANECHNIKOV (D=1), BK_GAUSSIAN (D=1),
+ * BK_EPANECHNIKOV_PRODUCT_2D, BK_GAUSSIAN_PRODUCT_2D.  Requires
+ * bk_alias_queries_to_refs. */
+int bk_lsq_conv_cv(bk_ctx *ctx, const bk_kernel_desc *desc,
+                   const double *weights, double *score_out);
+
+/* Adaptive-bandwidth pass (KernelDensity::adapt_density,
+ * include/KernelDensityImpl.h:291-359): pilot f_i = C * S_i (all pairs,
+ * self included) with the non-adaptive masses w_i and abw 1, geometric mean
+ * g = exp(sum_i w_i log f_i) reduced in FP64 on the GPU, local corrections
+ * lambda_i = (f_i / g)^(-alpha).  On return the context's reference set
+ * holds abw_i = lambda_i and mass_i = w_i * lambda_i^(-D); lambda_out
+ * (n_ref values, may be NULL) and g_out (may be NULL) receive the results.
+ * alpha must lie in [0, 1] (BK_ERR_INVALID_ARG otherwise); alpha == 0
+ * resets to the non-adaptive state.  Requires bk_alias_queries_to_refs. */
+int bk_adapt(bk_ctx *ctx, const bk_kernel_desc *desc, double alpha,
+             const double *weights, double *g_out, double *lambda_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBRCIT_HIP_H_ */

@@ -1,0 +1,86 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+The values come from the reference's own CPU implementation
+(dchao34/bbrcit_kde, compiled from its sources by oracle/Makefile into
+oracle/_ref/libbbrcit_ref.so).  Inputs are seeded synthetic points; each
+fixture stores inputs and expected outputs only.
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from oracle.oracle import Reference  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+# (kind, dim, hx, hy) cases of the direct sum; kinds as include/bbrcit_hip.h.
+DIRECT_CASES = [
+    (1, 1, 0.35, 0.35), (1, 2, 0.45, 0.45), (2, 1, 0.25, 0.25), (2, 2, 0.3, 0.3),
+    (3, 2, 0.4, 0.55), (4, 2, 0.3, 0.45), (5, 1, 0.35, 0.35), (6, 2, 0.4, 0.55),
+    (7, 1, 0.25, 0.25), (8, 2, 0.3, 0.45),
+]
+CV_CASES = [(1, 1, 0.35, 0.35), (2, 1, 0.25, 0.25), (3, 2, 0.4, 0.55), (4, 2, 0.3, 0.45)]
+
+
+def inputs(seed=2024, n=600, nq=200):
+    rng = np.random.default_rng(seed)
+    comp = rng.random(n) < 0.4
+    pts = np.where(comp[:, None], rng.normal([-1.0, 0.5], [0.4, 0.6], (n, 2)),
+                   rng.normal([0.8, -0.3], [0.5, 0.3], (n, 2)))
+    w = rng.uniform(0.5, 1.5, n)
+    q = rng.uniform(-2.5, 2.5, (nq, 2))
+    return pts, w, q
+
+
+def main() -> int:
+    ref = Reference()
+    pts, w, q = inputs()
+    out = {"pts": pts, "w": w, "q": q, "cases": np.array(DIRECT_CASES, dtype=np.float64)}
+    for i, (kind, dim, hx, hy) in enumerate(DIRECT_CASES):
+        for kf in (0, 1):
+            out[f"direct_{i}_kf{kf}"] = ref.direct_eval(kind, kf, hx, hy, pts[:, :dim], w, q[:, :dim])
+    np.savez_compressed(OUT / "direct_ref.npz", **out)
+
+    # cross-validation scores with tight dual-tree tolerances (near exact)
+    cv = {"pts": pts, "w": w, "cases": np.array(CV_CASES, dtype=np.float64)}
+    for i, (kind, dim, hx, hy) in enumerate(CV_CASES):
+        cv[f"lcv_{i}"] = ref.likelihood_cv(kind, 0, hx, hy, pts[:, :dim], w, rel=1e-12, abs_err=1e-15)
+        cv[f"lscv_{i}"] = ref.lsq_conv_cv(kind, 0, hx, hy, pts[:, :dim], w, rel=1e-12, abs_err=1e-15)
+    np.savez_compressed(OUT / "cv_ref.npz", **cv)
+
+    # adaptive densities: tree-ordered points with adapted abw and mass
+    ad = {"pts": pts, "w": w}
+    for alpha in (0.0, 0.5, 1.0):
+        c, ww, a, m = ref.adapt(4, 0, 0.3, 0.45, pts, w, alpha, rel=1e-12, abs_err=1e-15)
+        tag = str(alpha).replace(".", "p")
+        ad[f"coords_{tag}"], ad[f"w_{tag}"], ad[f"abw_{tag}"], ad[f"mass_{tag}"] = c, ww, a, m
+    np.savez_compressed(OUT / "adapt_ref.npz", **ad)
+
+    # dual-tree evaluation with its default-style tolerances: permuted
+    # queries and the lower/upper bounds of each
+    tr = {"pts": pts, "w": w, "q": q}
+    for kind, hx, hy, tag in ((2, 0.3, 0.3, "gauss"), (3, 0.4, 0.55, "epanprod")):
+        qo, lo, up = ref.tree_eval(kind, 0, hx, hy, pts, w, q, rel=1e-3, abs_err=1e-6, leaf_nmax=16,
+                                   q_leaf_nmax=16)
+        tr[f"q_{tag}"], tr[f"lower_{tag}"], tr[f"upper_{tag}"] = qo, lo, up
+    np.savez_compressed(OUT / "tree_ref.npz", **tr)
+
+    # least-squares CV by numerical integration on a 100x100 grid
+    ni = {"pts": pts, "w": w}
+    ni["numint_gaussprod"] = ref.lsq_numint_cv(4, 0, 0.3, 0.45, pts, w, -4.0, 4.0, 100, -4.0, 4.0, 100,
+                                               rel=1e-12, abs_err=1e-15)
+    np.savez_compressed(OUT / "numint_ref.npz", **ni)
+    print("fixtures written to", OUT)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

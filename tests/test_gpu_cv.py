@@ -1,0 +1,127 @@
+"""GPU parity of the batched leaf-pair evaluation, the leave-one-out self
+sums, the cross-validation scores and the adaptive pass (C ABI vs oracle).
+
+LCV and LSCV are FP64 reductions over FP32 per-point sums; each per-point
+sum carries <= 1e-5 relative error, so the scores are compared with an
+absolute tolerance of 1e-5 * sum_i w_i |term_i| (a bound on the propagated
+error).  lambda of the adaptive pass is a power -alpha of the pilot, so its
+relative error is <= alpha * 1e-5.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from bbrcit_kde_amd import datasets as ds
+from tests.test_gpu_direct import assert_parity, f32
+
+pytestmark = pytest.mark.gpu
+
+EPAN, GAUSS, EPANPROD, GAUSSPROD = 1, 2, 3, 4
+
+
+def mixture(n, seed):
+    pts, w = ds.gaussian_mixture(n, seed=seed)
+    perm = ds.kd_order(pts, 32)
+    pts, w = pts[perm], w[perm]
+    return pts, w / w.sum()
+
+
+def test_leaf_pairs_match_individual_ranges(bk, oracle):
+    pts, mass = mixture(6000, 31)
+    q = ds.uniform_box(4000, -3, 3, 32)
+    rng = np.random.default_rng(1)
+    pairs = []
+    for _ in range(40):
+        r_i = int(rng.integers(0, 6000))
+        r_j = int(rng.integers(r_i, min(6000, r_i + 3000)))
+        q_i = int(rng.integers(0, 4000))
+        q_j = int(rng.integers(q_i, min(4000, q_i + 2500)))
+        pairs.append((r_i, r_j, q_i, q_j))
+    pairs.append((0, 5999, 0, 3999))
+    pairs.append((17, 17, 3999, 3999))
+    desc = bk.KernelDesc(GAUSSPROD, 2, 0.09, 0.13)
+    P, Q, M = f32(pts), f32(q), f32(mass)
+    with bk.DirectKde(pts, q, mass) as kde:
+        outs = kde.eval_leaf_pairs(desc, pairs)
+        single = kde.unnormalized_eval(desc, *pairs[5])
+    for (r_i, r_j, q_i, q_j), got in zip(pairs, outs):
+        want = oracle.direct(GAUSSPROD, 0.09, 0.13, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
+        assert_parity(got, want, what=f"pair {(r_i, r_j, q_i, q_j)}")
+    np.testing.assert_allclose(outs[5], single, rtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", [GAUSS, EPAN, GAUSSPROD, EPANPROD])
+def test_self_eval_with_and_without_self(bk, oracle, kind):
+    pts, mass = mixture(5000, 40 + kind)
+    h = (0.06, 0.08) if kind in (GAUSS, GAUSSPROD) else (0.2, 0.25)
+    if kind in (GAUSS, EPAN):
+        h = (h[0], h[0])
+    desc = bk.KernelDesc(kind, 2, *h)
+    P, M = f32(pts), f32(mass)
+    with bk.DirectKde(pts, None, mass) as kde:
+        full = kde.self_eval(desc, exclude_self=False, normalize=False)
+        loo = kde.self_eval(desc, exclude_self=True, normalize=False)
+    assert_parity(full, oracle.direct(kind, *h, P, P, mass=M), what="full")
+    assert_parity(loo, oracle.direct(kind, *h, P, P, mass=M, exclude_self=True), what="loo")
+
+
+def test_isolated_point_loo_is_exact(bk):
+    # An isolated point's leave-one-out sum is tiny: by subtraction it would
+    # cancel in FP32; by exclusion it must keep full relative accuracy.
+    pts = np.array([[0.0, 0.0], [0.01, 0.0], [1.5, 1.5], [0.0, 0.013]])
+    mass = np.array([0.25, 0.25, 0.25, 0.25])
+    desc = bk.KernelDesc(GAUSS, 2, 0.5)
+    with bk.DirectKde(pts, None, mass) as kde:
+        loo = kde.self_eval(desc, exclude_self=True, normalize=False)
+    d = pts[:, None, :] - pts[None, :, :]
+    k = 0.25 * np.exp(-0.5 * (d ** 2).sum(-1) / 0.25)
+    np.fill_diagonal(k, 0.0)
+    want = k.sum(1)
+    np.testing.assert_allclose(loo, want, rtol=1e-5)
+    assert loo[2] > 0
+
+
+@pytest.mark.parametrize("kind,dim,h", [(GAUSSPROD, 2, (0.05, 0.07)), (EPANPROD, 2, (0.15, 0.2)),
+                                        (GAUSS, 1, (0.05, 0.05)), (EPAN, 1, (0.12, 0.12))])
+def test_likelihood_and_lsq_cv(bk, oracle, kind, dim, h):
+    pts, w = mixture(8000, 60 + kind)
+    pts = pts[:, :dim]
+    desc = bk.KernelDesc(kind, dim, *h)
+    P, W = f32(pts), w
+    with bk.DirectKde(pts, None, w) as kde:
+        lcv = kde.likelihood_cv(desc, w)
+        lscv = kde.lsq_conv_cv(desc, w)
+    want_lcv = oracle.likelihood_cv(kind, *h, P, W, mass=f32(w))
+    want_lscv = oracle.lsq_conv_cv(kind, *h, P, W, mass=f32(w))
+    assert abs(lcv - want_lcv) <= 1e-5 * (1 + abs(want_lcv))
+    assert abs(lscv - want_lscv) <= 1e-5 * abs(want_lscv) + 1e-8
+
+
+def test_adapt_matches_oracle(bk, oracle):
+    pts, w = mixture(8000, 77)
+    desc = bk.KernelDesc(GAUSSPROD, 2, 0.05, 0.05)
+    P = f32(pts)
+    with bk.DirectKde(pts, None, w) as kde:
+        g, lam = kde.adapt(desc, 0.5, w)
+        # the references now carry abw = lambda, mass = w lambda^-2
+        after = kde.self_eval(desc, exclude_self=False, normalize=True)
+    g_want, lam_want = oracle.adapt(GAUSSPROD, 0.05, 0.05, P, f32(w), 0.5)
+    assert g == pytest.approx(g_want, rel=1e-5)
+    np.testing.assert_allclose(lam, lam_want, rtol=1e-5)
+    assert abs((w * np.log(lam)).sum()) < 1e-6
+    want_after = oracle.direct(GAUSSPROD, 0.05, 0.05, P, P, mass=f32(w * lam ** -2.0), abw=f32(lam),
+                               normalize=True)
+    assert_parity(after, want_after, what="adapted self density")
+
+
+def test_adapt_alpha_zero_and_invalid(bk):
+    pts, w = mixture(500, 3)
+    desc = bk.KernelDesc(GAUSSPROD, 2, 0.1, 0.1)
+    with bk.DirectKde(pts, None, w) as kde:
+        _, lam = kde.adapt(desc, 0.0, w)
+        np.testing.assert_array_equal(lam, 1.0)
+        with pytest.raises(ValueError):
+            kde.adapt(desc, 1.5, w)
+        with pytest.raises(ValueError):
+            kde.adapt(desc, -0.1, w)

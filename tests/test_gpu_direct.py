@@ -1,0 +1,184 @@
+"""GPU parity of the direct sums through the C ABI, against the FP64 oracle.
+
+Tolerance: FP32 contexts must agree with the FP64 oracle (run on the same
+FP32-rounded points, masses and abw) to a relative error of 1e-5 wherever
+the exact value is at least FLOOR_REL times the largest value of the case;
+below that floor the absolute error must stay under 1e-5 * FLOOR_REL * max.
+FP64 contexts must agree to 1e-12.  Cases follow the reference's tests:
+every kernel family, index sub-ranges as in test/test_cukde2.cc:163-257,
+the range errors of CudaDirectKde::unnormalized_eval
+(include/CudaDirectKde.h:181-190), ragged and single-point sizes.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from bbrcit_kde_amd import datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+FLOOR_REL = 1e-12
+RTOL_F32 = 1e-5
+
+EPAN, GAUSS, EPANPROD, GAUSSPROD, EPANCONV1, EPANPRODCONV, GAUSSCONV1, GAUSSPRODCONV = range(1, 9)
+KINDS = [(EPAN, 1), (EPAN, 2), (GAUSS, 1), (GAUSS, 2), (EPANPROD, 2), (GAUSSPROD, 2), (EPANCONV1, 1),
+         (EPANPRODCONV, 2), (GAUSSCONV1, 1), (GAUSSPRODCONV, 2)]
+BW = {EPAN: (0.35, 0.35), GAUSS: (0.12, 0.12), EPANPROD: (0.3, 0.45), GAUSSPROD: (0.1, 0.16),
+      EPANCONV1: (0.3, 0.3), EPANPRODCONV: (0.3, 0.45), GAUSSCONV1: (0.12, 0.12), GAUSSPRODCONV: (0.1, 0.16)}
+
+
+def assert_parity(got, want, rtol=RTOL_F32, what=""):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    scale = np.abs(want).max() if want.size else 0.0
+    floor = FLOOR_REL * scale
+    above = np.abs(want) >= floor
+    if above.any():
+        rel = np.abs(got[above] - want[above]) / np.abs(want[above])
+        assert rel.max() <= rtol, f"{what}: max rel err {rel.max():.3e} at {np.argmax(rel)}"
+    if (~above).any():
+        assert np.abs(got[~above] - want[~above]).max() <= rtol * max(floor, 1e-300), f"{what}: below-floor"
+
+
+def make_case(n_ref, n_q, dim, seed, adaptive):
+    pts, w = ds.gaussian_mixture(n_ref, seed=seed)
+    q = ds.uniform_box(n_q, -3.0, 3.0, seed + 1)
+    pts, q = pts[:, :dim], q[:, :dim]
+    mass = w / w.sum()
+    rng = np.random.default_rng(seed + 2)
+    abw = rng.uniform(0.6, 1.8, n_ref) if adaptive else None
+    return pts, q, mass, abw
+
+
+def f32(a):
+    return None if a is None else ds.round_f32(a)
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+@pytest.mark.parametrize("kind,dim", KINDS)
+def test_direct_f32_matches_oracle(bk, oracle, kind, dim, adaptive):
+    pts, q, mass, abw = make_case(3001, 2050, dim, 100 + kind * 10 + dim, adaptive)
+    hx, hy = BW[kind]
+    desc = bk.KernelDesc(kind, dim, hx, hy)
+    with bk.DirectKde(pts, q, mass, abw) as kde:
+        got = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+    want = oracle.direct(kind, hx, hy, f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
+    assert_parity(got, want, what=f"kind {kind} dim {dim} adaptive {adaptive}")
+
+
+@pytest.mark.parametrize("kind,dim", KINDS)
+def test_direct_f64_matches_oracle(bk, oracle, kind, dim):
+    pts, q, mass, abw = make_case(1500, 700, dim, 300 + kind, True)
+    hx, hy = BW[kind]
+    desc = bk.KernelDesc(kind, dim, hx, hy)
+    with bk.DirectKde(pts, q, mass, abw, precision=bk.BK_PREC_F64) as kde:
+        got = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+    want = oracle.direct(kind, hx, hy, pts, q, mass=mass, abw=abw)
+    assert_parity(got, want, rtol=1e-12, what=f"f64 kind {kind}")
+
+
+def test_index_subranges_like_reference_test(bk, oracle):
+    # test/test_cukde2.cc:163-257 evaluates five (ref, query) sub-ranges.
+    pts, q, mass, _ = make_case(5000, 3000, 2, 17, False)
+    desc = bk.KernelDesc(GAUSS, 2, 0.1)
+    P, Q, M = f32(pts), f32(q), f32(mass)
+    ranges = [(0, 4999, 0, 2999), (0, 0, 0, 2999), (1234, 4321, 17, 17), (4999, 4999, 2999, 2999),
+              (250, 3333, 1000, 2600)]
+    with bk.DirectKde(pts, q, mass) as kde:
+        for r_i, r_j, q_i, q_j in ranges:
+            got = kde.unnormalized_eval(desc, r_i, r_j, q_i, q_j)
+            want = oracle.direct(GAUSS, 0.1, 0.1, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
+            assert got.shape == (q_j - q_i + 1,)
+            assert_parity(got, want, what=f"range {(r_i, r_j, q_i, q_j)}")
+
+
+def test_range_errors_match_reference(bk):
+    pts, q, mass, _ = make_case(100, 50, 2, 5, False)
+    desc = bk.KernelDesc(GAUSS, 2, 0.1)
+    with bk.DirectKde(pts, q, mass) as kde:
+        assert kde.reference_size() == 100 and kde.query_size() == 50
+        with pytest.raises(IndexError):
+            kde.unnormalized_eval(desc, 0, 100, 0, 10)  # j_r >= n_ref: out_of_range
+        with pytest.raises(IndexError):
+            kde.unnormalized_eval(desc, 0, 99, 0, 50)  # j_q >= n_query
+        with pytest.raises(ValueError):
+            kde.unnormalized_eval(desc, 10, 9, 0, 10)  # i_r > j_r: invalid_argument
+        with pytest.raises(ValueError):
+            kde.unnormalized_eval(desc, 0, 9, 20, 10)
+        with pytest.raises(bk.BkError):
+            kde.unnormalized_eval(bk.KernelDesc(EPANPROD, 1, 0.1), 0, 9, 0, 9)  # kind/dim mismatch
+        with pytest.raises(ValueError):
+            kde.unnormalized_eval(bk.KernelDesc(GAUSS, 2, -0.1), 0, 9, 0, 9)
+
+
+@pytest.mark.parametrize("n_ref,n_q", [(1, 1), (1, 777), (255, 1), (257, 1025), (1024, 1024), (70001, 33)])
+def test_ragged_sizes(bk, oracle, n_ref, n_q):
+    pts, q, mass, abw = make_case(n_ref, n_q, 2, 40 + n_ref % 97, True)
+    desc = bk.KernelDesc(GAUSSPROD, 2, 0.08, 0.12)
+    with bk.DirectKde(pts, q, mass, abw) as kde:
+        got = kde.unnormalized_eval(desc, 0, n_ref - 1, 0, n_q - 1)
+    want = oracle.direct(GAUSSPROD, 0.08, 0.12, f32(pts), f32(q), mass=f32(mass), abw=f32(abw))
+    assert_parity(got, want, what=f"ragged {n_ref}x{n_q}")
+
+
+def test_results_are_deterministic(bk):
+    pts, q, mass, _ = make_case(20000, 5000, 2, 9, False)
+    desc = bk.KernelDesc(GAUSS, 2, 0.05)
+    with bk.DirectKde(pts, q, mass) as kde:
+        a = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        b = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+    assert np.array_equal(a, b)
+
+
+def test_invariants_linearity_permutation(bk):
+    pts, q, mass, _ = make_case(8000, 2000, 2, 21, False)
+    rng = np.random.default_rng(0)
+    m1 = rng.random(len(pts))
+    m2 = rng.random(len(pts))
+    desc = bk.KernelDesc(GAUSSPROD, 2, 0.07, 0.11)
+    n, nq = len(pts), len(q)
+    with bk.DirectKde(pts, q, m1 + m2) as kde:
+        s12 = kde.unnormalized_eval(desc, 0, n - 1, 0, nq - 1)
+    with bk.DirectKde(pts, q, m1) as kde:
+        s1 = kde.unnormalized_eval(desc, 0, n - 1, 0, nq - 1)
+    with bk.DirectKde(pts, q, m2) as kde:
+        s2 = kde.unnormalized_eval(desc, 0, n - 1, 0, nq - 1)
+    perm = rng.permutation(n)
+    with bk.DirectKde(pts[perm], q, (m1 + m2)[perm]) as kde:
+        sp = kde.unnormalized_eval(desc, 0, n - 1, 0, nq - 1)
+    big = s12 > 1e-10 * s12.max()
+    np.testing.assert_allclose((s1 + s2)[big], s12[big], rtol=3e-6)
+    np.testing.assert_allclose(sp[big], s12[big], rtol=3e-6)
+
+
+@pytest.mark.parametrize("kind,dim", [(GAUSS, 3), (GAUSS, 8), (GAUSS, 16), (EPAN, 5), (EPAN, 16)])
+def test_general_dimension(bk, oracle, kind, dim):
+    rng = np.random.default_rng(dim)
+    pts = rng.standard_normal((2000, dim))
+    q = rng.standard_normal((700, dim))
+    mass = np.full(len(pts), 1.0 / len(pts))
+    abw = rng.uniform(0.8, 1.25, len(pts))
+    h = 0.5 if kind == GAUSS else 2.0
+    desc = bk.KernelDesc(kind, dim, h)
+    with bk.DirectKde(pts, q, mass, abw) as kde:
+        got = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+    want = oracle.direct(kind, h, h, f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
+    assert_parity(got, want, what=f"D={dim}")
+
+
+def test_mixture_1m_refs_sampled_queries(bk, oracle):
+    # Full-size reference set of the headline workload (1M mixture points,
+    # kd order), checked on sampled query ranges against the oracle.
+    pts, w = ds.gaussian_mixture(1_000_000, seed=ds.SEED_MIXTURE_1M)
+    perm = ds.kd_order(pts, 64)
+    pts, w = pts[perm], w[perm]
+    mass = w / w.sum()
+    q = ds.uniform_box(1_000_000, -4.0, 4.0, ds.SEED_MIXTURE_1M + 1)
+    desc = bk.KernelDesc(GAUSS, 2, 0.05)
+    with bk.DirectKde(pts, q, mass) as kde:
+        for q_i in (0, 500_000, 999_488):
+            got = kde.eval(desc, 0, len(pts) - 1, q_i, q_i + 511)
+            want = oracle.direct(GAUSS, 0.05, 0.05, f32(pts), f32(q[q_i:q_i + 512]), mass=f32(mass),
+                                 normalize=True)
+            assert_parity(got, want, what=f"1M sample at {q_i}")
