@@ -39,6 +39,10 @@ BK_ERR_STATE = -4
 BK_ERR_UNSUPPORTED = -5
 BK_ERR_ALLOC = -6
 
+BK_OPT_GAUSS_FORM = 1
+BK_GAUSS_FORM_AUTO = 0
+BK_GAUSS_FORM_DIRECT = 1
+
 BK_SET_REF = 0
 BK_SET_QUERY = 1
 BK_PREC_F32 = 0
@@ -114,6 +118,7 @@ def load_library() -> ctypes.CDLL:
         "bk_destroy": (ctypes.c_int, [c_ctx_p]),
         "bk_last_error": (ctypes.c_char_p, []),
         "bk_set_stream": (ctypes.c_int, [c_ctx_p, ctypes.c_void_p]),
+        "bk_set_option": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.c_int]),
         "bk_upload": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, _DBL_P, _DBL_P, _DBL_P]),
         "bk_alias_queries_to_refs": (ctypes.c_int, [c_ctx_p]),
         "bk_size": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
@@ -233,6 +238,9 @@ class DirectKde:
 
     def set_stream(self, stream_handle: int) -> None:
         _check(self._lib.bk_set_stream(self._ctx, ctypes.c_void_p(int(stream_handle))))
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(self._lib.bk_set_option(self._ctx, int(option), int(value)))
 
     def update_refs(self, mass=None, abw=None) -> None:
         n = self.reference_size()

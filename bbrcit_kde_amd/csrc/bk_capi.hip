@@ -113,11 +113,14 @@ struct bk_ctx {
   bool alias = false;
   unsigned long long version_counter = 0;
 
-  // feature cache
+  // feature cache (per reference set and kernel descriptor)
   bool feat_valid = false;
   bk_kernel_desc feat_desc{};
   unsigned long long feat_version = 0;
-  DevBuf f0, f1;
+  DevBuf f0, f1, tiles;
+
+  // options (bk_set_option)
+  int gauss_form = BK_GAUSS_FORM_AUTO;
 
   // staging
   DevBuf partial, out, items, work_a, work_b, weights, scratch, scalar;
@@ -388,6 +391,22 @@ bool same_desc(const bk_kernel_desc &a, const bk_kernel_desc &b) {
   return a.kind == b.kind && a.dim == b.dim && a.hx == b.hx && a.hy == b.hy;
 }
 
+// Which kernel family evaluates a descriptor in this context.
+enum class Path { F64, GAUSS32, EPAN64, GENERAL };
+
+Path path_of(const bk_ctx *ctx, const EvalParams &p) {
+  if (ctx->precision == BK_PREC_F64) return Path::F64;
+  if (p.dim > 2) return Path::GENERAL;
+  return p.family == bk::FAM_GAUSS ? Path::GAUSS32 : Path::EPAN64;
+}
+
+// Queries handled by one workgroup / work item on a path.
+int queries_per_item(Path path) {
+  return (path == Path::GAUSS32 || path == Path::EPAN64) ? bk::kQueriesPerBlock : 256;
+}
+
+int tile_of(Path path) { return path == Path::GENERAL ? 128 : 256; }
+
 // Make sure the per-reference features of `desc` are on the device.
 void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &p) {
   PointSet &r = ctx->refs;
@@ -396,23 +415,37 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
   const size_t n = r.n;
   if (n == 0) return;
   const hipStream_t s = ctx->stream;
-  if (ctx->precision == BK_PREC_F64) {
-    ctx->f0.reserve(n * sizeof(double4));
-    ctx->f1.reserve(n * sizeof(double4));
-    hip_check(bk::launch_prep_f64(p, r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
-                                  r.abwd.as<double>(), n, ctx->f0.as<double4>(), ctx->f1.as<double4>(), s),
-              "prep_f64 launch");
-  } else if (r.dim <= 2) {
-    ctx->f0.reserve(n * sizeof(float4));
-    ctx->f1.reserve(n * sizeof(float4));
-    hip_check(bk::launch_prep_f32(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
-                                  mass_scale(r), ctx->f0.as<float4>(), ctx->f1.as<float4>(), s),
-              "prep_f32 launch");
-  } else {
-    ctx->f0.reserve(n * sizeof(float2));
-    hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, mass_scale(r),
-                                      ctx->f0.as<float2>(), s),
-              "prep_general launch");
+  switch (path_of(ctx, p)) {
+    case Path::F64:
+      ctx->f0.reserve(n * sizeof(double4));
+      ctx->f1.reserve(n * sizeof(double4));
+      hip_check(bk::launch_prep_f64(p, r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
+                                    r.abwd.as<double>(), n, ctx->f0.as<double4>(), ctx->f1.as<double4>(), s),
+                "prep_f64 launch");
+      break;
+    case Path::GAUSS32: {
+      const size_t ntiles = (n + bk::kTile - 1) / bk::kTile;
+      ctx->f0.reserve(n * sizeof(float4));
+      ctx->f1.reserve(n * sizeof(float4));
+      ctx->tiles.reserve(ntiles * sizeof(float4));
+      hip_check(bk::launch_prep_gauss(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
+                                      mass_scale(r), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
+                                      ctx->tiles.as<float4>(), s),
+                "prep_gauss launch");
+      break;
+    }
+    case Path::EPAN64:
+      ctx->f0.reserve(n * sizeof(double4));
+      ctx->f1.reserve(n * sizeof(double4));
+      hip_check(bk::launch_prep_epan(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
+                                     ctx->f0.as<double4>(), ctx->f1.as<double4>(), s),
+                "prep_epan launch");
+      break;
+    case Path::GENERAL:
+      ctx->f0.reserve(n * sizeof(double2));
+      hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, ctx->f0.as<double2>(), s),
+                "prep_general launch");
+      break;
   }
   ctx->feat_desc = *desc;
   ctx->feat_version = r.version;
@@ -446,6 +479,42 @@ int choose_splits(long long nq, long long nr, int queries_per_block, int tile) {
   return (int)splits;
 }
 
+// Result scale of a path: FP32 Gaussian features carry masses divided by
+// the largest mass.
+double path_scale(const bk_ctx *ctx, Path path, const EvalParams &p, int normalize) {
+  double scale = normalize ? p.norm : 1.0;
+  if (path == Path::GAUSS32) scale *= mass_scale(ctx->refs);
+  return scale;
+}
+
+// Launch the evaluation kernels of one call: range mode (items == nullptr)
+// or work-item mode.
+hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkItem *items, int n_items,
+                       long long r_i, long long r_end, long long q_i, long long nq, int splits, double *dst,
+                       double scale, int self_excl) {
+  const hipStream_t s = ctx->stream;
+  const PointSet &r = ctx->refs;
+  const PointSet &q = query_set(ctx);
+  switch (path) {
+    case Path::F64:
+      return bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
+                                   ctx->f1.as<double4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
+                                   self_excl, s);
+    case Path::GAUSS32:
+      return bk::launch_gauss(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
+                              ctx->tiles.as<float4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
+                              self_excl,
+                              ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s);
+    case Path::EPAN64:
+      return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<double4>(), ctx->f1.as<double4>(),
+                             items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
+    case Path::GENERAL:
+    default:
+      return bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double2>(),
+                                       items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
+  }
+}
+
 // Direct sums over one range pair into device memory `out` (nq doubles).
 void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t r_j, size_t q_i,
                     size_t q_j, double *out, int normalize, int self_excl) {
@@ -453,48 +522,21 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
   check_compatible(ctx, p);
   check_range(ctx, r_i, r_j, q_i, q_j);
   ensure_features(ctx, desc, p);
-  const hipStream_t s = ctx->stream;
-  const PointSet &r = ctx->refs;
-  const PointSet &q = query_set(ctx);
+  const Path path = path_of(ctx, p);
   const long long nq = (long long)(q_j - q_i + 1);
   const long long nr = (long long)(r_j - r_i + 1);
   const long long r_end = (long long)r_j + 1;
-  double scale = normalize ? p.norm : 1.0;
-  int splits, qpb, tile;
-  if (ctx->precision == BK_PREC_F64) {
-    qpb = 256;
-    tile = 256;
-  } else if (p.dim <= 2) {
-    qpb = bk::kQueriesPerBlock;
-    tile = bk::kTile;
-    scale *= mass_scale(r);
-  } else {
-    qpb = 256;
-    tile = 128;
-    scale *= mass_scale(r);
-  }
-  splits = choose_splits(nq, nr, qpb, tile);
+  const double scale = path_scale(ctx, path, p, normalize);
+  const int splits = choose_splits(nq, nr, queries_per_item(path), tile_of(path));
   double *dst = out;
   if (splits > 1) {
     ctx->partial.reserve((size_t)splits * nq * sizeof(double));
     dst = ctx->partial.as<double>();
   }
-  hipError_t e;
-  if (ctx->precision == BK_PREC_F64) {
-    e = bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
-                              ctx->f1.as<double4>(), nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq,
-                              splits, dst, scale, self_excl, s);
-  } else if (p.dim <= 2) {
-    e = bk::launch_direct_f32_range(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(),
-                                    ctx->f1.as<float4>(), (long long)r_i, r_end, (long long)q_i, nq, splits,
-                                    dst, scale, self_excl, s);
-  } else {
-    e = bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<float2>(),
-                                  nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits, dst, scale,
-                                  self_excl, s);
-  }
-  hip_check(e, "direct kernel launch");
-  if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, s), "reduce launch");
+  hip_check(launch_path(ctx, path, p, nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits, dst, scale,
+                        self_excl),
+            "direct kernel launch");
+  if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, ctx->stream), "reduce launch");
 }
 
 void copy_to_host(bk_ctx *ctx, double *host, const double *dev, size_t n) {
@@ -513,10 +555,39 @@ void require_alias(const bk_ctx *ctx) {
   if (ctx->refs.n == 0) fail(BK_ERR_STATE, "no reference points uploaded");
 }
 
-// Self evaluation into device buffer `dst` (n_ref doubles).
+// FP32 Gaussian sums below 2^-40 of the largest mass are dominated by terms
+// whose base-2 exponent is below -40; they are recomputed in FP64 (only
+// where a leave-one-out value feeds a logarithm or is returned as such).
+constexpr double kRescueFraction = 9.094947017729282e-13;  // 2^-40
+
+void rescue_small(bk_ctx *ctx, const EvalParams &p, double *S_unnormalised, int self_excl) {
+  const PointSet &r = ctx->refs;
+  const PointSet &q = query_set(ctx);
+  const long long n = (long long)q.n;
+  ctx->scratch.reserve((size_t)(n + 1) * sizeof(int));
+  hip_check(bk::launch_rescue_gauss(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(),
+                                    (long long)r.n, q.x.as<float>(), q.y.as<float>(), 0, n, S_unnormalised,
+                                    kRescueFraction * mass_scale(r), self_excl, ctx->scratch.as<int>(),
+                                    ctx->stream),
+            "rescue launch");
+}
+
+// Self evaluation into device buffer `dst` (n_ref doubles, unnormalised
+// unless normalize).  Leave-one-out FP32 Gaussian sums are rescued.
 void self_eval_dev(bk_ctx *ctx, const bk_kernel_desc *desc, int exclude_self, double *dst, int normalize) {
   require_alias(ctx);
-  eval_range_dev(ctx, desc, 0, ctx->refs.n - 1, 0, ctx->refs.n - 1, dst, normalize, exclude_self ? 1 : 0);
+  const EvalParams p = params_of(desc);
+  const size_t n = ctx->refs.n;
+  if (exclude_self && ctx->precision == BK_PREC_F32 && p.dim <= 2 && p.family == bk::FAM_GAUSS) {
+    eval_range_dev(ctx, desc, 0, n - 1, 0, n - 1, dst, 0, 1);
+    rescue_small(ctx, p, dst, 1);
+    if (normalize) {
+      // scale in place through the split reducer (one split = copy with scale)
+      hip_check(bk::launch_scale(dst, (long long)n, p.norm, ctx->stream), "scale launch");
+    }
+    return;
+  }
+  eval_range_dev(ctx, desc, 0, n - 1, 0, n - 1, dst, normalize, exclude_self ? 1 : 0);
 }
 
 double *upload_weights(bk_ctx *ctx, const double *w) {
@@ -580,7 +651,7 @@ int bk_destroy(bk_ctx *ctx) {
       (void)hipStreamSynchronize(ctx->stream);
       ctx->refs.release();
       ctx->queries.release();
-      for (DevBuf *b : {&ctx->f0, &ctx->f1, &ctx->partial, &ctx->out, &ctx->items, &ctx->work_a, &ctx->work_b,
+      for (DevBuf *b : {&ctx->f0, &ctx->f1, &ctx->tiles, &ctx->partial, &ctx->out, &ctx->items, &ctx->work_a, &ctx->work_b,
                         &ctx->weights, &ctx->scratch, &ctx->scalar})
         b->release();
       if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -596,14 +667,26 @@ int bk_set_stream(bk_ctx *ctx, void *stream) {
   });
 }
 
+int bk_set_option(bk_ctx *ctx, int option, int value) {
+  return guarded([&] {
+    require_ctx(ctx);
+    switch (option) {
+      case BK_OPT_GAUSS_FORM:
+        if (value != BK_GAUSS_FORM_AUTO && value != BK_GAUSS_FORM_DIRECT)
+          fail(BK_ERR_INVALID_ARG, "BK_OPT_GAUSS_FORM takes BK_GAUSS_FORM_AUTO or BK_GAUSS_FORM_DIRECT");
+        ctx->gauss_form = value;
+        break;
+      default:
+        fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));
+    }
+  });
+}
+
 int bk_upload(bk_ctx *ctx, int set, int dim, size_t n, const double *coords, const double *mass,
               const double *abw) {
   return guarded([&] {
     require_ctx(ctx);
     DeviceGuard g(ctx->device);
-    if (set == BK_SET_REF && ctx->alias) {
-      // re-uploading the references keeps the alias
-    }
     PointSet &ps = set_of(ctx, set);
     upload_set(ctx, ps, dim, n, coords, mass, abw);
     if (set == BK_SET_REF) ctx->feat_valid = false;
@@ -673,7 +756,8 @@ int bk_eval_leaf_pairs(bk_ctx *ctx, const bk_kernel_desc *desc, const bk_range_p
     DeviceGuard g(ctx->device);
     const EvalParams p = params_of(desc);
     check_compatible(ctx, p);
-    const int qpb = (ctx->precision == BK_PREC_F32 && p.dim <= 2) ? bk::kQueriesPerBlock : 256;
+    const Path path = path_of(ctx, p);
+    const int qpb = queries_per_item(path);
     std::vector<WorkItem> items;
     long long total = 0;
     for (size_t k = 0; k < n_pairs; ++k) {
@@ -699,24 +783,8 @@ int bk_eval_leaf_pairs(bk_ctx *ctx, const bk_kernel_desc *desc, const bk_range_p
                              s),
               "upload work items");
     double *dev = staging_out(ctx, (size_t)total);
-    double scale = normalize ? p.norm : 1.0;
-    const PointSet &r = ctx->refs;
-    const PointSet &q = query_set(ctx);
-    hipError_t e;
-    if (ctx->precision == BK_PREC_F64) {
-      e = bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
-                                ctx->f1.as<double4>(), ctx->items.as<WorkItem>(), (int)items.size(), 0, 0, 0, 0,
-                                1, dev, scale, 0, s);
-    } else if (p.dim <= 2) {
-      scale *= mass_scale(r);
-      e = bk::launch_direct_f32_list(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(),
-                                     ctx->f1.as<float4>(), ctx->items.as<WorkItem>(), (int)items.size(), dev, scale,
-                                     0, s);
-    } else {
-      scale *= mass_scale(r);
-      e = bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<float2>(),
-                                    ctx->items.as<WorkItem>(), (int)items.size(), 0, 0, 0, 0, 1, dev, scale, 0, s);
-    }
+    const hipError_t e = launch_path(ctx, path, p, ctx->items.as<WorkItem>(), (int)items.size(), 0, 0, 0, 0, 1,
+                                     dev, path_scale(ctx, path, p, normalize), 0);
     hip_check(e, "leaf-pair kernel launch");
     copy_to_host(ctx, out_host, dev, (size_t)total);
   });

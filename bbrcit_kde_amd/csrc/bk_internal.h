@@ -9,7 +9,7 @@
 namespace bk {
 
 // Device evaluation families.  Every reference kernel class maps onto one
-// of them (see family_of() in bk_capi.hip):
+// of them (see params_of() in bk_capi.hip):
 //   FAM_GAUSS     m * exp(-c * sum_d (dx_d / (h_d a))^2)       Gaussian kinds
 //   FAM_EPAN      m * max(1 - sum_d (dx_d / (h a))^2, 0)       EpanechnikovKernel<D>
 //   FAM_EPANPROD  m * prod_d max(1 - (dx_d / (h_d a))^2, 0)    EpanechnikovProductKernel2d
@@ -40,38 +40,48 @@ struct WorkItem {
   int pad;
 };
 
-// Geometry of the FP32 direct kernel.
+// Geometry of the direct kernels (D <= 2).
 constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
 constexpr int kQpt = 4;            // queries held in registers per thread
 constexpr int kQueriesPerBlock = kBlock * kQpt;
-constexpr int kTile = 256;         // references per LDS stage
+constexpr int kTile = 256;         // references per LDS stage (tiles are aligned to kTile)
 
-// FP32 path, D <= 2 ------------------------------------------------------
-// Per-reference features for the FP32 path.  f1 is used by the families
-// that need more than four values per reference.
-hipError_t launch_prep_f32(const EvalParams &p, const float *x, const float *y,
-                           const float *m, const float *abw, size_t n,
-                           double m_scale, float4 *f0, float4 *f1,
-                           hipStream_t s);
+// Gaussian evaluation form selection.
+enum GaussMode : int {
+  GAUSS_AUTO = 0,    // tile-centred Gram form on compact tiles, direct form elsewhere
+  GAUSS_DIRECT = 1,  // direct form everywhere
+};
 
-// Range mode: grid (query blocks) x (reference splits).  dst holds
-// `splits` consecutive arrays of nq partial sums (one array if splits == 1).
-hipError_t launch_direct_f32_range(const EvalParams &p, const float *qx,
-                                   const float *qy, const float4 *f0,
-                                   const float4 *f1, long long r_i,
-                                   long long r_end, long long q_i,
-                                   long long nq, int splits, double *dst,
-                                   double out_scale, int self_excl,
-                                   hipStream_t s);
+// FP32 Gaussian path, D <= 2 ---------------------------------------------
+// Per reference: fd = direct-form features, fg = Gram-form features; per
+// aligned tile of kTile references: tiles = (centre x, centre y, gram?, 0).
+hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
+                             const float *m, const float *abw, size_t n,
+                             double m_scale, float4 *fd, float4 *fg,
+                             float4 *tiles, hipStream_t s);
 
-// List mode: one workgroup per work item.
-hipError_t launch_direct_f32_list(const EvalParams &p, const float *qx,
-                                  const float *qy, const float4 *f0,
-                                  const float4 *f1, const WorkItem *items,
-                                  int n_items, double *dst, double out_scale,
-                                  int self_excl, hipStream_t s);
+// items == nullptr: range mode, grid (query blocks) x (reference splits),
+// dst holds `splits` consecutive arrays of nq partial sums.
+// items != nullptr: one workgroup per work item.
+hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
+                        const float4 *fd, const float4 *fg, const float4 *tiles,
+                        const WorkItem *items, int n_items, long long r_i,
+                        long long r_end, long long q_i, long long nq,
+                        int splits, double *dst, double out_scale,
+                        int self_excl, int mode, hipStream_t s);
 
-// FP64 path, D <= 2 ------------------------------------------------------
+// FP64 Epanechnikov-family path, D <= 2, on FP32 point data ----------------
+hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
+                            const float *m, const float *abw, size_t n,
+                            double4 *f0, double4 *f1, hipStream_t s);
+hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
+                       const double4 *f0, const double4 *f1,
+                       const WorkItem *items, int n_items, long long r_i,
+                       long long r_end, long long q_i, long long nq, int splits,
+                       double *dst, double out_scale, int self_excl,
+                       hipStream_t s);
+
+// FP64 context path, D <= 2 ------------------------------------------------
 hipError_t launch_prep_f64(const EvalParams &p, const double *x,
                            const double *y, const double *m,
                            const double *abw, size_t n, double4 *f0,
@@ -84,18 +94,33 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              double *dst, double out_scale, int self_excl,
                              hipStream_t s);
 
-// FP32 path, 3 <= D <= 16 (FAM_GAUSS and FAM_EPAN) ------------------------
-// Features: row-major coordinates (n x D) plus (w, mass term) per point.
+// General dimension, 3 <= D <= 16 (FAM_GAUSS and FAM_EPAN) ---------------
+// FP32 point data, FP64 distances.  wm = (w = g / abw^2, mass) per point.
 hipError_t launch_prep_general(const EvalParams &p, const float *abw,
-                               const float *m, size_t n, double m_scale,
-                               float2 *wm, hipStream_t s);
+                               const float *m, size_t n, double2 *wm,
+                               hipStream_t s);
 hipError_t launch_direct_general(const EvalParams &p, const float *qc,
-                                 const float *rc, const float2 *wm,
+                                 const float *rc, const double2 *wm,
                                  const WorkItem *items, int n_items,
                                  long long r_i, long long r_end, long long q_i,
                                  long long nq, int splits, double *dst,
                                  double out_scale, int self_excl,
                                  hipStream_t s);
+
+// Exact FP64 recomputation of FP32 Gaussian sums that fell below
+// `threshold` (they may have lost accuracy to FP32 underflow).  S holds nq
+// unnormalised sums of queries q_i.. (query index k -> S[k - q_i]) over the
+// references [0, n_ref); entries below threshold are recomputed with the
+// same masses.  scratch needs room for nq + 1 ints.
+hipError_t launch_rescue_gauss(const EvalParams &p, const float *x,
+                               const float *y, const float *m, const float *abw,
+                               long long n_ref, const float *qx, const float *qy,
+                               long long q_i, long long nq, double *S,
+                               double threshold, int self_excl, int *scratch,
+                               hipStream_t s);
+
+// v[i] *= a for i < n.
+hipError_t launch_scale(double *v, long long n, double a, hipStream_t s);
 
 // Reductions -----------------------------------------------------------
 // out[q] = sum_s partial[s * nq + q], fixed order.
@@ -112,8 +137,7 @@ hipError_t launch_weighted_reduce(int mode, const double *w, const double *u,
                                   long long n, double *scratch,
                                   double *result, hipStream_t s);
 
-// lambda_i = (norm * S_i / g)^(-alpha), with g read from device memory
-// (g_dev holds log(g) on entry: the mode-0 reduction result).
+// lambda_i = (norm * S_i / g)^(-alpha), with log(g) read from device memory.
 hipError_t launch_adapt_lambda(const double *S, long long n, double norm,
                                const double *log_g_dev, double alpha,
                                double *lambda, hipStream_t s);

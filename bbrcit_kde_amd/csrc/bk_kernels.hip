@@ -7,17 +7,26 @@
 //   * queries are read once, coalesced, from SoA arrays and kept in
 //     registers, kQpt per thread;
 //   * references are pre-transformed once per kernel descriptor into
-//     per-point feature vectors (scaled bandwidth factor, mass term), so the
-//     per-pair division by h^2 a^2 of the reference becomes a multiply;
+//     per-point feature vectors, so the reference's per-pair division by
+//     h^2 a^2 becomes a multiply;
 //   * a workgroup stages kTile feature vectors in LDS (double-buffered, one
-//     barrier per stage); every lane of a wavefront reads the same entry, so
-//     each LDS read is a broadcast;
-//   * the Gaussian term m*exp(-c*d2/(h a)^2) is evaluated as
-//     exp2(log2(m/m_scale) - d2 * w) with one v_exp_f32;
-//   * per-query sums are FP32 within one LDS stage (<= kTile terms) and are
-//     then added into an FP64 accumulator (block compensation);
-//   * large reference ranges are split over workgroups, with a fixed-order
-//     FP64 reduction of the partial sums, so results are deterministic.
+//     barrier per stage, the next stage prefetched into registers); every
+//     lane of a wavefront reads the same entry, so each LDS read is a
+//     broadcast;
+//   * Gaussian kinds (FP32): the term m*exp(-c d2 / (h a)^2) is one
+//     v_exp_f32 of a base-2 exponent.  On spatially compact tiles the
+//     exponent is formed from tile-centred coordinates as
+//         C_r + A_r . (q - c) - W_r |q - c|^2_g        (3 FMA per pair)
+//     and on the others from the differences q - r directly (5 VALU per
+//     pair).  Per-query sums are FP32 within one stage (<= kTile terms)
+//     and then added into an FP64 accumulator (block compensation);
+//   * Epanechnikov kinds (FP64 arithmetic on the FP32 points): their
+//     compact support makes 1 - d2/(h a)^2 cancel near the support edge,
+//     which FP32 cannot resolve to 1e-5; MI355X runs FP64 VALU at the FP32
+//     (non-packed) rate, so exact differences cost few extra instructions;
+//   * large reference ranges are split over workgroups at tile boundaries,
+//     with a fixed-order FP64 reduction of the partial sums, so results are
+//     deterministic.
 #include "bk_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -26,76 +35,64 @@ namespace bk {
 
 namespace {
 
-constexpr float kMaskedLog = -1.0e30f;  // log2 mass of a masked reference
+constexpr float kMaskedLog = -1.0e30f;  // base-2 log mass of a masked reference
+constexpr float kGramMaxRho2 = 1.0f;    // tile compactness bound of the Gram form
 
 __device__ __forceinline__ float clamp01(float v) {
   return __builtin_fminf(__builtin_fmaxf(v, 0.0f), 1.0f);
 }
 
-template <int FAM>
-struct Fam {
-  static constexpr int kNf = (FAM == FAM_EPANPROD || FAM == FAM_EPANCONV) ? 2 : 1;
-};
-
-// Feature vector of a reference that must contribute nothing.
-template <int FAM>
-__device__ __forceinline__ float4 masked_f0() {
-  if constexpr (FAM == FAM_GAUSS) {
-    return make_float4(0.f, 0.f, 0.f, kMaskedLog);
-  } else {
-    return make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+__device__ __forceinline__ void write_zeros(double *dst, int q_count) {
+  for (int qi = threadIdx.x; qi < q_count; qi += kBlock) dst[qi] = 0.0;
 }
 
-// Contribution m_r * U(q - r) of one reference with features (a, b).
-//   FAM_GAUSS:    a = (x, y, w, log2(m/m_scale))     w = c log2(e) / (hx a_r)^2
-//   FAM_EPAN:     a = (x, y, w, m/m_scale)           w = 1 / (h a_r)^2
-//   FAM_EPANPROD: a = (x, y, wx, wy), b.x = m/m_scale
-//   FAM_EPANCONV: a = (x, y, 1/(hx a_r), 1/(hy a_r)), b.x = m/m_scale * 2^-10
-// rho = (weight of y) / (weight of x) for the quadratic forms.
-template <int FAM, bool ISO>
-__device__ __forceinline__ float pair_term(float qx, float qy, float4 a,
-                                           float4 b, float rho) {
-  const float dx = qx - a.x;
-  const float dy = qy - a.y;
-  if constexpr (FAM == FAM_GAUSS) {
-    const float d2 = ISO ? __builtin_fmaf(dy, dy, dx * dx)
-                         : __builtin_fmaf(dy * rho, dy, dx * dx);
-    return __builtin_amdgcn_exp2f(__builtin_fmaf(-a.z, d2, a.w));
-  } else if constexpr (FAM == FAM_EPAN) {
-    const float d2 = ISO ? __builtin_fmaf(dy, dy, dx * dx)
-                         : __builtin_fmaf(dy * rho, dy, dx * dx);
-    return a.w * clamp01(__builtin_fmaf(-a.z, d2, 1.0f));
-  } else if constexpr (FAM == FAM_EPANPROD) {
-    const float tx = clamp01(__builtin_fmaf(-a.z * dx, dx, 1.0f));
-    const float ty = clamp01(__builtin_fmaf(-a.w * dy, dy, 1.0f));
-    return b.x * (tx * ty);
-  } else {
-    const float ux = __builtin_fabsf(dx) * a.z;
-    const float uy = __builtin_fabsf(dy) * a.w;
-    const float cx = __builtin_fmaxf(2.0f - ux, 0.0f);
-    const float cy = __builtin_fmaxf(2.0f - uy, 0.0f);
-    const float px = __builtin_fmaf(ux, ux + 6.0f, 4.0f);
-    const float py = __builtin_fmaf(uy, uy + 6.0f, 4.0f);
-    const float kx = cx * cx * cx * px;
-    const float ky = cy * cy * cy * py;
-    return b.x * (kx * ky);
-  }
+// Reference range of split `sp`: splits start at tile-aligned positions.
+__device__ __forceinline__ void split_bounds(long long r_i, long long r_end,
+                                             long long len, int sp,
+                                             long long &rb, long long &re) {
+  const long long base = (r_i / kTile) * kTile;
+  rb = base + (long long)sp * len;
+  re = rb + len;
+  if (rb < r_i) rb = r_i;
+  if (re > r_end) re = r_end;
 }
 
-// One work item of the FP32 direct kernel: references [r_begin, r_end)
-// against up to kQueriesPerBlock queries starting at q_begin.
-template <int FAM, bool ISO>
-__device__ __forceinline__ void direct_f32_item(
+// ---- FP32 Gaussian ----------------------------------------------------------
+//   fd (direct form) = (x, y, w, lm)        w = gx / a_r^2,  lm = log2(m_r / m_scale)
+//   fg (Gram form)   = (A0, A1, W, C)       relative to the tile centre c:
+//       A0 = 2 gx W (x - cx), A1 = 2 gy W (y - cy), W = 1 / a_r^2,
+//       C = lm - W (gx (x - cx)^2 + gy (y - cy)^2)
+//   exponent(q, r) = lm - W (gx dx^2 + gy dy^2)
+//                  = C + A0 (qx - cx) + A1 (qy - cy) - W (gx (qx - cx)^2 + gy (qy - cy)^2)
+
+template <bool ISO>
+__device__ __forceinline__ float gauss_direct_term(float qx, float qy, float4 f, float rho) {
+  const float dx = qx - f.x;
+  const float dy = qy - f.y;
+  const float d2 = ISO ? __builtin_fmaf(dy, dy, dx * dx) : __builtin_fmaf(dy * rho, dy, dx * dx);
+  return __builtin_amdgcn_exp2f(__builtin_fmaf(-f.z, d2, f.w));
+}
+
+__device__ __forceinline__ float gauss_gram_term(float ax, float ay, float aq, float4 f) {
+  float a = __builtin_fmaf(-f.z, aq, f.w);
+  a = __builtin_fmaf(f.y, ay, a);
+  a = __builtin_fmaf(f.x, ax, a);
+  return __builtin_amdgcn_exp2f(a);
+}
+
+template <bool ISO>
+__device__ __forceinline__ void gauss_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
-    const float4 *__restrict__ f0, const float4 *__restrict__ f1,
-    long long r_begin, long long r_end, long long q_begin, int q_count,
-    double *__restrict__ dst, double out_scale, int self_excl, float rho) {
-  constexpr int kNf = Fam<FAM>::kNf;
-  __shared__ float4 s0[2][kTile];
-  __shared__ float4 s1[2][kNf == 2 ? kTile : 1];
-
+    const float4 *__restrict__ fd, const float4 *__restrict__ fg,
+    const float4 *__restrict__ tiles, long long r_begin, long long r_end,
+    long long q_begin, int q_count, double *__restrict__ dst, double out_scale,
+    int self_excl, float rho, float gx, float gy, int mode) {
+  __shared__ float4 sf[2][kTile];
   const int t = threadIdx.x;
+  if (r_begin >= r_end) {
+    write_zeros(dst, q_count);
+    return;
+  }
   float px[kQpt], py[kQpt];
   double acc[kQpt];
 #pragma unroll
@@ -106,59 +103,80 @@ __device__ __forceinline__ void direct_f32_item(
     py[k] = qy[q];
     acc[k] = 0.0;
   }
-
-  const float4 m0 = masked_f0<FAM>();
-  const float4 m1 = make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 n0 = m0, n1 = m1;
-  {
-    const long long r = r_begin + t;
-    if (r < r_end) {
-      n0 = f0[r];
-      if constexpr (kNf == 2) n1 = f1[r];
-    }
-  }
-
-  // The self pair (i, i) can only occur in stages whose reference range
-  // overlaps this workgroup's query range.
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
 
+  long long tile = (r_begin / kTile) * kTile;
+  float4 meta = tiles[tile / kTile];
+  bool gram = (mode == GAUSS_AUTO) && meta.z != 0.0f;
+  float4 nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
+  {
+    const long long r = tile + t;
+    if (r >= r_begin && r < r_end) nf = gram ? fg[r] : fd[r];
+  }
   int buf = 0;
-  for (long long tile = r_begin; tile < r_end; tile += kTile) {
-    s0[buf][t] = n0;
-    if constexpr (kNf == 2) s1[buf][t] = n1;
+  for (; tile < r_end; tile += kTile) {
+    sf[buf][t] = nf;
     __syncthreads();
+    const float4 cur = meta;
+    const bool cur_gram = gram;
     {
-      const long long r = tile + kTile + t;
-      n0 = m0;
-      n1 = m1;
-      if (r < r_end) {
-        n0 = f0[r];
-        if constexpr (kNf == 2) n1 = f1[r];
+      const long long nt = tile + kTile;
+      nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
+      if (nt < r_end) {
+        meta = tiles[nt / kTile];
+        gram = (mode == GAUSS_AUTO) && meta.z != 0.0f;
+        const long long r = nt + t;
+        if (r < r_end) nf = gram ? fg[r] : fd[r];
       }
     }
     const int cnt = (int)((r_end - tile) < kTile ? (r_end - tile) : kTile);
+    const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
     float s[kQpt];
 #pragma unroll
     for (int k = 0; k < kQpt; ++k) s[k] = 0.0f;
-
-    const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
-    if (!diag) {
-#pragma unroll 4
-      for (int j = 0; j < cnt; ++j) {
-        const float4 a = s0[buf][j];
-        const float4 b = (kNf == 2) ? s1[buf][j] : a;
+    if (cur_gram) {
+      float ax[kQpt], ay[kQpt], aq[kQpt];
 #pragma unroll
-        for (int k = 0; k < kQpt; ++k) s[k] += pair_term<FAM, ISO>(px[k], py[k], a, b, rho);
+      for (int k = 0; k < kQpt; ++k) {
+        ax[k] = px[k] - cur.x;
+        ay[k] = py[k] - cur.y;
+        aq[k] = __builtin_fmaf(gy * ay[k], ay[k], gx * ax[k] * ax[k]);
+      }
+      if (!diag) {
+#pragma unroll 4
+        for (int j = 0; j < cnt; ++j) {
+          const float4 f = sf[buf][j];
+#pragma unroll
+          for (int k = 0; k < kQpt; ++k) s[k] += gauss_gram_term(ax[k], ay[k], aq[k], f);
+        }
+      } else {
+        for (int j = 0; j < cnt; ++j) {
+          const float4 f = sf[buf][j];
+          const long long rj = tile + j;
+#pragma unroll
+          for (int k = 0; k < kQpt; ++k) {
+            const float v = gauss_gram_term(ax[k], ay[k], aq[k], f);
+            s[k] += (rj == q_begin + t + k * kBlock) ? 0.0f : v;
+          }
+        }
       }
     } else {
-      for (int j = 0; j < cnt; ++j) {
-        const float4 a = s0[buf][j];
-        const float4 b = (kNf == 2) ? s1[buf][j] : a;
-        const long long rj = tile + j;
+      if (!diag) {
+#pragma unroll 4
+        for (int j = 0; j < cnt; ++j) {
+          const float4 f = sf[buf][j];
 #pragma unroll
-        for (int k = 0; k < kQpt; ++k) {
-          const float v = pair_term<FAM, ISO>(px[k], py[k], a, b, rho);
-          s[k] += (rj == q_begin + t + k * kBlock) ? 0.0f : v;
+          for (int k = 0; k < kQpt; ++k) s[k] += gauss_direct_term<ISO>(px[k], py[k], f, rho);
+        }
+      } else {
+        for (int j = 0; j < cnt; ++j) {
+          const float4 f = sf[buf][j];
+          const long long rj = tile + j;
+#pragma unroll
+          for (int k = 0; k < kQpt; ++k) {
+            const float v = gauss_direct_term<ISO>(px[k], py[k], f, rho);
+            s[k] += (rj == q_begin + t + k * kBlock) ? 0.0f : v;
+          }
         }
       }
     }
@@ -166,7 +184,6 @@ __device__ __forceinline__ void direct_f32_item(
     for (int k = 0; k < kQpt; ++k) acc[k] += (double)s[k];
     buf ^= 1;
   }
-
 #pragma unroll
   for (int k = 0; k < kQpt; ++k) {
     const int qi = t + k * kBlock;
@@ -174,71 +191,260 @@ __device__ __forceinline__ void direct_f32_item(
   }
 }
 
-template <int FAM, bool ISO>
-__global__ __launch_bounds__(kBlock) void direct_f32_range_kernel(
+template <bool ISO>
+__global__ __launch_bounds__(kBlock) void gauss_range_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
-    const float4 *__restrict__ f0, const float4 *__restrict__ f1,
-    long long r_i, long long r_end, long long split_len, long long q_i,
-    long long nq, double *__restrict__ dst, double out_scale, int self_excl,
-    float rho) {
+    const float4 *__restrict__ fd, const float4 *__restrict__ fg,
+    const float4 *__restrict__ tiles, long long r_i, long long r_end,
+    long long split_len, long long q_i, long long nq, double *__restrict__ dst,
+    double out_scale, int self_excl, float rho, float gx, float gy, int mode) {
   const long long qb = (long long)blockIdx.x * kQueriesPerBlock;
-  const long long rb = r_i + (long long)blockIdx.y * split_len;
-  long long re = rb + split_len;
-  if (re > r_end) re = r_end;
+  long long rb, re;
+  split_bounds(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
   const long long left = nq - qb;
   const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
-  direct_f32_item<FAM, ISO>(qx, qy, f0, f1, rb, re, q_i + qb, q_count,
-                            dst + (long long)blockIdx.y * nq + qb, out_scale,
-                            self_excl, rho);
+  gauss_item<ISO>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count,
+                  dst + (long long)blockIdx.y * nq + qb, out_scale, self_excl, rho, gx, gy, mode);
 }
 
-template <int FAM, bool ISO>
-__global__ __launch_bounds__(kBlock) void direct_f32_list_kernel(
+template <bool ISO>
+__global__ __launch_bounds__(kBlock) void gauss_list_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
-    const float4 *__restrict__ f0, const float4 *__restrict__ f1,
-    const WorkItem *__restrict__ items, double *__restrict__ dst,
-    double out_scale, int self_excl, float rho) {
+    const float4 *__restrict__ fd, const float4 *__restrict__ fg,
+    const float4 *__restrict__ tiles, const WorkItem *__restrict__ items,
+    double *__restrict__ dst, double out_scale, int self_excl, float rho,
+    float gx, float gy, int mode) {
   const WorkItem it = items[blockIdx.x];
-  direct_f32_item<FAM, ISO>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin,
-                            it.q_count, dst + it.out_index, out_scale,
-                            self_excl, rho);
+  gauss_item<ISO>(qx, qy, fd, fg, tiles, it.r_begin, it.r_end, it.q_begin, it.q_count,
+                  dst + it.out_index, out_scale, self_excl, rho, gx, gy, mode);
 }
 
-// Per-reference features, computed in FP64 from the FP32 point data and
-// rounded once.
-__global__ void prep_f32_kernel(EvalParams p, const float *__restrict__ x,
-                                const float *__restrict__ y,
-                                const float *__restrict__ m,
-                                const float *__restrict__ abw, long long n,
-                                double inv_m_scale, float4 *__restrict__ f0,
-                                float4 *__restrict__ f1) {
+// Block-wide reductions over kBlock threads (one value per thread).
+__device__ float block_min_f(float v, float *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = __builtin_fminf(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ float block_max_f(float v, float *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = __builtin_fmaxf(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup per aligned tile of kTile references.
+__global__ __launch_bounds__(kBlock) void prep_gauss_kernel(
+    EvalParams p, const float *__restrict__ x, const float *__restrict__ y,
+    const float *__restrict__ m, const float *__restrict__ abw, long long n,
+    double inv_m_scale, float4 *__restrict__ fd, float4 *__restrict__ fg,
+    float4 *__restrict__ tiles) {
+  __shared__ float sh[kBlock];
+  const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
+  const bool valid = i < n;
+  const float xi = valid ? x[i] : 0.0f;
+  const float yi = valid ? y[i] : 0.0f;
+  const float big = 3.0e38f;
+  const float xmin = block_min_f(valid ? xi : big, sh);
+  const float xmax = block_max_f(valid ? xi : -big, sh);
+  const float ymin = block_min_f(valid ? yi : big, sh);
+  const float ymax = block_max_f(valid ? yi : -big, sh);
+  const float cx = 0.5f * (xmin + xmax);
+  const float cy = 0.5f * (ymin + ymax);
+  float rho2 = 0.0f;
+  if (valid) {
+    const double a = (double)abw[i];
+    const double W = 1.0 / (a * a);
+    const double mass = (double)m[i] * inv_m_scale;
+    const double lm = mass > 0.0 ? log2(mass) : (double)kMaskedLog;
+    fd[i] = make_float4(xi, yi, (float)(p.gx * W), (float)lm);
+    const double rx = (double)xi - (double)cx;
+    const double ry = (double)yi - (double)cy;
+    const double q = p.gx * rx * rx + p.gy * ry * ry;
+    fg[i] = make_float4((float)(2.0 * p.gx * W * rx), (float)(2.0 * p.gy * W * ry), (float)W,
+                        mass > 0.0 ? (float)(lm - W * q) : kMaskedLog);
+    rho2 = (float)(W * q);
+  }
+  const float rmax = block_max_f(rho2, sh);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = make_float4(cx, cy, rmax <= kGramMaxRho2 ? 1.0f : 0.0f, rmax);
+}
+
+// ---- FP64 Epanechnikov kinds on FP32 points ----------------------------------
+//   FAM_EPAN:     f0 = (x, y, w, m),     w = 1/(h a)^2
+//   FAM_EPANPROD: f0 = (x, y, wx, wy), f1.x = m
+//   FAM_EPANCONV: f0 = (x, y, 1/(hx a), 1/(hy a)), f1.x = m * 2^-10
+template <int FAM>
+struct EpanNf {
+  static constexpr int kNf = (FAM == FAM_EPAN) ? 1 : 2;
+};
+
+template <int FAM>
+__device__ __forceinline__ double epan_term(double qx, double qy, double4 a, double4 b) {
+  const double dx = qx - a.x;
+  const double dy = qy - a.y;
+  if constexpr (FAM == FAM_EPAN) {
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    const double t = __builtin_fmax(__builtin_fma(-a.z, d2, 1.0), 0.0);
+    return a.w * t;
+  } else if constexpr (FAM == FAM_EPANPROD) {
+    const double tx = __builtin_fmax(__builtin_fma(-a.z * dx, dx, 1.0), 0.0);
+    const double ty = __builtin_fmax(__builtin_fma(-a.w * dy, dy, 1.0), 0.0);
+    return b.x * (tx * ty);
+  } else {
+    const double ux = __builtin_fabs(dx) * a.z;
+    const double uy = __builtin_fabs(dy) * a.w;
+    const double cx = __builtin_fmax(2.0 - ux, 0.0);
+    const double cy = __builtin_fmax(2.0 - uy, 0.0);
+    const double kx = cx * cx * cx * __builtin_fma(ux, ux + 6.0, 4.0);
+    const double ky = cy * cy * cy * __builtin_fma(uy, uy + 6.0, 4.0);
+    return b.x * (kx * ky);
+  }
+}
+
+template <int FAM>
+__device__ __forceinline__ void epan_item(
+    const float *__restrict__ qx, const float *__restrict__ qy,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    long long r_begin, long long r_end, long long q_begin, int q_count,
+    double *__restrict__ dst, double out_scale, int self_excl) {
+  constexpr int kNf = EpanNf<FAM>::kNf;
+  __shared__ double4 s0[2][kTile];
+  __shared__ double4 s1[2][kNf == 2 ? kTile : 1];
+  const int t = threadIdx.x;
+  if (r_begin >= r_end) {
+    write_zeros(dst, q_count);
+    return;
+  }
+  double px[kQpt], py[kQpt], acc[kQpt];
+#pragma unroll
+  for (int k = 0; k < kQpt; ++k) {
+    const int qi = t + k * kBlock;
+    const long long q = q_begin + (qi < q_count ? qi : 0);
+    px[k] = (double)qx[q];
+    py[k] = (double)qy[q];
+    acc[k] = 0.0;
+  }
+  const long long q_lo = q_begin, q_hi = q_begin + q_count;
+  const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
+  long long tile = (r_begin / kTile) * kTile;
+  double4 n0 = zero, n1 = zero;
+  {
+    const long long r = tile + t;
+    if (r >= r_begin && r < r_end) {
+      n0 = f0[r];
+      if constexpr (kNf == 2) n1 = f1[r];
+    }
+  }
+  int buf = 0;
+  for (; tile < r_end; tile += kTile) {
+    s0[buf][t] = n0;
+    if constexpr (kNf == 2) s1[buf][t] = n1;
+    __syncthreads();
+    {
+      const long long r = tile + kTile + t;
+      n0 = zero;
+      n1 = zero;
+      if (r < r_end) {
+        n0 = f0[r];
+        if constexpr (kNf == 2) n1 = f1[r];
+      }
+    }
+    const int cnt = (int)((r_end - tile) < kTile ? (r_end - tile) : kTile);
+    const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
+    if (!diag) {
+#pragma unroll 2
+      for (int j = 0; j < cnt; ++j) {
+        const double4 a = s0[buf][j];
+        const double4 b = (kNf == 2) ? s1[buf][j] : a;
+#pragma unroll
+        for (int k = 0; k < kQpt; ++k) acc[k] += epan_term<FAM>(px[k], py[k], a, b);
+      }
+    } else {
+      for (int j = 0; j < cnt; ++j) {
+        const double4 a = s0[buf][j];
+        const double4 b = (kNf == 2) ? s1[buf][j] : a;
+        const long long rj = tile + j;
+#pragma unroll
+        for (int k = 0; k < kQpt; ++k) {
+          const double v = epan_term<FAM>(px[k], py[k], a, b);
+          acc[k] += (rj == q_begin + t + k * kBlock) ? 0.0 : v;
+        }
+      }
+    }
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int k = 0; k < kQpt; ++k) {
+    const int qi = t + k * kBlock;
+    if (qi < q_count) dst[qi] = acc[k] * out_scale;
+  }
+}
+
+template <int FAM>
+__global__ __launch_bounds__(kBlock) void epan_range_kernel(
+    const float *__restrict__ qx, const float *__restrict__ qy,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    long long r_i, long long r_end, long long split_len, long long q_i,
+    long long nq, double *__restrict__ dst, double out_scale, int self_excl) {
+  const long long qb = (long long)blockIdx.x * kQueriesPerBlock;
+  long long rb, re;
+  split_bounds(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
+  const long long left = nq - qb;
+  const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
+  epan_item<FAM>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + (long long)blockIdx.y * nq + qb,
+                 out_scale, self_excl);
+}
+
+template <int FAM>
+__global__ __launch_bounds__(kBlock) void epan_list_kernel(
+    const float *__restrict__ qx, const float *__restrict__ qy,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const WorkItem *__restrict__ items, double *__restrict__ dst,
+    double out_scale, int self_excl) {
+  const WorkItem it = items[blockIdx.x];
+  epan_item<FAM>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
+                 out_scale, self_excl);
+}
+
+__global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
+                                 const float *__restrict__ y,
+                                 const float *__restrict__ m,
+                                 const float *__restrict__ abw, long long n,
+                                 double4 *__restrict__ f0,
+                                 double4 *__restrict__ f1) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = (double)abw[i];
   const double ia2 = 1.0 / (a * a);
-  const double mass = (double)m[i] * inv_m_scale;
-  const float xi = x[i], yi = y[i];
+  const double mass = (double)m[i];
+  const double xi = (double)x[i], yi = (double)y[i];
   switch (p.family) {
-    case FAM_GAUSS: {
-      const float lm = mass > 0.0 ? (float)log2(mass) : kMaskedLog;
-      f0[i] = make_float4(xi, yi, (float)(p.gx * ia2), lm);
-      break;
-    }
     case FAM_EPAN:
-      f0[i] = make_float4(xi, yi, (float)(p.gx * ia2), (float)mass);
+      f0[i] = make_double4(xi, yi, p.gx * ia2, mass);
       break;
     case FAM_EPANPROD:
-      f0[i] = make_float4(xi, yi, (float)(p.gx * ia2), (float)(p.gy * ia2));
-      f1[i] = make_float4((float)mass, 0.f, 0.f, 0.f);
+      f0[i] = make_double4(xi, yi, p.gx * ia2, p.gy * ia2);
+      f1[i] = make_double4(mass, 0.0, 0.0, 0.0);
       break;
     default:
-      f0[i] = make_float4(xi, yi, (float)(p.ix / a), (float)(p.iy / a));
-      f1[i] = make_float4((float)(mass * p.mass_fold), 0.f, 0.f, 0.f);
+      f0[i] = make_double4(xi, yi, p.ix / a, p.iy / a);
+      f1[i] = make_double4(mass * p.mass_fold, 0.0, 0.0, 0.0);
       break;
   }
 }
 
-// ---- FP64 path ---------------------------------------------------------
+// ---- FP64 context path ---------------------------------------------------
 constexpr int kBlock64 = 256;
 constexpr int kTile64 = 256;
 
@@ -249,23 +455,10 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   const double dx = qx - a.x;
   const double dy = qy - a.y;
   if constexpr (FAM == FAM_GAUSS) {
-    // a = (x, y, c/(hx a_r)^2, m)
-    return a.w * exp(-a.z * (dx * dx + rho * dy * dy));
-  } else if constexpr (FAM == FAM_EPAN) {
-    const double t = 1.0 - a.z * (dx * dx + rho * dy * dy);
-    return a.w * (t > 0.0 ? t : 0.0);
-  } else if constexpr (FAM == FAM_EPANPROD) {
-    double tx = 1.0 - a.z * dx * dx;
-    double ty = 1.0 - a.w * dy * dy;
-    tx = tx > 0.0 ? tx : 0.0;
-    ty = ty > 0.0 ? ty : 0.0;
-    return b.x * tx * ty;
+    // a = (x, y, gx/a_r^2, m): exp2 of the base-2 exponent
+    return a.w * exp2(-a.z * (dx * dx + rho * dy * dy));
   } else {
-    const double ux = fabs(dx) * a.z, uy = fabs(dy) * a.w;
-    const double cx = ux < 2.0 ? 2.0 - ux : 0.0;
-    const double cy = uy < 2.0 ? 2.0 - uy : 0.0;
-    return b.x * (cx * cx * cx * (ux * ux + 6.0 * ux + 4.0)) *
-           (cy * cy * cy * (uy * uy + 6.0 * uy + 4.0));
+    return epan_term<FAM>(qx, qy, a, b);
   }
 }
 
@@ -275,7 +468,7 @@ __device__ __forceinline__ void direct_f64_item(
     const double4 *__restrict__ f0, const double4 *__restrict__ f1,
     long long r_begin, long long r_end, long long q_begin, int q_count,
     double *__restrict__ dst, double out_scale, int self_excl, double rho) {
-  constexpr int kNf = Fam<FAM>::kNf;
+  constexpr int kNf = (FAM == FAM_EPANPROD || FAM == FAM_EPANCONV) ? 2 : 1;
   __shared__ double4 s0[kTile64];
   __shared__ double4 s1[kNf == 2 ? kTile64 : 1];
   const int t = threadIdx.x;
@@ -327,7 +520,6 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_list_kernel(
     const WorkItem *__restrict__ items, double *__restrict__ dst,
     double out_scale, int self_excl, double rho) {
   const WorkItem it = items[blockIdx.x];
-  // The FP64 kernel covers kBlock64 queries per item.
   direct_f64_item<FAM>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin,
                        it.q_count, dst + it.out_index, out_scale, self_excl,
                        rho);
@@ -345,7 +537,7 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
   const double ia2 = 1.0 / (a * a);
   switch (p.family) {
     case FAM_GAUSS:
-      f0[i] = make_double4(x[i], y[i], p.c * p.gx * ia2, m[i]);
+      f0[i] = make_double4(x[i], y[i], p.gx * ia2, m[i]);
       break;
     case FAM_EPAN:
       f0[i] = make_double4(x[i], y[i], p.gx * ia2, m[i]);
@@ -361,48 +553,57 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
   }
 }
 
-// ---- general dimension (FP32, 3 <= D <= 16) -------------------------------
+// ---- general dimension (3 <= D <= 16), FP64 distances -----------------------
 constexpr int kBlockG = 256;
 constexpr int kTileG = 128;
+
+// m * 2^e for an FP64 base-2 exponent e <= 0: exp2 of the fractional part in
+// FP32 (|f| <= 1/2, so its FP32 rounding costs < 2^-25 absolute), the
+// integer part applied exactly by ldexp.
+__device__ __forceinline__ double exp2_split(double e) {
+  double n = rint(e);
+  if (n < -1000.0) n = -1000.0;
+  const float f = (float)(e - n);
+  return (double)__builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
 
 template <int D, int FAM>
 __device__ __forceinline__ void direct_general_item(
     const float *__restrict__ qc, const float *__restrict__ rc,
-    const float2 *__restrict__ wm, long long r_begin, long long r_end,
+    const double2 *__restrict__ wm, long long r_begin, long long r_end,
     long long q_begin, int q_count, double *__restrict__ dst,
     double out_scale, int self_excl) {
-  __shared__ float sc[kTileG * D];
-  __shared__ float2 sw[kTileG];
+  __shared__ double sc[kTileG * D];
+  __shared__ double2 sw[kTileG];
   const int t = threadIdx.x;
   const bool valid = t < q_count;
   const long long q = q_begin + (valid ? t : 0);
-  float pq[D];
+  double pq[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) pq[d] = qc[q * D + d];
+  for (int d = 0; d < D; ++d) pq[d] = (double)qc[q * D + d];
   double acc = 0.0;
   for (long long tile = r_begin; tile < r_end; tile += kTileG) {
     const int cnt = (int)((r_end - tile) < kTileG ? (r_end - tile) : kTileG);
-    for (int e = t; e < cnt * D; e += kBlockG) sc[e] = rc[tile * D + e];
+    for (int e = t; e < cnt * D; e += kBlockG) sc[e] = (double)rc[tile * D + e];
     for (int e = t; e < cnt; e += kBlockG) sw[e] = wm[tile + e];
     __syncthreads();
-    float s = 0.0f;
     for (int j = 0; j < cnt; ++j) {
-      float d2 = 0.0f;
+      double d2 = 0.0;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const float df = pq[d] - sc[j * D + d];
-        d2 = __builtin_fmaf(df, df, d2);
+        const double df = pq[d] - sc[j * D + d];
+        d2 = __builtin_fma(df, df, d2);
       }
-      const float2 w = sw[j];
-      float v;
+      const double2 w = sw[j];
+      double v;
       if constexpr (FAM == FAM_GAUSS) {
-        v = __builtin_amdgcn_exp2f(__builtin_fmaf(-w.x, d2, w.y));
+        // w = (gx / a^2 in base-2 exponent units, mass)
+        v = w.y * exp2_split(-w.x * d2);
       } else {
-        v = w.y * clamp01(__builtin_fmaf(-w.x, d2, 1.0f));
+        v = w.y * __builtin_fmax(__builtin_fma(-w.x, d2, 1.0), 0.0);
       }
-      if (!(self_excl && tile + j == q)) s += v;
+      if (!(self_excl && tile + j == q)) acc += v;
     }
-    acc += (double)s;
     __syncthreads();
   }
   if (valid) dst[t] = acc * out_scale;
@@ -411,7 +612,7 @@ __device__ __forceinline__ void direct_general_item(
 template <int D, int FAM>
 __global__ __launch_bounds__(kBlockG) void direct_general_range_kernel(
     const float *__restrict__ qc, const float *__restrict__ rc,
-    const float2 *__restrict__ wm, long long r_i, long long r_end,
+    const double2 *__restrict__ wm, long long r_i, long long r_end,
     long long split_len, long long q_i, long long nq, double *__restrict__ dst,
     double out_scale, int self_excl) {
   const long long qb = (long long)blockIdx.x * kBlockG;
@@ -428,7 +629,7 @@ __global__ __launch_bounds__(kBlockG) void direct_general_range_kernel(
 template <int D, int FAM>
 __global__ __launch_bounds__(kBlockG) void direct_general_list_kernel(
     const float *__restrict__ qc, const float *__restrict__ rc,
-    const float2 *__restrict__ wm, const WorkItem *__restrict__ items,
+    const double2 *__restrict__ wm, const WorkItem *__restrict__ items,
     double *__restrict__ dst, double out_scale, int self_excl) {
   const WorkItem it = items[blockIdx.x];
   direct_general_item<D, FAM>(qc, rc, wm, it.r_begin, it.r_end, it.q_begin,
@@ -438,21 +639,70 @@ __global__ __launch_bounds__(kBlockG) void direct_general_list_kernel(
 
 __global__ void prep_general_kernel(EvalParams p, const float *__restrict__ abw,
                                     const float *__restrict__ m, long long n,
-                                    double inv_m_scale,
-                                    float2 *__restrict__ wm) {
+                                    double2 *__restrict__ wm) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = (double)abw[i];
-  const double w = p.gx / (a * a);
-  const double mass = (double)m[i] * inv_m_scale;
-  if (p.family == FAM_GAUSS) {
-    wm[i] = make_float2((float)w, mass > 0.0 ? (float)log2(mass) : kMaskedLog);
-  } else {
-    wm[i] = make_float2((float)w, (float)mass);
+  wm[i] = make_double2(p.gx / (a * a), (double)m[i]);
+}
+
+// ---- FP64 rescue of underflowed Gaussian sums ---------------------------------
+__global__ void zero_int_kernel(int *p) { *p = 0; }
+
+__global__ void flag_small_kernel(const double *__restrict__ S, long long nq,
+                                  double threshold, int *__restrict__ scratch) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  if (S[k] < threshold) {
+    const int slot = atomicAdd(scratch, 1);
+    scratch[1 + slot] = (int)k;
   }
 }
 
-// ---- reductions ----------------------------------------------------------
+constexpr int kRescueThreads = 256;
+
+__device__ double block_sum_d(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kRescueThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kRescueThreads) void rescue_gauss_kernel(
+    EvalParams p, const float *__restrict__ x, const float *__restrict__ y,
+    const float *__restrict__ m, const float *__restrict__ abw, long long n_ref,
+    const float *__restrict__ qx, const float *__restrict__ qy, long long q_i,
+    double *__restrict__ S, int self_excl, const int *__restrict__ scratch) {
+  __shared__ double sh[kRescueThreads];
+  const int count = scratch[0];
+  for (int k = blockIdx.x; k < count; k += gridDim.x) {
+    const long long qk = scratch[1 + k];
+    const long long qa = q_i + qk;
+    const double px = (double)qx[qa], py = (double)qy[qa];
+    double s = 0.0;
+    for (long long r = threadIdx.x; r < n_ref; r += kRescueThreads) {
+      if (self_excl && r == qa) continue;
+      const double dx = px - (double)x[r];
+      const double dy = py - (double)y[r];
+      const double a = (double)abw[r];
+      s += (double)m[r] * exp2(-(p.gx * dx * dx + p.gy * dy * dy) / (a * a));
+    }
+    const double tot = block_sum_d(s, sh);
+    if (threadIdx.x == 0) S[qk] = tot;
+  }
+}
+
+// ---- reductions ---------------------------------------------------------------
+__global__ void scale_kernel(double *__restrict__ v, long long n, double a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= a;
+}
+
 __global__ void reduce_splits_kernel(const double *__restrict__ partial,
                                      int splits, long long nq,
                                      double *__restrict__ out) {
@@ -465,21 +715,11 @@ __global__ void reduce_splits_kernel(const double *__restrict__ partial,
 
 constexpr int kReduceThreads = 256;
 
-__device__ double block_sum(double v) {
-  __shared__ double sh[kReduceThreads];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = kReduceThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  return sh[0];
-}
-
 __global__ __launch_bounds__(kReduceThreads) void weighted_reduce_kernel(
     int mode, const double *__restrict__ w, const double *__restrict__ u,
     const double *__restrict__ v, double a, double b, long long n,
     double *__restrict__ partial) {
+  __shared__ double sh[kReduceThreads];
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * kReduceThreads + threadIdx.x; i < n;
        i += (long long)gridDim.x * kReduceThreads) {
@@ -489,15 +729,16 @@ __global__ __launch_bounds__(kReduceThreads) void weighted_reduce_kernel(
       s += w[i] * (a * u[i] - b * v[i]);
     }
   }
-  const double tot = block_sum(s);
+  const double tot = block_sum_d(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(kReduceThreads) void final_reduce_kernel(
     const double *__restrict__ partial, int n, double *__restrict__ result) {
+  __shared__ double sh[kReduceThreads];
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += kReduceThreads) s += partial[i];
-  const double tot = block_sum(s);
+  const double tot = block_sum_d(s, sh);
   if (threadIdx.x == 0) *result = tot;
 }
 
@@ -511,88 +752,98 @@ __global__ void adapt_lambda_kernel(const double *__restrict__ S, long long n,
   lambda[i] = pow(norm * S[i] / g, -alpha);
 }
 
-long long split_length(long long nr, int splits, int tile) {
-  long long len = (nr + splits - 1) / splits;
+// Split length: a multiple of `tile`, counted from the tile boundary at or
+// below r_i.
+long long split_length(long long r_i, long long r_end, int splits, int tile) {
+  const long long base = (r_i / tile) * tile;
+  long long len = (r_end - base + splits - 1) / splits;
   len = (len + tile - 1) / tile * tile;
   return len;
 }
 
 }  // namespace
 
-// ---- launchers -----------------------------------------------------------
+// ---- launchers -----------------------------------------------------------------
 
-hipError_t launch_prep_f32(const EvalParams &p, const float *x, const float *y,
-                           const float *m, const float *abw, size_t n,
-                           double m_scale, float4 *f0, float4 *f1,
-                           hipStream_t s) {
+hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
+                             const float *m, const float *abw, size_t n,
+                             double m_scale, float4 *fd, float4 *fg,
+                             float4 *tiles, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned ntiles = (unsigned)((n + kTile - 1) / kTile);
+  prep_gauss_kernel<<<ntiles, kBlock, 0, s>>>(p, x, y, m, abw, (long long)n, 1.0 / m_scale, fd, fg, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
+                        const float4 *fd, const float4 *fg, const float4 *tiles,
+                        const WorkItem *items, int n_items, long long r_i,
+                        long long r_end, long long q_i, long long nq,
+                        int splits, double *dst, double out_scale,
+                        int self_excl, int mode, hipStream_t s) {
+  const float rho = (float)(p.gy / p.gx);
+  const float gx = (float)p.gx, gy = (float)p.gy;
+  const bool iso = p.iso != 0;
+  if (items) {
+    if (n_items <= 0) return hipSuccess;
+    if (iso)
+      gauss_list_kernel<true><<<n_items, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, items, dst, out_scale, self_excl,
+                                                         rho, gx, gy, mode);
+    else
+      gauss_list_kernel<false><<<n_items, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, items, dst, out_scale, self_excl,
+                                                          rho, gx, gy, mode);
+  } else {
+    if (nq <= 0) return hipSuccess;
+    const long long len = split_length(r_i, r_end, splits, kTile);
+    const dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
+    if (iso)
+      gauss_range_kernel<true><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst,
+                                                       out_scale, self_excl, rho, gx, gy, mode);
+    else
+      gauss_range_kernel<false><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst,
+                                                        out_scale, self_excl, rho, gx, gy, mode);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
+                            const float *m, const float *abw, size_t n,
+                            double4 *f0, double4 *f1, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int th = 256;
-  const unsigned blocks = (unsigned)((n + th - 1) / th);
-  prep_f32_kernel<<<blocks, th, 0, s>>>(p, x, y, m, abw, (long long)n,
-                                        1.0 / m_scale, f0, f1);
+  prep_epan_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, x, y, m, abw, (long long)n, f0, f1);
   return hipGetLastError();
 }
 
-hipError_t launch_direct_f32_range(const EvalParams &p, const float *qx,
-                                   const float *qy, const float4 *f0,
-                                   const float4 *f1, long long r_i,
-                                   long long r_end, long long q_i,
-                                   long long nq, int splits, double *dst,
-                                   double out_scale, int self_excl,
-                                   hipStream_t s) {
-  if (nq <= 0) return hipSuccess;
-  const long long len = split_length(r_end - r_i, splits, kTile);
-  const dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock),
-                  (unsigned)splits);
-  const float rho = (float)(p.gy / p.gx);
-  const bool iso = p.iso != 0;
-#define BK_RANGE(F, I)                                                        \
-  direct_f32_range_kernel<F, I><<<grid, kBlock, 0, s>>>(                      \
-      qx, qy, f0, f1, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho)
-  switch (p.family) {
-    case FAM_GAUSS:
-      if (iso) BK_RANGE(FAM_GAUSS, true); else BK_RANGE(FAM_GAUSS, false);
-      break;
-    case FAM_EPAN:
-      if (iso) BK_RANGE(FAM_EPAN, true); else BK_RANGE(FAM_EPAN, false);
-      break;
-    case FAM_EPANPROD:
-      BK_RANGE(FAM_EPANPROD, false);
-      break;
-    default:
-      BK_RANGE(FAM_EPANCONV, false);
-      break;
+hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
+                       const double4 *f0, const double4 *f1,
+                       const WorkItem *items, int n_items, long long r_i,
+                       long long r_end, long long q_i, long long nq, int splits,
+                       double *dst, double out_scale, int self_excl,
+                       hipStream_t s) {
+  if (items) {
+    if (n_items <= 0) return hipSuccess;
+#define BK_EL(F) epan_list_kernel<F><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, items, dst, out_scale, self_excl)
+    switch (p.family) {
+      case FAM_EPAN: BK_EL(FAM_EPAN); break;
+      case FAM_EPANPROD: BK_EL(FAM_EPANPROD); break;
+      default: BK_EL(FAM_EPANCONV); break;
+    }
+#undef BK_EL
+  } else {
+    if (nq <= 0) return hipSuccess;
+    const long long len = split_length(r_i, r_end, splits, kTile);
+    const dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
+#define BK_ER(F)                                                                                 \
+  epan_range_kernel<F><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, r_i, r_end, len, q_i, nq, dst, out_scale, \
+                                               self_excl)
+    switch (p.family) {
+      case FAM_EPAN: BK_ER(FAM_EPAN); break;
+      case FAM_EPANPROD: BK_ER(FAM_EPANPROD); break;
+      default: BK_ER(FAM_EPANCONV); break;
+    }
+#undef BK_ER
   }
-#undef BK_RANGE
-  return hipGetLastError();
-}
-
-hipError_t launch_direct_f32_list(const EvalParams &p, const float *qx,
-                                  const float *qy, const float4 *f0,
-                                  const float4 *f1, const WorkItem *items,
-                                  int n_items, double *dst, double out_scale,
-                                  int self_excl, hipStream_t s) {
-  if (n_items <= 0) return hipSuccess;
-  const float rho = (float)(p.gy / p.gx);
-  const bool iso = p.iso != 0;
-#define BK_LIST(F, I)                                                         \
-  direct_f32_list_kernel<F, I><<<n_items, kBlock, 0, s>>>(                    \
-      qx, qy, f0, f1, items, dst, out_scale, self_excl, rho)
-  switch (p.family) {
-    case FAM_GAUSS:
-      if (iso) BK_LIST(FAM_GAUSS, true); else BK_LIST(FAM_GAUSS, false);
-      break;
-    case FAM_EPAN:
-      if (iso) BK_LIST(FAM_EPAN, true); else BK_LIST(FAM_EPAN, false);
-      break;
-    case FAM_EPANPROD:
-      BK_LIST(FAM_EPANPROD, false);
-      break;
-    default:
-      BK_LIST(FAM_EPANCONV, false);
-      break;
-  }
-#undef BK_LIST
   return hipGetLastError();
 }
 
@@ -629,7 +880,8 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
 #undef BK_L64
   } else {
     if (nq <= 0) return hipSuccess;
-    const long long len = split_length(r_end - r_i, splits, kTile64);
+    long long len = (r_end - r_i + splits - 1) / splits;
+    len = (len + kTile64 - 1) / kTile64 * kTile64;
     const dim3 grid((unsigned)((nq + kBlock64 - 1) / kBlock64), (unsigned)splits);
 #define BK_R64(F)                                                             \
   direct_f64_range_kernel<F><<<grid, kBlock64, 0, s>>>(                       \
@@ -646,18 +898,17 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
 }
 
 hipError_t launch_prep_general(const EvalParams &p, const float *abw,
-                               const float *m, size_t n, double m_scale,
-                               float2 *wm, hipStream_t s) {
+                               const float *m, size_t n, double2 *wm,
+                               hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int th = 256;
-  prep_general_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(
-      p, abw, m, (long long)n, 1.0 / m_scale, wm);
+  prep_general_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, abw, m, (long long)n, wm);
   return hipGetLastError();
 }
 
 namespace {
 template <int D, int FAM>
-void general_dispatch(const float *qc, const float *rc, const float2 *wm,
+void general_dispatch(const float *qc, const float *rc, const double2 *wm,
                       const WorkItem *items, int n_items, long long r_i,
                       long long r_end, long long q_i, long long nq, int splits,
                       double *dst, double out_scale, int self_excl,
@@ -666,7 +917,8 @@ void general_dispatch(const float *qc, const float *rc, const float2 *wm,
     direct_general_list_kernel<D, FAM><<<n_items, kBlockG, 0, s>>>(
         qc, rc, wm, items, dst, out_scale, self_excl);
   } else {
-    const long long len = split_length(r_end - r_i, splits, kTileG);
+    long long len = (r_end - r_i + splits - 1) / splits;
+    len = (len + kTileG - 1) / kTileG * kTileG;
     const dim3 grid((unsigned)((nq + kBlockG - 1) / kBlockG), (unsigned)splits);
     direct_general_range_kernel<D, FAM><<<grid, kBlockG, 0, s>>>(
         qc, rc, wm, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl);
@@ -675,7 +927,7 @@ void general_dispatch(const float *qc, const float *rc, const float2 *wm,
 
 template <int D>
 void general_dispatch_fam(int fam, const float *qc, const float *rc,
-                          const float2 *wm, const WorkItem *items, int n_items,
+                          const double2 *wm, const WorkItem *items, int n_items,
                           long long r_i, long long r_end, long long q_i,
                           long long nq, int splits, double *dst,
                           double out_scale, int self_excl, hipStream_t s) {
@@ -690,7 +942,7 @@ void general_dispatch_fam(int fam, const float *qc, const float *rc,
 }  // namespace
 
 hipError_t launch_direct_general(const EvalParams &p, const float *qc,
-                                 const float *rc, const float2 *wm,
+                                 const float *rc, const double2 *wm,
                                  const WorkItem *items, int n_items,
                                  long long r_i, long long r_end, long long q_i,
                                  long long nq, int splits, double *dst,
@@ -709,6 +961,28 @@ hipError_t launch_direct_general(const EvalParams &p, const float *qc,
       return hipErrorInvalidValue;
   }
 #undef BK_GD
+  return hipGetLastError();
+}
+
+hipError_t launch_rescue_gauss(const EvalParams &p, const float *x,
+                               const float *y, const float *m, const float *abw,
+                               long long n_ref, const float *qx, const float *qy,
+                               long long q_i, long long nq, double *S,
+                               double threshold, int self_excl, int *scratch,
+                               hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  zero_int_kernel<<<1, 1, 0, s>>>(scratch);
+  const int th = 256;
+  flag_small_kernel<<<(unsigned)((nq + th - 1) / th), th, 0, s>>>(S, nq, threshold, scratch);
+  rescue_gauss_kernel<<<1024, kRescueThreads, 0, s>>>(p, x, y, m, abw, n_ref, qx, qy, q_i, S, self_excl,
+                                                      scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(double *v, long long n, double a, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int th = 256;
+  scale_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(v, n, a);
   return hipGetLastError();
 }
 

@@ -46,6 +46,8 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
     ap.add_argument("--kernels", default="gauss,epanprod", help="comma list: gauss, epanprod")
+    ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
+                    help="Gaussian exponent form (auto: tile-centred on compact tiles)")
     return ap.parse_args()
 
 
@@ -68,7 +70,9 @@ def time_kernel(kde, desc, steps: int, warmup: int):
     import torch
 
     out = torch.empty(N_QUERY, dtype=torch.float64, device="cuda")
-    stream = torch.cuda.current_stream()
+    # a stream of our own (default priority): the library launches on it and
+    # the events are recorded on it
+    stream = torch.cuda.Stream()
     kde.set_stream(stream.cuda_stream)
     for _ in range(warmup):
         kde.eval_device(desc, 0, N_REF - 1, 0, N_QUERY - 1, out.data_ptr(), True)
@@ -83,6 +87,7 @@ def time_kernel(kde, desc, steps: int, warmup: int):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    kde.set_stream(0)
     return out.cpu().numpy(), ms, wall
 
 
@@ -101,7 +106,8 @@ def cpu_baseline(pts, mass, q, gpu_vals, seconds: float):
     t0 = time.perf_counter()
     want = orc.direct(2, H_GAUSS, H_GAUSS, P, ds.round_f32(q[idx]), mass=M, normalize=True, threads=threads)
     dt = time.perf_counter() - t0
-    floor = 1e-12 * float(np.abs(want).max())
+    # floor: 1e-10 of the density's upper bound C * sum(m) (U <= 1)
+    floor = 1e-10 * (1.0 / (2.0 * np.pi * H_GAUSS ** 2)) * float(M.sum())
     above = np.abs(want) >= floor
     rel = np.abs(gpu_vals[idx][above] - want[above]) / np.abs(want[above])
     result["parity"] = {"oracle": "FP64 restatement", "queries": int(len(idx)), "max_rel_err": float(rel.max()),
@@ -147,6 +153,8 @@ def main() -> int:
     kinds = [k.strip() for k in args.kernels.split(",") if k.strip()]
     results = {}
     with bk.DirectKde(pts, q, mass) as kde:
+        if args.gauss_form == "direct":
+            kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         for k in kinds:
             if k == "gauss":
                 desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS

@@ -61,7 +61,7 @@ extern "C" {
 
 /* Kernel families: one per kernel class template of the reference
  * (include/Kernels/*.h).  `dim` of the descriptor must be 1 or 2, except for
- * BK_GAUSSIAN and BK_EPANECHNIKOV, which accept any D from 1 to 64. */
+ * BK_GAUSSIAN and BK_EPANECHNIKOV, which accept any D from 1 to 16. */
 #define BK_EPANECHNIKOV                 1 /* EpanechnikovKernel<D,T>          */
 #define BK_GAUSSIAN                     2 /* GaussianKernel<D,T>              */
 #define BK_EPANECHNIKOV_PRODUCT_2D      3 /* EpanechnikovProductKernel2d<T>   */
@@ -70,6 +70,11 @@ extern "C" {
 #define BK_EPANECHNIKOV_PRODUCT_CONV_2D 6 /* EpanechnikovProductConvKernel2d<T> */
 #define BK_GAUSSIAN_CONV_1D             7 /* GaussianConvKernel1d<T>          */
 #define BK_GAUSSIAN_PRODUCT_CONV_2D     8 /* GaussianProductConvKernel2d<T>   */
+
+/* Options of bk_set_option. */
+#define BK_OPT_GAUSS_FORM      1  /* how FP32 Gaussian exponents are formed:         */
+#define BK_GAUSS_FORM_AUTO     0  /*   tile-centred form on compact tiles (default)  */
+#define BK_GAUSS_FORM_DIRECT   1  /*   differences q - r on every tile               */
 
 /* ---- POD types -------------------------------------------------------- */
 typedef struct bk_ctx bk_ctx;
@@ -108,6 +113,10 @@ const char *bk_last_error(void);
 /* Make the context run its work on `stream` (a hipStream_t); NULL selects
  * the context's own stream.  The reference uses the default stream only. */
 int bk_set_stream(bk_ctx *ctx, void *stream);
+
+/* Set a context option (BK_OPT_*).  The reference has no counterpart: it
+ * lets callers choose between evaluation forms of equal accuracy class. */
+int bk_set_option(bk_ctx *ctx, int option, int value);
 
 /* Upload point set `set` (BK_SET_REF or BK_SET_QUERY): n points of dimension
  * dim, coords row-major (coords[i*dim + d]), one mass and one abw per point.

@@ -18,7 +18,8 @@ from bbrcit_kde_amd import datasets as ds
 
 pytestmark = pytest.mark.gpu
 
-FLOOR_REL = 1e-12
+# Floor: a fraction of an upper bound of the sum (C * sum of masses, U <= 1).
+FLOOR_REL = 1e-10
 RTOL_F32 = 1e-5
 
 EPAN, GAUSS, EPANPROD, GAUSSPROD, EPANCONV1, EPANPRODCONV, GAUSSCONV1, GAUSSPRODCONV = range(1, 9)
@@ -28,11 +29,11 @@ BW = {EPAN: (0.35, 0.35), GAUSS: (0.12, 0.12), EPANPROD: (0.3, 0.45), GAUSSPROD:
       EPANCONV1: (0.3, 0.3), EPANPRODCONV: (0.3, 0.45), GAUSSCONV1: (0.12, 0.12), GAUSSPRODCONV: (0.1, 0.16)}
 
 
-def assert_parity(got, want, rtol=RTOL_F32, what=""):
+def assert_parity(got, want, bound, rtol=RTOL_F32, what=""):
+    """bound: an upper bound of every exact value (C * sum of masses)."""
     got = np.asarray(got)
     want = np.asarray(want)
-    scale = np.abs(want).max() if want.size else 0.0
-    floor = FLOOR_REL * scale
+    floor = FLOOR_REL * bound
     above = np.abs(want) >= floor
     if above.any():
         rel = np.abs(got[above] - want[above]) / np.abs(want[above])
@@ -64,7 +65,8 @@ def test_direct_f32_matches_oracle(bk, oracle, kind, dim, adaptive):
     with bk.DirectKde(pts, q, mass, abw) as kde:
         got = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
     want = oracle.direct(kind, hx, hy, f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
-    assert_parity(got, want, what=f"kind {kind} dim {dim} adaptive {adaptive}")
+    bound = oracle.normalization(kind, dim, hx, hy) * mass.sum()
+    assert_parity(got, want, bound, what=f"kind {kind} dim {dim} adaptive {adaptive}")
 
 
 @pytest.mark.parametrize("kind,dim", KINDS)
@@ -75,7 +77,7 @@ def test_direct_f64_matches_oracle(bk, oracle, kind, dim):
     with bk.DirectKde(pts, q, mass, abw, precision=bk.BK_PREC_F64) as kde:
         got = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
     want = oracle.direct(kind, hx, hy, pts, q, mass=mass, abw=abw)
-    assert_parity(got, want, rtol=1e-12, what=f"f64 kind {kind}")
+    assert_parity(got, want, mass.sum(), rtol=1e-12, what=f"f64 kind {kind}")
 
 
 def test_index_subranges_like_reference_test(bk, oracle):
@@ -90,7 +92,7 @@ def test_index_subranges_like_reference_test(bk, oracle):
             got = kde.unnormalized_eval(desc, r_i, r_j, q_i, q_j)
             want = oracle.direct(GAUSS, 0.1, 0.1, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
             assert got.shape == (q_j - q_i + 1,)
-            assert_parity(got, want, what=f"range {(r_i, r_j, q_i, q_j)}")
+            assert_parity(got, want, M[r_i:r_j + 1].sum(), what=f"range {(r_i, r_j, q_i, q_j)}")
 
 
 def test_range_errors_match_reference(bk):
@@ -119,7 +121,7 @@ def test_ragged_sizes(bk, oracle, n_ref, n_q):
     with bk.DirectKde(pts, q, mass, abw) as kde:
         got = kde.unnormalized_eval(desc, 0, n_ref - 1, 0, n_q - 1)
     want = oracle.direct(GAUSSPROD, 0.08, 0.12, f32(pts), f32(q), mass=f32(mass), abw=f32(abw))
-    assert_parity(got, want, what=f"ragged {n_ref}x{n_q}")
+    assert_parity(got, want, mass.sum(), what=f"ragged {n_ref}x{n_q}")
 
 
 def test_results_are_deterministic(bk):
@@ -164,7 +166,7 @@ def test_general_dimension(bk, oracle, kind, dim):
     with bk.DirectKde(pts, q, mass, abw) as kde:
         got = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
     want = oracle.direct(kind, h, h, f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
-    assert_parity(got, want, what=f"D={dim}")
+    assert_parity(got, want, oracle.normalization(kind, dim, h, h) * mass.sum(), what=f"D={dim}")
 
 
 def test_mixture_1m_refs_sampled_queries(bk, oracle):
@@ -181,4 +183,28 @@ def test_mixture_1m_refs_sampled_queries(bk, oracle):
             got = kde.eval(desc, 0, len(pts) - 1, q_i, q_i + 511)
             want = oracle.direct(GAUSS, 0.05, 0.05, f32(pts), f32(q[q_i:q_i + 512]), mass=f32(mass),
                                  normalize=True)
-            assert_parity(got, want, what=f"1M sample at {q_i}")
+            assert_parity(got, want, oracle.normalization(GAUSS, 2, 0.05, 0.05), what=f"1M sample at {q_i}")
+
+
+@pytest.mark.parametrize("kind,hx,hy", [(GAUSS, 0.05, 0.05), (GAUSSPROD, 0.04, 0.07), (GAUSSCONV1, 0.05, 0.05)])
+def test_gaussian_forms_agree_with_oracle(bk, oracle, kind, hx, hy):
+    # The tile-centred form (compact tiles) and the difference form must both
+    # meet the tolerance; kd-ordered mixture points give mostly compact tiles.
+    dim = 1 if kind == GAUSSCONV1 else 2
+    pts, w = ds.gaussian_mixture(60000, seed=91)
+    perm = ds.kd_order(pts, 64)
+    pts, w = pts[perm][:, :dim], w[perm]
+    mass = w / w.sum()
+    rng = np.random.default_rng(5)
+    abw = rng.uniform(0.7, 1.4, len(pts))
+    q = ds.uniform_box(3000, -2.5, 2.5, 92)[:, :dim]
+    desc = bk.KernelDesc(kind, dim, hx, hy)
+    want = oracle.direct(kind, hx, hy, f32(pts), f32(q), mass=f32(mass), abw=f32(abw))
+    with bk.DirectKde(pts, q, mass, abw) as kde:
+        auto = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
+        direct = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        with pytest.raises(ValueError):
+            kde.set_option(bk.BK_OPT_GAUSS_FORM, 7)
+    assert_parity(auto, want, mass.sum(), what="tile-centred form")
+    assert_parity(direct, want, mass.sum(), what="difference form")
