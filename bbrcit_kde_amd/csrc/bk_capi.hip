@@ -402,10 +402,14 @@ Path path_of(const bk_ctx *ctx, const EvalParams &p) {
 
 // Queries handled by one workgroup / work item on a path.
 int queries_per_item(Path path) {
-  return (path == Path::GAUSS32 || path == Path::EPAN64) ? bk::kQueriesPerBlock : 256;
+  if (path == Path::GAUSS32) return bk::kGaussQueriesPerBlock;
+  return path == Path::EPAN64 ? bk::kQueriesPerBlock : 256;
 }
 
-int tile_of(Path path) { return path == Path::GENERAL ? 128 : 256; }
+int tile_of(Path path) {
+  if (path == Path::GAUSS32) return bk::kGaussTile;
+  return path == Path::GENERAL ? 128 : 256;
+}
 
 // Make sure the per-reference features of `desc` are on the device.
 void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &p) {
@@ -424,7 +428,7 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
                 "prep_f64 launch");
       break;
     case Path::GAUSS32: {
-      const size_t ntiles = (n + bk::kTile - 1) / bk::kTile;
+      const size_t ntiles = (n + bk::kGaussTile - 1) / bk::kGaussTile;
       ctx->f0.reserve(n * sizeof(float4));
       ctx->f1.reserve(n * sizeof(float4));
       ctx->tiles.reserve(ntiles * sizeof(float4));
@@ -471,7 +475,7 @@ void check_range(const bk_ctx *ctx, size_t r_i, size_t r_j, size_t q_i, size_t q
 
 int choose_splits(long long nq, long long nr, int queries_per_block, int tile) {
   const long long qblocks = (nq + queries_per_block - 1) / queries_per_block;
-  const long long target = 32LL * bk::device_cu_count();
+  const long long target = (long long)bk::kSplitTarget * bk::device_cu_count();
   long long splits = (target + qblocks - 1) / qblocks;
   const long long max_splits = std::max(1LL, nr / (4LL * tile));
   splits = std::max(1LL, std::min(splits, max_splits));

@@ -42,9 +42,29 @@ struct WorkItem {
 
 // Geometry of the direct kernels (D <= 2).
 constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
-constexpr int kQpt = 4;            // queries held in registers per thread
+#ifndef BK_QPT
+#define BK_QPT 4
+#endif
+#ifndef BK_GAUSS_QPT
+#define BK_GAUSS_QPT 2
+#endif
+#ifndef BK_GAUSS_UNROLL
+#define BK_GAUSS_UNROLL 4
+#endif
+constexpr int kQpt = BK_QPT;       // queries held in registers per thread (FP64 kernels)
+constexpr int kGaussQpt = BK_GAUSS_QPT;  // queries per thread of the FP32 Gaussian kernel
 constexpr int kQueriesPerBlock = kBlock * kQpt;
+constexpr int kGaussQueriesPerBlock = kBlock * kGaussQpt;
 constexpr int kTile = 256;         // references per LDS stage (tiles are aligned to kTile)
+#ifndef BK_GAUSS_TILE
+#define BK_GAUSS_TILE 256
+#endif
+#ifndef BK_SPLIT_TARGET
+#define BK_SPLIT_TARGET 256
+#endif
+constexpr int kGaussTile = BK_GAUSS_TILE;  // references per LDS stage of the FP32 Gaussian kernel
+constexpr int kSplitTarget = BK_SPLIT_TARGET;  // workgroups per CU aimed at by reference splitting
+static_assert(kGaussTile <= kBlock, "a Gaussian LDS stage is loaded one reference per thread");
 
 // Gaussian evaluation form selection.
 enum GaussMode : int {
@@ -53,8 +73,8 @@ enum GaussMode : int {
 };
 
 // FP32 Gaussian path, D <= 2 ---------------------------------------------
-// Per reference: fd = direct-form features, fg = Gram-form features; per
-// aligned tile of kTile references: tiles = (centre x, centre y, gram?, 0).
+// Per reference: fd = direct-form features, fg = centred-form features; per
+// aligned tile of kGaussTile references: tiles = (centre x, centre y, form, W).
 hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
                              const float *m, const float *abw, size_t n,
                              double m_scale, float4 *fd, float4 *fg,

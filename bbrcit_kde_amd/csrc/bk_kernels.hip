@@ -36,7 +36,17 @@ namespace bk {
 namespace {
 
 constexpr float kMaskedLog = -1.0e30f;  // base-2 log mass of a masked reference
-constexpr float kGramMaxRho2 = 1.0f;    // tile compactness bound of the Gram form
+constexpr float kGramMaxRho2 = 1.0f;      // compactness bound of the Gram form
+#ifndef BK_FACTORED_RHO2
+#define BK_FACTORED_RHO2 9.0f
+#endif
+constexpr float kFactoredMaxRho2 = BK_FACTORED_RHO2;  // compactness bound of the factored form
+constexpr float kFarExp = 100.0f;         // W Q2 beyond which a tile's terms are < 2^-64
+
+// Evaluation form of an aligned tile (tiles[t].z).
+constexpr float kFormDirect = 0.0f;     // differences q - r
+constexpr float kFormGram = 1.0f;       // tile-centred, per-reference W
+constexpr float kFormFactored = 2.0f;   // tile-centred, one W for the tile (tiles[t].w)
 
 __device__ __forceinline__ float clamp01(float v) {
   return __builtin_fminf(__builtin_fmaxf(v, 0.0f), 1.0f);
@@ -47,10 +57,11 @@ __device__ __forceinline__ void write_zeros(double *dst, int q_count) {
 }
 
 // Reference range of split `sp`: splits start at tile-aligned positions.
+template <int TILE>
 __device__ __forceinline__ void split_bounds(long long r_i, long long r_end,
                                              long long len, int sp,
                                              long long &rb, long long &re) {
-  const long long base = (r_i / kTile) * kTile;
+  const long long base = (r_i / TILE) * TILE;
   rb = base + (long long)sp * len;
   re = rb + len;
   if (rb < r_i) rb = r_i;
@@ -58,19 +69,45 @@ __device__ __forceinline__ void split_bounds(long long r_i, long long r_end,
 }
 
 // ---- FP32 Gaussian ----------------------------------------------------------
-//   fd (direct form) = (x, y, w, lm)        w = gx / a_r^2,  lm = log2(m_r / m_scale)
-//   fg (Gram form)   = (A0, A1, W, C)       relative to the tile centre c:
-//       A0 = 2 gx W (x - cx), A1 = 2 gy W (y - cy), W = 1 / a_r^2,
-//       C = lm - W (gx (x - cx)^2 + gy (y - cy)^2)
-//   exponent(q, r) = lm - W (gx dx^2 + gy dy^2)
-//                  = C + A0 (qx - cx) + A1 (qy - cy) - W (gx (qx - cx)^2 + gy (qy - cy)^2)
+// References are grouped in aligned tiles of kGaussTile points; prep_gauss
+// gives every tile a centre c = (cx, cy), a form and (for the factored
+// form) the tile's common W.  Per reference:
+//   fd (direct form) = (x, y, w, lm)        w = gx W,  W = 1/a_r^2 (FP32-rounded),
+//                                           lm = log2(m_r / m_scale)
+//   fg (centred)     = (A0, A1, W, C)       A0 = 2 gx W (x - cx), A1 = 2 gy W (y - cy),
+//                                           C = lm - W (gx (x - cx)^2 + gy (y - cy)^2)
+// The base-2 exponent of the term m_r U(q - r) is
+//   lm - W (gx dx^2 + gy dy^2) = C + A0 ax + A1 ay - W Q2,
+//   ax = qx - cx, ay = qy - cy, Q2 = gx ax^2 + gy ay^2.
+// Forms (rho^2 = max_r W (gx (x-cx)^2 + gy (y-cy)^2), the tile's radius in
+// bandwidth units):
+//   direct:   5 VALU per pair from the differences q - r (any tile);
+//   Gram:     3 FMA per pair (C + A.a - W Q2), rho^2 <= kGramMaxRho2;
+//   factored: one W for the whole tile and rho^2 <= kFactoredMaxRho2: the
+//             factor exp2(-W Q2) leaves the tile's sum, 2 FMA per pair.  It
+//             is computed per (query, tile) in FP64 (exact differences, split
+//             exp2).  C + A.a = exponent + W Q2 <= W Q2, so the partial sums
+//             stay finite for W Q2 <= kFarExp; farther queries take factor 0
+//             (all their terms in the tile are below 2^-64 of the mass scale).
+// The rounding error of a centred exponent grows like (rho + distance)^2 in
+// bandwidth units, which bounds rho for the centred forms.
 
-template <bool ISO>
 __device__ __forceinline__ float gauss_direct_term(float qx, float qy, float4 f, float rho) {
   const float dx = qx - f.x;
   const float dy = qy - f.y;
-  const float d2 = ISO ? __builtin_fmaf(dy, dy, dx * dx) : __builtin_fmaf(dy * rho, dy, dx * dx);
+  const float d2 = __builtin_fmaf(dy * rho, dy, dx * dx);
   return __builtin_amdgcn_exp2f(__builtin_fmaf(-f.z, d2, f.w));
+}
+
+__device__ __forceinline__ float gauss_direct_term_iso(float qx, float qy, float4 f) {
+  const float dx = qx - f.x;
+  const float dy = qy - f.y;
+  const float d2 = __builtin_fmaf(dy, dy, dx * dx);
+  return __builtin_amdgcn_exp2f(__builtin_fmaf(-f.z, d2, f.w));
+}
+
+__device__ __forceinline__ float gauss_factored_term(float ax, float ay, float4 f) {
+  return __builtin_amdgcn_exp2f(__builtin_fmaf(f.x, ax, __builtin_fmaf(f.y, ay, f.w)));
 }
 
 __device__ __forceinline__ float gauss_gram_term(float ax, float ay, float aq, float4 f) {
@@ -80,23 +117,75 @@ __device__ __forceinline__ float gauss_gram_term(float ax, float ay, float aq, f
   return __builtin_amdgcn_exp2f(a);
 }
 
+// 2^e for an FP64 exponent e <= 0: exp2 of the fractional part in FP32
+// (|f| <= 1/2, so its FP32 rounding costs < 2^-25 absolute) and the integer
+// part applied exactly by ldexp.
+__device__ __forceinline__ double exp2_split(double e) {
+  double n = rint(e);
+  if (n < -1000.0) n = -1000.0;
+  const float f = (float)(e - n);
+  return (double)__builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+// Sum of one tile's terms for kGaussQpt queries (self pair excluded when diag).
+// FORM: 0 direct, 1 Gram, 2 factored.
+template <int FORM, bool ISO, bool DIAG>
+__device__ __forceinline__ void gauss_tile_loop(const float4 *__restrict__ sf, int cnt, long long tile,
+                                                long long q_self0, const float *px, const float *py,
+                                                const float *ax, const float *ay, const float *aq, float rho,
+                                                float *s) {
+#pragma unroll BK_GAUSS_UNROLL
+  for (int j = 0; j < cnt; ++j) {
+    const float4 f = sf[j];
+#pragma unroll
+    for (int k = 0; k < kGaussQpt; ++k) {
+      float v;
+      if constexpr (FORM == 2) {
+        v = gauss_factored_term(ax[k], ay[k], f);
+      } else if constexpr (FORM == 1) {
+        v = gauss_gram_term(ax[k], ay[k], aq[k], f);
+      } else if constexpr (ISO) {
+        v = gauss_direct_term_iso(px[k], py[k], f);
+      } else {
+        v = gauss_direct_term(px[k], py[k], f, rho);
+      }
+      if constexpr (DIAG) {
+        s[k] += (tile + j == q_self0 + k * kBlock) ? 0.0f : v;
+      } else {
+        s[k] += v;
+      }
+    }
+  }
+}
+
+template <int FORM, bool ISO>
+__device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cnt, long long tile, bool diag,
+                                           long long q_self0, const float *px, const float *py, const float *ax,
+                                           const float *ay, const float *aq, float rho, float *s) {
+  if (diag)
+    gauss_tile_loop<FORM, ISO, true>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
+  else
+    gauss_tile_loop<FORM, ISO, false>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
+}
+
 template <bool ISO>
 __device__ __forceinline__ void gauss_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_begin, long long r_end,
     long long q_begin, int q_count, double *__restrict__ dst, double out_scale,
-    int self_excl, float rho, float gx, float gy, int mode) {
-  __shared__ float4 sf[2][kTile];
+    int self_excl, float rho, double gx, double gy, int mode) {
+  constexpr int T = kGaussTile;
+  __shared__ float4 sf[2][T];
   const int t = threadIdx.x;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
     return;
   }
-  float px[kQpt], py[kQpt];
-  double acc[kQpt];
+  float px[kGaussQpt], py[kGaussQpt];
+  double acc[kGaussQpt];
 #pragma unroll
-  for (int k = 0; k < kQpt; ++k) {
+  for (int k = 0; k < kGaussQpt; ++k) {
     const int qi = t + k * kBlock;
     const long long q = q_begin + (qi < q_count ? qi : 0);
     px[k] = qx[q];
@@ -104,88 +193,73 @@ __device__ __forceinline__ void gauss_item(
     acc[k] = 0.0;
   }
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
+  const long long q_self0 = q_begin + t;
 
-  long long tile = (r_begin / kTile) * kTile;
-  float4 meta = tiles[tile / kTile];
-  bool gram = (mode == GAUSS_AUTO) && meta.z != 0.0f;
+  long long tile = (r_begin / T) * T;
+  float4 meta = tiles[tile / T];
+  bool centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
   float4 nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
-  {
+  if (t < T) {
     const long long r = tile + t;
-    if (r >= r_begin && r < r_end) nf = gram ? fg[r] : fd[r];
+    if (r >= r_begin && r < r_end) nf = centred ? fg[r] : fd[r];
   }
   int buf = 0;
-  for (; tile < r_end; tile += kTile) {
-    sf[buf][t] = nf;
+  for (; tile < r_end; tile += T) {
+    if (t < T) sf[buf][t] = nf;
     __syncthreads();
     const float4 cur = meta;
-    const bool cur_gram = gram;
+    const int form = centred ? (int)cur.z : 0;
     {
-      const long long nt = tile + kTile;
+      const long long nt = tile + T;
       nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
       if (nt < r_end) {
-        meta = tiles[nt / kTile];
-        gram = (mode == GAUSS_AUTO) && meta.z != 0.0f;
+        meta = tiles[nt / T];
+        centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
         const long long r = nt + t;
-        if (r < r_end) nf = gram ? fg[r] : fd[r];
+        if (t < T && r < r_end) nf = centred ? fg[r] : fd[r];
       }
     }
-    const int cnt = (int)((r_end - tile) < kTile ? (r_end - tile) : kTile);
+    const int cnt = (int)((r_end - tile) < T ? (r_end - tile) : T);
     const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
-    float s[kQpt];
+    float s[kGaussQpt], ax[kGaussQpt], ay[kGaussQpt], aq[kGaussQpt];
+    double fac[kGaussQpt];
 #pragma unroll
-    for (int k = 0; k < kQpt; ++k) s[k] = 0.0f;
-    if (cur_gram) {
-      float ax[kQpt], ay[kQpt], aq[kQpt];
+    for (int k = 0; k < kGaussQpt; ++k) {
+      s[k] = 0.0f;
+      fac[k] = 1.0;
+      ax[k] = ay[k] = aq[k] = 0.0f;
+    }
+    if (form != 0) {
 #pragma unroll
-      for (int k = 0; k < kQpt; ++k) {
-        ax[k] = px[k] - cur.x;
-        ay[k] = py[k] - cur.y;
-        aq[k] = __builtin_fmaf(gy * ay[k], ay[k], gx * ax[k] * ax[k]);
-      }
-      if (!diag) {
-#pragma unroll 4
-        for (int j = 0; j < cnt; ++j) {
-          const float4 f = sf[buf][j];
-#pragma unroll
-          for (int k = 0; k < kQpt; ++k) s[k] += gauss_gram_term(ax[k], ay[k], aq[k], f);
-        }
-      } else {
-        for (int j = 0; j < cnt; ++j) {
-          const float4 f = sf[buf][j];
-          const long long rj = tile + j;
-#pragma unroll
-          for (int k = 0; k < kQpt; ++k) {
-            const float v = gauss_gram_term(ax[k], ay[k], aq[k], f);
-            s[k] += (rj == q_begin + t + k * kBlock) ? 0.0f : v;
-          }
-        }
-      }
-    } else {
-      if (!diag) {
-#pragma unroll 4
-        for (int j = 0; j < cnt; ++j) {
-          const float4 f = sf[buf][j];
-#pragma unroll
-          for (int k = 0; k < kQpt; ++k) s[k] += gauss_direct_term<ISO>(px[k], py[k], f, rho);
-        }
-      } else {
-        for (int j = 0; j < cnt; ++j) {
-          const float4 f = sf[buf][j];
-          const long long rj = tile + j;
-#pragma unroll
-          for (int k = 0; k < kQpt; ++k) {
-            const float v = gauss_direct_term<ISO>(px[k], py[k], f, rho);
-            s[k] += (rj == q_begin + t + k * kBlock) ? 0.0f : v;
-          }
+      for (int k = 0; k < kGaussQpt; ++k) {
+        const double dx = (double)px[k] - (double)cur.x;
+        const double dy = (double)py[k] - (double)cur.y;
+        const double q2 = __builtin_fma(gy * dy, dy, gx * dx * dx);
+        ax[k] = (float)dx;
+        ay[k] = (float)dy;
+        aq[k] = (float)q2;
+        if (form == 2) {
+          const double e = (double)cur.w * q2;
+          const bool near = e <= (double)kFarExp;
+          fac[k] = near ? exp2_split(-e) : 0.0;
+          ax[k] = near ? ax[k] : 0.0f;
+          ay[k] = near ? ay[k] : 0.0f;
         }
       }
     }
+    if (form == 2) {
+      gauss_tile<2, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+    } else if (form == 1) {
+      gauss_tile<1, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+    } else {
+      gauss_tile<0, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+    }
 #pragma unroll
-    for (int k = 0; k < kQpt; ++k) acc[k] += (double)s[k];
+    for (int k = 0; k < kGaussQpt; ++k) acc[k] = __builtin_fma((double)s[k], fac[k], acc[k]);
     buf ^= 1;
   }
 #pragma unroll
-  for (int k = 0; k < kQpt; ++k) {
+  for (int k = 0; k < kGaussQpt; ++k) {
     const int qi = t + k * kBlock;
     if (qi < q_count) dst[qi] = acc[k] * out_scale;
   }
@@ -197,12 +271,12 @@ __global__ __launch_bounds__(kBlock) void gauss_range_kernel(
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_i, long long r_end,
     long long split_len, long long q_i, long long nq, double *__restrict__ dst,
-    double out_scale, int self_excl, float rho, float gx, float gy, int mode) {
-  const long long qb = (long long)blockIdx.x * kQueriesPerBlock;
+    double out_scale, int self_excl, float rho, double gx, double gy, int mode) {
+  const long long qb = (long long)blockIdx.x * kGaussQueriesPerBlock;
   long long rb, re;
-  split_bounds(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
+  split_bounds<kGaussTile>(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
   const long long left = nq - qb;
-  const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
+  const int q_count = (int)(left < kGaussQueriesPerBlock ? left : kGaussQueriesPerBlock);
   gauss_item<ISO>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count,
                   dst + (long long)blockIdx.y * nq + qb, out_scale, self_excl, rho, gx, gy, mode);
 }
@@ -213,18 +287,22 @@ __global__ __launch_bounds__(kBlock) void gauss_list_kernel(
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, const WorkItem *__restrict__ items,
     double *__restrict__ dst, double out_scale, int self_excl, float rho,
-    float gx, float gy, int mode) {
+    double gx, double gy, int mode) {
   const WorkItem it = items[blockIdx.x];
   gauss_item<ISO>(qx, qy, fd, fg, tiles, it.r_begin, it.r_end, it.q_begin, it.q_count,
                   dst + it.out_index, out_scale, self_excl, rho, gx, gy, mode);
 }
 
-// Block-wide reductions over kBlock threads (one value per thread).
-__device__ float block_min_f(float v, float *sh) {
+// Block-wide min / max over N threads (one value per thread).
+template <int N, bool MAX>
+__device__ float block_reduce_f(float v, float *sh) {
   sh[threadIdx.x] = v;
   __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] = __builtin_fminf(sh[threadIdx.x], sh[threadIdx.x + o]);
+  for (int o = N / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const float a = sh[threadIdx.x], b = sh[threadIdx.x + o];
+      sh[threadIdx.x] = MAX ? __builtin_fmaxf(a, b) : __builtin_fminf(a, b);
+    }
     __syncthreads();
   }
   const float r = sh[0];
@@ -232,52 +310,54 @@ __device__ float block_min_f(float v, float *sh) {
   return r;
 }
 
-__device__ float block_max_f(float v, float *sh) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] = __builtin_fmaxf(sh[threadIdx.x], sh[threadIdx.x + o]);
-    __syncthreads();
-  }
-  const float r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-// One workgroup per aligned tile of kTile references.
-__global__ __launch_bounds__(kBlock) void prep_gauss_kernel(
+// One workgroup of kGaussTile threads per aligned tile.
+__global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
     EvalParams p, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ m, const float *__restrict__ abw, long long n,
     double inv_m_scale, float4 *__restrict__ fd, float4 *__restrict__ fg,
     float4 *__restrict__ tiles) {
-  __shared__ float sh[kBlock];
-  const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
+  constexpr int T = kGaussTile;
+  __shared__ float sh[T];
+  const long long i = (long long)blockIdx.x * T + threadIdx.x;
   const bool valid = i < n;
   const float xi = valid ? x[i] : 0.0f;
   const float yi = valid ? y[i] : 0.0f;
   const float big = 3.0e38f;
-  const float xmin = block_min_f(valid ? xi : big, sh);
-  const float xmax = block_max_f(valid ? xi : -big, sh);
-  const float ymin = block_min_f(valid ? yi : big, sh);
-  const float ymax = block_max_f(valid ? yi : -big, sh);
+  const float xmin = block_reduce_f<T, false>(valid ? xi : big, sh);
+  const float xmax = block_reduce_f<T, true>(valid ? xi : -big, sh);
+  const float ymin = block_reduce_f<T, false>(valid ? yi : big, sh);
+  const float ymax = block_reduce_f<T, true>(valid ? yi : -big, sh);
   const float cx = 0.5f * (xmin + xmax);
   const float cy = 0.5f * (ymin + ymax);
   float rho2 = 0.0f;
+  float w_lo = big, w_hi = -big;
   if (valid) {
     const double a = (double)abw[i];
-    const double W = 1.0 / (a * a);
+    const float Wf = (float)(1.0 / (a * a));
+    const double W = (double)Wf;
     const double mass = (double)m[i] * inv_m_scale;
     const double lm = mass > 0.0 ? log2(mass) : (double)kMaskedLog;
     fd[i] = make_float4(xi, yi, (float)(p.gx * W), (float)lm);
     const double rx = (double)xi - (double)cx;
     const double ry = (double)yi - (double)cy;
     const double q = p.gx * rx * rx + p.gy * ry * ry;
-    fg[i] = make_float4((float)(2.0 * p.gx * W * rx), (float)(2.0 * p.gy * W * ry), (float)W,
+    fg[i] = make_float4((float)(2.0 * p.gx * W * rx), (float)(2.0 * p.gy * W * ry), Wf,
                         mass > 0.0 ? (float)(lm - W * q) : kMaskedLog);
     rho2 = (float)(W * q);
+    w_lo = w_hi = Wf;
   }
-  const float rmax = block_max_f(rho2, sh);
-  if (threadIdx.x == 0) tiles[blockIdx.x] = make_float4(cx, cy, rmax <= kGramMaxRho2 ? 1.0f : 0.0f, rmax);
+  const float rmax = block_reduce_f<T, true>(rho2, sh);
+  const float wmin = block_reduce_f<T, false>(w_lo, sh);
+  const float wmax = block_reduce_f<T, true>(w_hi, sh);
+  if (threadIdx.x == 0) {
+    float form = kFormDirect;
+    if (wmin == wmax && rmax <= kFactoredMaxRho2) {
+      form = kFormFactored;
+    } else if (rmax <= kGramMaxRho2) {
+      form = kFormGram;
+    }
+    tiles[blockIdx.x] = make_float4(cx, cy, form, wmax);
+  }
 }
 
 // ---- FP64 Epanechnikov kinds on FP32 points ----------------------------------
@@ -399,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void epan_range_kernel(
     long long nq, double *__restrict__ dst, double out_scale, int self_excl) {
   const long long qb = (long long)blockIdx.x * kQueriesPerBlock;
   long long rb, re;
-  split_bounds(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
+  split_bounds<kTile>(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
   const long long left = nq - qb;
   const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
   epan_item<FAM>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + (long long)blockIdx.y * nq + qb,
@@ -556,16 +636,6 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
 // ---- general dimension (3 <= D <= 16), FP64 distances -----------------------
 constexpr int kBlockG = 256;
 constexpr int kTileG = 128;
-
-// m * 2^e for an FP64 base-2 exponent e <= 0: exp2 of the fractional part in
-// FP32 (|f| <= 1/2, so its FP32 rounding costs < 2^-25 absolute), the
-// integer part applied exactly by ldexp.
-__device__ __forceinline__ double exp2_split(double e) {
-  double n = rint(e);
-  if (n < -1000.0) n = -1000.0;
-  const float f = (float)(e - n);
-  return (double)__builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
-}
 
 template <int D, int FAM>
 __device__ __forceinline__ void direct_general_item(
@@ -770,8 +840,8 @@ hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y
                              double m_scale, float4 *fd, float4 *fg,
                              float4 *tiles, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  const unsigned ntiles = (unsigned)((n + kTile - 1) / kTile);
-  prep_gauss_kernel<<<ntiles, kBlock, 0, s>>>(p, x, y, m, abw, (long long)n, 1.0 / m_scale, fd, fg, tiles);
+  const unsigned ntiles = (unsigned)((n + kGaussTile - 1) / kGaussTile);
+  prep_gauss_kernel<<<ntiles, kGaussTile, 0, s>>>(p, x, y, m, abw, (long long)n, 1.0 / m_scale, fd, fg, tiles);
   return hipGetLastError();
 }
 
@@ -782,7 +852,7 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
                         int splits, double *dst, double out_scale,
                         int self_excl, int mode, hipStream_t s) {
   const float rho = (float)(p.gy / p.gx);
-  const float gx = (float)p.gx, gy = (float)p.gy;
+  const double gx = p.gx, gy = p.gy;
   const bool iso = p.iso != 0;
   if (items) {
     if (n_items <= 0) return hipSuccess;
@@ -794,8 +864,8 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
                                                           rho, gx, gy, mode);
   } else {
     if (nq <= 0) return hipSuccess;
-    const long long len = split_length(r_i, r_end, splits, kTile);
-    const dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
+    const long long len = split_length(r_i, r_end, splits, kGaussTile);
+    const dim3 grid((unsigned)((nq + kGaussQueriesPerBlock - 1) / kGaussQueriesPerBlock), (unsigned)splits);
     if (iso)
       gauss_range_kernel<true><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst,
                                                        out_scale, self_excl, rho, gx, gy, mode);

@@ -36,6 +36,11 @@ H_EPAN = (0.08, 0.12)
 FLOP_PER_PAIR_GAUSS = 9     # dx, dy, dx^2, fma dy^2, scale, exp, fma (exp counted as 1)
 FLOP_PER_PAIR_EPANPROD = 12
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 peak (vector and matrix)
+# HBM bytes per launch of the Gaussian kernel at this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one run each) and scaled by a
+# 256 MiB copy kernel's readings (fetch x2.00, write x1.00):
+# 391,710 KiB fetched + 265,625 KiB written.
+GAUSS_HBM_BYTES_PER_LAUNCH = (391_710 + 265_625) * 1024
 
 
 def parse_args():
@@ -190,11 +195,14 @@ def main() -> int:
         "roofline": {"bound": "mfma", "pipe": "FP32 VALU + v_exp_f32 (compute-bound, FP32 peak)",
                      "achieved": head["tflops"], "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": head["tflops"] / PEAK_FP32_TFLOPS, "flop_per_pair": FLOP_PER_PAIR_GAUSS,
-                     "traffic": None},
+                     "traffic": GAUSS_HBM_BYTES_PER_LAUNCH,
+                     "traffic_note": "bytes/launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a copy-kernel calibration",
+                     "hbm_gb_s": GAUSS_HBM_BYTES_PER_LAUNCH / (head["avg_ms"] * 1e-3) / 1e9},
     }
     if "epanprod" in results:
         e = results["epanprod"]
         line["epanechnikov_product"] = {"pair_evals_per_s": e["pair_evals_per_s"], "ms_per_step": e["avg_ms"],
+                                        "arithmetic": "f64 on f32 points",
                                         "tflops": e["tflops"], "frac": e["tflops"] / PEAK_FP32_TFLOPS,
                                         "flop_per_pair": FLOP_PER_PAIR_EPANPROD, "hx": H_EPAN[0],
                                         "hy": H_EPAN[1]}
