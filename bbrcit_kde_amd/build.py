@@ -71,9 +71,29 @@ def build_library(verbose: bool = False) -> Path:
     return LIB_PATH
 
 
+CPP_DRIVER_SRC = REPO_DIR / "tests" / "cpp" / "api_driver.cc"
+CPP_DRIVER_LIB = REPO_DIR / "tests" / "cpp" / "libapi_driver.so"
+
+
+def build_cpp_driver(verbose: bool = False) -> Path:
+    """Compile the C++ API test driver (host C++ over the C ABI) with g++."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = CPP_DRIVER_LIB.with_suffix(".so.tmp")
+    cmd = [gxx, "-std=c++14", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
+           "-I", str(PKG_DIR / "include"), "-I", str(REPO_DIR / "include"), str(CPP_DRIVER_SRC),
+           "-L", str(LIB_DIR), "-lbbrcit_hip", "-Wl,-rpath,$ORIGIN/../../bbrcit_kde_amd/lib", "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(tmp, CPP_DRIVER_LIB)
+    return CPP_DRIVER_LIB
+
+
 def main() -> int:
     path = build_library(verbose=True)
     print(f"built {path}")
+    if CPP_DRIVER_SRC.exists():
+        print(f"built {build_cpp_driver(verbose=True)}")
     return 0
 
 

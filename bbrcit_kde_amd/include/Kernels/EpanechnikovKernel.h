@@ -1,0 +1,71 @@
+// EpanechnikovKernel.h -- isotropic EpanechnikovKernel in D dimensions with bandwidth h.
+// Interface of the reference's include/Kernels/EpanechnikovKernel.h: normalization(),
+// unnormalized_eval(p, q[, a]) = U(p - q; h a), bandwidth get/set, simulate.
+#ifndef BBRCIT_AMD_EPANECHNIKOVKERNEL_H_
+#define BBRCIT_AMD_EPANECHNIKOVKERNEL_H_
+
+#include <random>
+#include <vector>
+
+#include "../KdeTraits.h"
+#include "KernelDesc.h"
+#include "KernelTraits.h"
+#include "Utils.h"
+
+namespace bbrcit {
+
+template <int D, typename T = double>
+class EpanechnikovKernel {
+ public:
+  using FloatType = T;
+  static constexpr int dim() { return D; }
+
+  EpanechnikovKernel() : h_(T(1)) {}
+  EpanechnikovKernel(T bandwidth) : h_(bandwidth) {}
+
+  T normalization() const { return kernel_detail::epanechnikov_norm<T>(D, h_); }
+
+  template <typename PointT>
+  T unnormalized_eval(const PointT &p, const PointT &q) const {
+    return unnormalized_eval(p, q, ConstantTraits<T>::one());
+  }
+
+  // U(p - q; h a) from the squared distance over (h a)^2
+  template <typename PointT>
+  T unnormalized_eval(const PointT &p, const PointT &q, T a) const {
+    T s = T(0);
+    for (int i = 0; i < D; ++i) {
+      const T d = T(p[i]) - T(q[i]);
+      s += d * d;
+    }
+    return profile(s / (h_ * h_ * a * a));
+  }
+
+  template <typename RNG>
+  void simulate(RNG &e, std::vector<T> &p, T a = ConstantTraits<T>::one()) const;
+
+  T bandwidth() const { return h_; }
+  void set_bandwidth(T h) { h_ = h; }
+
+  bk_kernel_desc desc() const { return make_kernel_desc(BK_EPANECHNIKOV, D, (double)h_, (double)h_); }
+
+ private:
+  static T profile(T arg) { return kernel_detail::epan_profile<T>(arg); }
+  T h_;
+};
+
+// Sampling is defined in one dimension (as in the reference,
+// EpanechnikovKernel.h:162-180): three uniform(-1, 1) draws per coordinate.
+template <int D, typename T>
+template <typename RNG>
+void EpanechnikovKernel<D, T>::simulate(RNG &e, std::vector<T> &p, T a) const {
+  static_assert(D == 1, "EpanechnikovKernel::simulate is defined for D = 1");
+  std::uniform_real_distribution<T> u(T(-1), T(1));
+  p.resize(1);
+  const T v1 = u(e), v2 = u(e), v3 = u(e);
+  p[0] = a * h_ * epanechnikov_choice(v1, v2, v3);
+}
+
+}  // namespace bbrcit
+
+#endif

@@ -1,0 +1,348 @@
+// api_driver.cc -- extern "C" entry points over the C++ drop-in API
+// (bbrcit_kde_amd/include: KernelDensity, Kdtree, HipDirectKde, kernels),
+// loaded by the tests with ctypes.  Test infrastructure only.
+//
+// Kinds use the numbering of include/bbrcit_hip.h; kernel_float selects the
+// kernel's float type (0: double, 1: float).
+#include <random>
+#include <vector>
+
+#include <HipDirectKde.h>
+#include <KernelDensity.h>
+#include <Kernels/ConvKernelAssociator.h>
+
+#include <exception>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+using namespace bbrcit;
+
+namespace {
+
+template <int D, class K>
+struct Tag {
+  static constexpr int dim = D;
+  using Kernel = K;
+};
+
+template <int D, class T>
+void set_bw(GaussianKernel<D, T> &k, double hx, double) { k.set_bandwidth(hx); }
+template <int D, class T>
+void set_bw(EpanechnikovKernel<D, T> &k, double hx, double) { k.set_bandwidth(hx); }
+template <class T>
+void set_bw(GaussianProductKernel2d<T> &k, double hx, double hy) { k.set_bandwidths(hx, hy); }
+template <class T>
+void set_bw(EpanechnikovProductKernel2d<T> &k, double hx, double hy) { k.set_bandwidths(hx, hy); }
+template <class T>
+void set_bw(EpanechnikovProductConvKernel2d<T> &k, double hx, double hy) { k.set_bandwidths(hx, hy); }
+template <class T>
+void set_bw(GaussianProductConvKernel2d<T> &k, double hx, double hy) { k.set_bandwidths(hx, hy); }
+template <class T>
+void set_bw(EpanechnikovConvKernel1d<T> &k, double hx, double) { k.set_hx(hx); }
+template <class T>
+void set_bw(GaussianConvKernel1d<T> &k, double hx, double) { k.set_hx(hx); }
+
+template <class T, class Fn>
+int dispatch_t(int kind, int dim, Fn fn) {
+  switch (kind) {
+    case 1:
+      if (dim == 1) return fn(Tag<1, EpanechnikovKernel<1, T>>());
+      if (dim == 2) return fn(Tag<2, EpanechnikovKernel<2, T>>());
+      break;
+    case 2:
+      if (dim == 1) return fn(Tag<1, GaussianKernel<1, T>>());
+      if (dim == 2) return fn(Tag<2, GaussianKernel<2, T>>());
+      break;
+    case 3:
+      if (dim == 2) return fn(Tag<2, EpanechnikovProductKernel2d<T>>());
+      break;
+    case 4:
+      if (dim == 2) return fn(Tag<2, GaussianProductKernel2d<T>>());
+      break;
+    case 5:
+      if (dim == 1) return fn(Tag<1, EpanechnikovConvKernel1d<T>>());
+      break;
+    case 6:
+      if (dim == 2) return fn(Tag<2, EpanechnikovProductConvKernel2d<T>>());
+      break;
+    case 7:
+      if (dim == 1) return fn(Tag<1, GaussianConvKernel1d<T>>());
+      break;
+    case 8:
+      if (dim == 2) return fn(Tag<2, GaussianProductConvKernel2d<T>>());
+      break;
+  }
+  return -1;
+}
+
+template <class Fn>
+int dispatch(int kind, int dim, int kf, Fn fn) {
+  return kf ? dispatch_t<float>(kind, dim, fn) : dispatch_t<double>(kind, dim, fn);
+}
+
+template <class T, class Fn>
+int dispatch_conv_t(int kind, int dim, Fn fn) {
+  if (kind == 1 && dim == 1) return fn(Tag<1, EpanechnikovKernel<1, T>>());
+  if (kind == 2 && dim == 1) return fn(Tag<1, GaussianKernel<1, T>>());
+  if (kind == 3 && dim == 2) return fn(Tag<2, EpanechnikovProductKernel2d<T>>());
+  if (kind == 4 && dim == 2) return fn(Tag<2, GaussianProductKernel2d<T>>());
+  return -1;
+}
+
+template <class Fn>
+int dispatch_conv(int kind, int dim, int kf, Fn fn) {
+  return kf ? dispatch_conv_t<float>(kind, dim, fn) : dispatch_conv_t<double>(kind, dim, fn);
+}
+
+template <class T, class Fn>
+int dispatch2_t(int kind, Fn fn) {
+  switch (kind) {
+    case 1: return fn(Tag<2, EpanechnikovKernel<2, T>>());
+    case 2: return fn(Tag<2, GaussianKernel<2, T>>());
+    case 3: return fn(Tag<2, EpanechnikovProductKernel2d<T>>());
+    case 4: return fn(Tag<2, GaussianProductKernel2d<T>>());
+  }
+  return -1;
+}
+
+template <class KDE>
+std::vector<typename KDE::DataPointType> make_points(const double *x, const double *w, size_t n) {
+  using P = typename KDE::DataPointType;
+  constexpr int D = KDE::dim();
+  std::vector<P> pts;
+  pts.reserve(n);
+  for (size_t i = 0; i < n; ++i) {
+    typename P::PointType g;
+    for (int d = 0; d < D; ++d) g[d] = x[i * D + d];
+    pts.emplace_back(g, typename KDE::AttributeType(w ? w[i] : 1.0));
+  }
+  return pts;
+}
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::out_of_range &e) {
+    g_err = e.what();
+    return -102;
+  } catch (const std::invalid_argument &e) {
+    g_err = e.what();
+    return -101;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -100;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *api_last_error() { return g_err.c_str(); }
+
+// Kdtree order of the data points (host only).
+long api_tree_points(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                     int leaf_nmax, double *coords_out, double *w_out, double *mass_out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      const auto &tp = kde.points();
+      for (size_t i = 0; i < tp.size(); ++i) {
+        for (int d = 0; d < D; ++d) coords_out[i * D + d] = tp[i][d];
+        w_out[i] = tp[i].attributes().weight();
+        mass_out[i] = tp[i].attributes().mass();
+      }
+      return (int)tp.size();
+    });
+  });
+}
+
+// Kernel normalization() and unnormalized_eval(p, q, a) (host only).
+int api_kernel(int kind, int dim, int kf, double hx, double hy, const double *p, const double *q, double a,
+               double *norm_out, double *u_out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      using T = typename K::FloatType;
+      constexpr int D = decltype(tag)::dim;
+      K k;
+      set_bw(k, hx, hy);
+      Point<D, T> pp, qq;
+      for (int d = 0; d < D; ++d) {
+        pp[d] = (T)p[d];
+        qq[d] = (T)q[d];
+      }
+      *norm_out = (double)k.normalization();
+      *u_out = (double)k.unnormalized_eval(pp, qq, (T)a);
+      return 0;
+    });
+  });
+}
+
+// KernelDensity::direct_eval(queries): values in query order.
+int api_direct_eval(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                    int leaf_nmax, const double *qry, size_t nq, double *out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      auto q = make_points<KDE>(qry, nullptr, nq);
+      kde.direct_eval(q);
+      for (size_t i = 0; i < nq; ++i) out[i] = q[i].attributes().value();
+      return 0;
+    });
+  });
+}
+
+// Dual-tree KernelDensity::eval(queries, rel, abs, leaf): queries in the
+// query tree's order with their bounds.
+long api_tree_eval(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                   int leaf_nmax, const double *qry, size_t nq, double rel, double abs_err, int q_leaf_nmax,
+                   double *q_out, double *lower, double *upper) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      auto q = make_points<KDE>(qry, nullptr, nq);
+      std::ostringstream sink;
+      std::streambuf *old = std::cerr.rdbuf(sink.rdbuf());
+      kde.eval(q, rel, abs_err, q_leaf_nmax);
+      std::cerr.rdbuf(old);
+      for (size_t i = 0; i < q.size(); ++i) {
+        for (int d = 0; d < D; ++d) q_out[i * D + d] = q[i][d];
+        lower[i] = q[i].attributes().lower();
+        upper[i] = q[i].attributes().upper();
+      }
+      return (int)q.size();
+    });
+  });
+}
+
+int api_likelihood_cv(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                      int leaf_nmax, double *out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      *out = kde.likelihood_cross_validate();
+      return 0;
+    });
+  });
+}
+
+int api_lsq_conv_cv(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                    int leaf_nmax, double *out) {
+  return guarded([&] {
+    return dispatch_conv(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      *out = kde.lsq_convolution_cross_validate();
+      return 0;
+    });
+  });
+}
+
+int api_lsq_numint_cv(int kind, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                      int leaf_nmax, double sx, double ex, int nx, double sy, double ey, int ny, double *out) {
+  return guarded([&] {
+    auto fn = [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      using KDE = KernelDensity<2, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      *out = lsq_numint_cross_validate(kde, sx, ex, nx, sy, ey, ny);
+      return 0;
+    };
+    return kf ? dispatch2_t<float>(kind, fn) : dispatch2_t<double>(kind, fn);
+  });
+}
+
+// adapt_density(alpha): data points in tree order with their abw and mass;
+// then the adaptive density at the queries (direct_eval) and its LCV.
+long api_adapt(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+               int leaf_nmax, double alpha, double *coords_out, double *abw_out, double *mass_out,
+               const double *qry, size_t nq, double *q_values, double *lcv_out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      kde.adapt_density(alpha);
+      const auto &tp = kde.points();
+      for (size_t i = 0; i < tp.size(); ++i) {
+        for (int d = 0; d < D; ++d) coords_out[i * D + d] = tp[i][d];
+        abw_out[i] = tp[i].attributes().abw();
+        mass_out[i] = tp[i].attributes().mass();
+      }
+      if (nq > 0) {
+        auto q = make_points<KDE>(qry, nullptr, nq);
+        kde.direct_eval(q);
+        for (size_t i = 0; i < nq; ++i) q_values[i] = q[i].attributes().value();
+      }
+      if (lcv_out) *lcv_out = kde.likelihood_cross_validate();
+      return (int)tp.size();
+    });
+  });
+}
+
+// HipDirectKde over (refs, queries) with unit masses scaled by 1/n:
+// unnormalized_eval on [r_i, r_j] x [q_i, q_j]; returns the exception class
+// on misuse (-102 out_of_range, -101 invalid_argument).
+int api_hip_direct_range(int kind, int dim, int kf, double hx, double hy, const double *pts, size_t n,
+                         const double *qry, size_t nq, size_t r_i, size_t r_j, size_t q_i, size_t q_j,
+                         int normalize, double *out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kf, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      using T = typename K::FloatType;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      auto r = make_points<KDE>(pts, nullptr, n);
+      for (auto &p : r) p.attributes().set_mass(1.0 / n);
+      auto q = make_points<KDE>(qry, nullptr, nq);
+      HipDirectKde<D, T, K> dev(r, q);
+      set_bw(dev.kernel(), hx, hy);
+      if (dev.reference_size() != n || dev.query_size() != nq) throw std::logic_error("sizes");
+      std::vector<T> res;
+      if (normalize)
+        dev.eval(r_i, r_j, q_i, q_j, res);
+      else
+        dev.unnormalized_eval(r_i, r_j, q_i, q_j, res);
+      HipDirectKde<D, T, K> copy(dev);  // copy construction re-uploads
+      std::vector<T> res2;
+      copy.unnormalized_eval(r_i, r_j, q_i, q_j, res2);
+      const double c = normalize ? (double)dev.kernel().normalization() : 1.0;
+      for (size_t k = 0; k <= q_j - q_i; ++k) {
+        out[k] = (double)res[k];
+        if (std::abs((double)res2[k] * c - (double)res[k]) > 1e-6 * std::abs((double)res[k]) + 1e-300)
+          throw std::logic_error("copy differs");
+      }
+      return 0;
+    });
+  });
+}
+
+}  // extern "C"
