@@ -27,7 +27,7 @@ __all__ = [
     "BK_EPANECHNIKOV", "BK_GAUSSIAN", "BK_EPANECHNIKOV_PRODUCT_2D", "BK_GAUSSIAN_PRODUCT_2D",
     "BK_EPANECHNIKOV_CONV_1D", "BK_EPANECHNIKOV_PRODUCT_CONV_2D", "BK_GAUSSIAN_CONV_1D",
     "BK_GAUSSIAN_PRODUCT_CONV_2D", "BK_PREC_F32", "BK_PREC_F64", "BkError", "KernelDesc",
-    "DirectKde", "load_library", "library_path",
+    "DirectKde", "CvGrid", "load_library", "library_path",
 ]
 
 # ---- constants (mirror include/bbrcit_hip.h) ---------------------------------
@@ -42,6 +42,10 @@ BK_ERR_ALLOC = -6
 BK_OPT_GAUSS_FORM = 1
 BK_GAUSS_FORM_AUTO = 0
 BK_GAUSS_FORM_DIRECT = 1
+
+BK_CVG_VARIANT = 1
+BK_CVG_GRAPH = 2
+BK_CVG_VARIANTS = 2
 
 BK_SET_REF = 0
 BK_SET_QUERY = 1
@@ -134,6 +138,18 @@ def load_library() -> ctypes.CDLL:
         "bk_likelihood_cv": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), _DBL_P, _DBL_P]),
         "bk_lsq_conv_cv": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), _DBL_P, _DBL_P]),
         "bk_adapt": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_double, _DBL_P, _DBL_P, _DBL_P]),
+        "bk_cv_grid_create": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.c_size_t,
+                                             ctypes.POINTER(ctypes.c_size_t), _DBL_P, _DBL_P, _DBL_P,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+        "bk_cv_grid_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+        "bk_cv_grid_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _DBL_P, ctypes.c_size_t, _DBL_P,
+                                          _DBL_P, _DBL_P]),
+        "bk_cv_grid_loo": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                          _DBL_P]),
+        "bk_cv_grid_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+        "bk_cv_grid_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+        "bk_cv_grid_last_timing": (ctypes.c_int, [ctypes.c_void_p, _DBL_P, _DBL_P,
+                                                  ctypes.POINTER(ctypes.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -310,3 +326,95 @@ class DirectKde:
         for v in idx:
             if int(v) < 0:
                 raise IndexError("indices must be non-negative")
+
+
+class CvGrid:
+    """Batched cross-validation over datasets x a bandwidth grid.
+
+    Mirror of the bandwidth loop of the reference's grid_search tool
+    (grid_search/grid_search.cc:366-402) as one device pipeline
+    (``bk_cv_grid_*``).  ``datasets`` is a sequence of (n_d, 2) point arrays,
+    ``weights`` the matching per-point weights (used as given) and ``abw``
+    optional adaptive bandwidth corrections.  :meth:`run` returns
+    ``(lcv, lscv)`` arrays of shape (n_sets, n_hx, n_hy).
+    """
+
+    def __init__(self, kind: int, datasets, weights, abw=None, *, device: int = 0):
+        lib = load_library()
+        self._lib = lib
+        self._ctx = ctypes.c_void_p()
+        self._plan = ctypes.c_void_p()
+        _check(lib.bk_create(ctypes.byref(self._ctx), int(device), BK_PREC_F32))
+        pts = [_as_points(d) for d in datasets]
+        self.sizes = [len(p) for p in pts]
+        coords = np.ascontiguousarray(np.concatenate(pts, axis=0)) if pts else np.zeros((0, 2))
+        n = len(coords)
+        w = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in weights]))
+        if w.shape != (n,):
+            raise ValueError("weights must match the datasets")
+        a = None
+        if abw is not None:
+            a = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in abw]))
+            if a.shape != (n,):
+                raise ValueError("abw must match the datasets")
+        sizes = (ctypes.c_size_t * max(1, len(self.sizes)))(*self.sizes)
+        try:
+            _check(lib.bk_cv_grid_create(self._ctx, int(kind), len(self.sizes), sizes, _dptr(coords), _dptr(w),
+                                         _dptr(a), ctypes.byref(self._plan)))
+        except Exception:
+            self.close()
+            raise
+        self.kind = kind
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            self._lib.bk_cv_grid_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._lib.bk_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(self._lib.bk_cv_grid_set_option(self._plan, int(option), int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        _check(self._lib.bk_cv_grid_get_option(self._plan, int(option), ctypes.byref(v)))
+        return v.value
+
+    def run(self, hx, hy):
+        hx = np.ascontiguousarray(np.asarray(hx, dtype=np.float64).ravel())
+        hy = np.ascontiguousarray(np.asarray(hy, dtype=np.float64).ravel())
+        shape = (len(self.sizes), len(hx), len(hy))
+        lcv = np.empty(shape, dtype=np.float64)
+        lscv = np.empty(shape, dtype=np.float64)
+        _check(self._lib.bk_cv_grid_run(self._plan, len(hx), _dptr(hx), len(hy), _dptr(hy), _dptr(lcv),
+                                        _dptr(lscv)))
+        return lcv, lscv
+
+    def loo(self, set_index: int, ix: int, iy: int) -> np.ndarray:
+        """Normalised leave-one-out densities of one dataset and bandwidth pair."""
+        if min(set_index, ix, iy) < 0:
+            raise IndexError("indices must be non-negative")
+        n = self.sizes[set_index] if set_index < len(self.sizes) else 0
+        out = np.empty(max(n, 1), dtype=np.float64)
+        _check(self._lib.bk_cv_grid_loo(self._plan, int(set_index), int(ix), int(iy), _dptr(out)))
+        return out[:n]
+
+    def last_timing(self):
+        """(pair-kernel ms or nan under graph replay, pipeline ms, rescued entries) of the last run."""
+        m, t, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        _check(self._lib.bk_cv_grid_last_timing(self._plan, ctypes.byref(m), ctypes.byref(t), ctypes.byref(r)))
+        return m.value, t.value, r.value

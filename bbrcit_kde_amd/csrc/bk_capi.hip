@@ -7,6 +7,7 @@
 // (include/CudaDirectKde.h:181-190), checks every HIP return code, and turns
 // failures into status codes plus a thread-local message.
 #include "../../include/bbrcit_hip.h"
+#include "bk_common.h"
 #include "bk_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -21,69 +22,11 @@
 
 namespace {
 
-thread_local std::string g_last_error;
-
-struct BkError {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] void fail(int code, const std::string &msg) { throw BkError{code, msg}; }
-
-void hip_check(hipError_t e, const char *what) {
-  if (e != hipSuccess) {
-    fail(BK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-  }
-}
-
-template <class F>
-int guarded(F &&f) {
-  try {
-    f();
-    g_last_error.clear();
-    return BK_OK;
-  } catch (const BkError &e) {
-    g_last_error = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return BK_ERR_ALLOC;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return BK_ERR_STATE;
-  } catch (...) {
-    g_last_error = "unknown failure";
-    return BK_ERR_STATE;
-  }
-}
-
-// Device buffer that only grows.
-struct DevBuf {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-  void reserve(size_t need) {
-    if (need <= bytes) return;
-    if (ptr) {
-      hip_check(hipFree(ptr), "hipFree");
-      ptr = nullptr;
-      bytes = 0;
-    }
-    if (hipMalloc(&ptr, need) != hipSuccess) {
-      ptr = nullptr;
-      fail(BK_ERR_ALLOC, "hipMalloc of " + std::to_string(need) + " bytes failed");
-    }
-    bytes = need;
-  }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-  }
-  template <class T>
-  T *as() const {
-    return static_cast<T *>(ptr);
-  }
-};
+using bk::DevBuf;
+using bk::DeviceGuard;
+using bk::fail;
+using bk::guarded;
+using bk::hip_check;
 
 struct PointSet {
   size_t n = 0;
@@ -127,6 +70,15 @@ struct bk_ctx {
   std::vector<double> host_tmp;
 };
 
+namespace bk {
+std::string &last_error() {
+  thread_local std::string msg;
+  return msg;
+}
+int ctx_device(const bk_ctx *ctx) { return ctx->device; }
+hipStream_t ctx_stream(const bk_ctx *ctx) { return ctx->stream; }
+}  // namespace bk
+
 namespace {
 
 using bk::EvalParams;
@@ -134,18 +86,6 @@ using bk::WorkItem;
 
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kPi = 3.14159265358979323846;
-
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    hip_check(hipGetDevice(&prev), "hipGetDevice");
-    if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
-  }
-};
 
 void require_ctx(const bk_ctx *ctx) {
   if (!ctx) fail(BK_ERR_INVALID_ARG, "null context");
@@ -621,7 +561,7 @@ double reduce_to_host(bk_ctx *ctx, int mode, const double *w, const double *u, c
 
 extern "C" {
 
-const char *bk_last_error(void) { return g_last_error.c_str(); }
+const char *bk_last_error(void) { return bk::last_error().c_str(); }
 
 int bk_create(bk_ctx **ctx_out, int device, int precision) {
   return guarded([&] {

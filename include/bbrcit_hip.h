@@ -60,7 +60,7 @@ extern "C" {
 #define BK_PREC_F64  1
 
 /* Kernel families: one per kernel class template of the reference
- * (include/Kernels/*.h).  `dim` of the descriptor must be 1 or 2, except for
+ * (the headers of include/Kernels/).  `dim` of the descriptor must be 1 or 2, except for
  * BK_GAUSSIAN and BK_EPANECHNIKOV, which accept any D from 1 to 16. */
 #define BK_EPANECHNIKOV                 1 /* EpanechnikovKernel<D,T>          */
 #define BK_GAUSSIAN                     2 /* GaussianKernel<D,T>              */
@@ -213,6 +213,76 @@ int bk_lsq_conv_cv(bk_ctx *ctx, const bk_kernel_desc *desc,
  * resets to the non-adaptive state.  Requires bk_alias_queries_to_refs. */
 int bk_adapt(bk_ctx *ctx, const bk_kernel_desc *desc, double alpha,
              const double *weights, double *g_out, double *lambda_out);
+
+/* ---- batched cross-validation grid ------------------------------------ */
+/* Replaces the bandwidth loop of the reference's grid_search tool
+ * (grid_search/grid_search.cc:366-402), which sets each (hx, hy) of a grid
+ * in turn and runs one likelihood or least-squares CV per bandwidth
+ * (KernelDensity::likelihood_cross_validate, KernelDensityImpl.h:215-252;
+ * lsq_convolution_cross_validate, KernelDensityImpl.h:142-209).  Here one
+ * persistent launch evaluates every dataset of a plan against every
+ * bandwidth pair of a cross-product grid hx[ix] x hy[iy], and both scores
+ * come out of the same leave-one-out sums:
+ *     LCV_d(ix,iy)  = sum_i w_i log(C S_i)
+ *     LSCV_d(ix,iy) = sum_i w_i (C* (T_i + m_i) - 2 C S_i)
+ * with S_i = sum_{j != i} m_j U(x_i - x_j; (hx,hy) abw_j) and T_i the same
+ * sum with the convolution kernel U* (for BK_GAUSSIAN_PRODUCT_2D the
+ * reference's U* = U and C* = C/2, KernelTraits.h:423-447).  Pair terms are
+ * FP32 with per-stage FP32 sums and FP64 accumulation across stages; the
+ * score reductions are FP64 on the GPU; FP32 Gaussian sums below 2^-40 of
+ * the dataset's largest mass are recomputed in FP64.
+ *
+ * Supported kinds: BK_GAUSSIAN_PRODUCT_2D (the default of this path) and
+ * BK_EPANECHNIKOV_PRODUCT_2D (the kernel of the reference's grid_search).
+ * A plan belongs to the context it was created on (device and error
+ * reporting) and runs on a stream of its own; it is not thread-safe. */
+typedef struct bk_cv_grid bk_cv_grid;
+
+/* Options of a plan (bk_cv_grid_set_option / bk_cv_grid_get_option). */
+#define BK_CVG_VARIANT    1  /* -1 (default): autotune on the first run of a
+                                grid shape; 0..BK_CVG_VARIANTS-1: force one
+                                register tiling of the bandwidth grid */
+#define BK_CVG_GRAPH      2  /* 1 (default): replay a captured hipGraph of the
+                                whole pipeline; 0: launch eagerly */
+#define BK_CVG_VARIANTS   2  /* number of tilings: 0 = 8x8, 1 = 4x8 (Gaussian),
+                                0 = 4x8, 1 = 4x4 (Epanechnikov) bandwidth
+                                pairs per thread */
+
+/* Create a plan over n_sets datasets of 2-D points stored back to back:
+ * set d holds sizes[d] >= 1 points; coords is row-major (x, y) over all
+ * points, weights the per-point weights w_i (used as given; the reference
+ * normalises them to sum 1 per dataset, KernelDensityImpl.h:444-455) and
+ * abw the adaptive bandwidth corrections (NULL = all 1).  Masses are
+ * m_i = w_i * abw_i^-2 (KernelDensityImpl.h:430-442).  Points, masses and
+ * abw are rounded to FP32 once, here. */
+int bk_cv_grid_create(bk_ctx *ctx, int kind, size_t n_sets,
+                      const size_t *sizes, const double *coords,
+                      const double *weights, const double *abw,
+                      bk_cv_grid **plan_out);
+int bk_cv_grid_destroy(bk_cv_grid *plan);
+
+/* Evaluate the grid: lcv_out and lscv_out (either may be NULL) receive
+ * n_sets * n_hx * n_hy scores, index (d * n_hx + ix) * n_hy + iy.
+ * 1 <= n_hx, n_hy <= 64; bandwidths must lie in [1e-15, 1e15]. */
+int bk_cv_grid_run(bk_cv_grid *plan, size_t n_hx, const double *hx,
+                   size_t n_hy, const double *hy, double *lcv_out,
+                   double *lscv_out);
+
+/* Normalised leave-one-out densities C S_i of dataset `set` at bandwidth
+ * (hx[ix], hy[iy]) of the last run (sizes[set] values). */
+int bk_cv_grid_loo(bk_cv_grid *plan, size_t set, size_t ix, size_t iy,
+                   double *out);
+
+int bk_cv_grid_set_option(bk_cv_grid *plan, int option, int value);
+/* BK_CVG_VARIANT reports the tiling the last run used. */
+int bk_cv_grid_get_option(const bk_cv_grid *plan, int option, int *value);
+
+/* Timing of the last run: main_ms = the persistent pair kernel alone,
+ * total_ms = the whole device pipeline (both from HIP events on the plan's
+ * stream); rescued = number of (point, bandwidth tile) entries recomputed
+ * in FP64.  Any pointer may be NULL. */
+int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
+                           double *total_ms, long long *rescued);
 
 #ifdef __cplusplus
 }
