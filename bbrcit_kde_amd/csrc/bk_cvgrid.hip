@@ -496,7 +496,8 @@ struct bk_cv_grid {
   int n_sets = 0;
   std::vector<int> begin;        // n_sets + 1 offsets
   std::vector<double> m_scale;   // per set
-  std::vector<int> splits;       // per set
+  std::vector<int> splits;       // per set: reference splits
+  std::vector<int> split_len;    // per set: references per split (multiple of the LDS tile)
   int max_splits = 1;
   long long n_points = 0;
   bool adapt = false;
@@ -549,6 +550,7 @@ void choose_splits(bk_cv_grid *g) {
   }
   const double fair = total / slots;
   g->splits.assign(g->n_sets, 1);
+  g->split_len.assign(g->n_sets, 1);
   g->max_splits = 1;
   for (int d = 0; d < g->n_sets; ++d) {
     const long long n = g->begin[d + 1] - g->begin[d];
@@ -556,6 +558,11 @@ void choose_splits(bk_cv_grid *g) {
     long long s = (long long)std::ceil(item / (fair / 16.0));
     const long long cap = std::max<long long>(1, n / (4LL * kCvTile));
     s = std::max<long long>(1, std::min<long long>(s, std::min<long long>(cap, 64)));
+    // tile-aligned split length; the count follows from it, so no split is empty
+    long long len = (n + s - 1) / s;
+    len = (len + kCvTile - 1) / kCvTile * kCvTile;
+    s = (n + len - 1) / len;
+    g->split_len[d] = (int)len;
     g->splits[d] = (int)s;
     g->max_splits = std::max(g->max_splits, (int)s);
   }
@@ -568,14 +575,12 @@ std::vector<CvItem> make_items(const bk_cv_grid *g, int A, int B, int nsx, int n
     const int b = g->begin[d], e = g->begin[d + 1];
     const long long n = e - b;
     const int s = g->splits[d];
-    long long len = (n + s - 1) / s;
-    len = (len + kCvTile - 1) / kCvTile * kCvTile;
+    const long long len = g->split_len[d];
     for (int q0 = b; q0 < e; q0 += kCvBlock) {
       const int qc = std::min(kCvBlock, e - q0);
       for (int sp = 0; sp < s; ++sp) {
         const long long r0 = b + sp * len;
         const long long r1 = std::min<long long>(e, r0 + len);
-        if (r0 >= r1) continue;  // never for the split rule above; kept for safety
         for (int sx = 0; sx < nsx; ++sx)
           for (int sy = 0; sy < nsy; ++sy) {
             CvItem it{};
@@ -604,9 +609,8 @@ void check_split_cover(const bk_cv_grid *g) {
   for (int d = 0; d < g->n_sets; ++d) {
     const long long n = g->begin[d + 1] - g->begin[d];
     const int s = g->splits[d];
-    long long len = (n + s - 1) / s;
-    len = (len + kCvTile - 1) / kCvTile * kCvTile;
-    if ((long long)(s - 1) * len >= n) fail(BK_ERR_STATE, "internal: empty reference split");
+    const long long len = g->split_len[d];
+    if ((long long)(s - 1) * len >= n || (long long)s * len < n) fail(BK_ERR_STATE, "internal: bad reference split");
   }
 }
 

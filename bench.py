@@ -11,6 +11,13 @@ points to the device.  One step = one full 1M x 1M evaluation
 Epanechnikov product kernel (hx = 0.08, hy = 0.12) of the same
 configuration is timed after it and reported beside it.
 
+Also reported (object "cv_grid"): the batched cross-validation grid, four
+seeded mixtures of 50k, 200k, 500k and 1M points x the 8 x 8 log grid
+hx, hy in 0.01 * 50^(k/7) (GaussianProductKernel2d), LCV and convolution
+LSCV for all 256 (dataset, bandwidth pair) cases in one device pipeline
+(bk_cv_grid_run, graph replay).  Its unit is the (ordered pair, bandwidth
+pair) evaluation: sum n^2 x 64 = 8.27e13 per step.
+
 Prints one JSON line (see the keys below).  --gpus above 1 is refused: this
 build measures one GPU.
 """
@@ -41,6 +48,12 @@ PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 peak (vector and matrix)
 # 256 MiB copy kernel's readings (fetch x2.00, write x1.00):
 # 391,710 KiB fetched + 265,625 KiB written.
 GAUSS_HBM_BYTES_PER_LAUNCH = (391_710 + 265_625) * 1024
+# VALU issue peak of MI355X without packed FP32: 256 CUs x 4 SIMDs x 16
+# lanes per cycle x 2.4 GHz (lane-instructions per second)
+PEAK_VALU_LANE_INSTR = 256 * 4 * 16 * 2.4e9
+CV_SIZES = (50_000, 200_000, 500_000, 1_000_000)
+CV_SEED = 10040
+CV_GRID = 0.01 * 50.0 ** (np.arange(8) / 7.0)
 
 
 def parse_args():
@@ -50,7 +63,7 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
-    ap.add_argument("--kernels", default="gauss,epanprod", help="comma list: gauss, epanprod")
+    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid", help="comma list: gauss, epanprod, cvgrid")
     ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
                     help="Gaussian exponent form (auto: tile-centred on compact tiles)")
     return ap.parse_args()
@@ -144,6 +157,109 @@ def cpu_baseline(pts, mass, q, gpu_vals, seconds: float):
     return result
 
 
+def cv_datasets():
+    from bbrcit_kde_amd import datasets as ds
+
+    sets = []
+    for k, n in enumerate(CV_SIZES):
+        p, w = ds.gaussian_mixture(n, seed=CV_SEED + k)
+        sets.append((p, w / w.sum()))
+    return sets
+
+
+def cv_instr_per_ref(A: int, B: int) -> int:
+    """VALU instructions per reference and lane of the Gaussian grid kernel
+    (non-adaptive): 2 differences, 2 squares, A (fma + exp) for the x-factors,
+    B (mul + exp) for the y-factors, A*B fma; matches the kernel's gfx950 ISA."""
+    return 4 + 2 * A + 2 * B + A * B
+
+
+def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
+    from bbrcit_kde_amd import datasets as ds
+
+    sets = cv_datasets()
+    units = float(sum(n * n for n in CV_SIZES)) * len(CV_GRID) ** 2
+    res = {}
+    with bk.CvGrid(bk.BK_GAUSSIAN_PRODUCT_2D, [p for p, _ in sets], [w for _, w in sets]) as cv:
+        # eager pass: the persistent pair kernel timed alone with HIP events on
+        # the plan's stream (it is also the autotuning run)
+        cv.set_option(bk.BK_CVG_GRAPH, 0)
+        lcv, lscv = cv.run(CV_GRID, CV_GRID)
+        main_ms, _, rescued = cv.last_timing()
+        variant = cv.get_option(bk.BK_CVG_VARIANT)
+        cv.set_option(bk.BK_CVG_GRAPH, 1)
+        for _ in range(warmup):
+            cv.run(CV_GRID, CV_GRID)
+        ms = []
+        for _ in range(steps):
+            lcv2, lscv2 = cv.run(CV_GRID, CV_GRID)
+            ms.append(cv.last_timing()[1])
+        assert np.array_equal(lcv, lcv2) and np.array_equal(lscv, lscv2), "graph replay differs from eager run"
+    avg = float(np.mean(ms))
+    A, B = (8, 8) if variant == 0 else (4, 8)
+    instr = cv_instr_per_ref(A, B) / (A * B)
+    rate_main = units / (main_ms * 1e-3)
+    res.update({
+        "workload": "4 seeded 2-D mixtures (50k, 200k, 500k, 1M points, seeds 10040-10043) x 8x8 log grid "
+                    "hx, hy = 0.01*50^(k/7); GaussianProductKernel2d; LCV + convolution LSCV per case",
+        "unit": "pair-bw-evals/s", "value": units / (avg * 1e-3), "ms_per_step": avg, "steps": steps,
+        "units_per_step": units, "main_kernel_ms": main_ms, "rescued_entries": int(rescued),
+        "tiling": f"{A}x{B}", "graph": True,
+        "roofline": {"bound": "valu", "pipe": "FP32 VALU + v_exp_f32 issue",
+                     "achieved": rate_main * instr, "peak": PEAK_VALU_LANE_INSTR, "unit": "VALU lane-instr/s",
+                     "frac": rate_main * instr / PEAK_VALU_LANE_INSTR, "instr_per_pair_bw": instr,
+                     "note": "separable product kernel: 16 exp per reference serve 64 bandwidth pairs, so the "
+                             "unfactored 9 FLOP/pair accounting exceeds the FP32 peak",
+                     "unfactored_tflops": rate_main * 9 / 1e12},
+        "best_lcv": [list(map(int, np.unravel_index(np.argmax(np.where(np.isfinite(lcv[d]), lcv[d], -np.inf)),
+                                                    lcv[d].shape))) for d in range(len(CV_SIZES))],
+        "best_lscv": [list(map(int, np.unravel_index(np.argmin(lscv[d]), lscv[d].shape)))
+                      for d in range(len(CV_SIZES))],
+    })
+    # independent check: the single-bandwidth C-ABI path on the 50k dataset
+    p0, w0 = sets[0]
+    checks = []
+    with bk.DirectKde(p0, None, w0) as kde:
+        for ix, iy in [(2, 2), (4, 5), (6, 1)]:
+            desc = bk.KernelDesc(bk.BK_GAUSSIAN_PRODUCT_2D, 2, CV_GRID[ix], CV_GRID[iy])
+            checks.append(max(abs(lcv[0, ix, iy] - kde.likelihood_cv(desc, w0)),
+                              abs(lscv[0, ix, iy] - kde.lsq_conv_cv(desc, w0)) / abs(lscv[0, ix, iy])))
+    res["parity"] = {"vs": "bk_likelihood_cv / bk_lsq_conv_cv (single-bandwidth kernels), 50k dataset, 3 cases",
+                     "max_err": float(max(checks))}
+    if cpu:
+        from oracle.oracle import REF_LIB, Oracle, Reference, default_threads
+
+        # FP64 oracle on a 20k subset at one bandwidth pair (the grid on that subset)
+        n_s = 20_000
+        ps, ws = p0[:n_s], w0[:n_s] / w0[:n_s].sum()
+        h = CV_GRID[3]
+        with bk.CvGrid(bk.BK_GAUSSIAN_PRODUCT_2D, [ps], [ws]) as cv:
+            gl, gs = cv.run([h], [h])
+        orc = Oracle()
+        M = ds.round_f32(ws)
+        P = ds.round_f32(ps)
+        want_l = orc.likelihood_cv(bk.BK_GAUSSIAN_PRODUCT_2D, h, h, P, ws, mass=M, threads=default_threads())
+        want_s = orc.lsq_conv_cv(bk.BK_GAUSSIAN_PRODUCT_2D, h, h, P, ws, mass=M, threads=default_threads())
+        res["parity"]["oracle"] = {"sample": f"{n_s} points of the 50k dataset, hx=hy={h:.4f}",
+                                   "lcv_abs_err": float(abs(gl[0, 0, 0] - want_l)),
+                                   "lscv_rel_err": float(abs(gs[0, 0, 0] - want_s) / abs(want_s))}
+        if REF_LIB.exists():
+            ref = Reference()
+            n_r = 20_000
+            pr, wr = p0[:n_r], w0[:n_r] / w0[:n_r].sum()
+            hb = CV_GRID[2]
+            t0 = time.perf_counter()
+            ref.likelihood_cv(bk.BK_GAUSSIAN_PRODUCT_2D, 0, hb, hb, pr, wr, rel=1e-6, abs_err=1e-8, leaf_nmax=1024)
+            ref.lsq_conv_cv(bk.BK_GAUSSIAN_PRODUCT_2D, 0, hb, hb, pr, wr, rel=1e-6, abs_err=1e-8, leaf_nmax=1024)
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": n_r * n_r / dt, "unit": "pair-bw-evals/s", "cores": 1, "kind": "reference",
+                "sample": f"KernelDensity<2,GaussianProductKernel2d<double>>::likelihood_cross_validate + "
+                          f"lsq_convolution_cross_validate (rel 1e-6, abs 1e-8, leaf 1024, dual tree) on "
+                          f"{n_r} points at hx=hy={hb:.4f}: one bandwidth pair of the grid, 1 thread"}
+    return res
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -156,11 +272,15 @@ def main() -> int:
     log(f"host {platform.node()} cpus={os.cpu_count()}; generating 1M x 1M workload")
     pts, mass, q = make_workload()
     kinds = [k.strip() for k in args.kernels.split(",") if k.strip()]
+    if not any(k in kinds for k in ("gauss", "epanprod")):
+        kinds.insert(0, "gauss")
     results = {}
     with bk.DirectKde(pts, q, mass) as kde:
         if args.gauss_form == "direct":
             kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         for k in kinds:
+            if k == "cvgrid":
+                continue
             if k == "gauss":
                 desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS
             elif k == "epanprod":
@@ -174,6 +294,12 @@ def main() -> int:
             results[k] = {"vals": vals, "ms": ms, "avg_ms": avg_ms, "pair_evals_per_s": pairs / (avg_ms * 1e-3),
                           "tflops": pairs * flop / (avg_ms * 1e-3) / 1e12, "wall_s": wall}
             log(f"{k}: {avg_ms:.3f} ms/step, {pairs / (avg_ms * 1e-3):.4e} pair-evals/s")
+
+    cv_res = None
+    if "cvgrid" in kinds:
+        log(f"timing cvgrid: warmup {args.warmup}, steps {min(args.steps, 2)}")
+        cv_res = bench_cv_grid(bk, min(args.steps, 2), args.warmup, not args.no_cpu_baseline, args.cpu_seconds)
+        log(f"cvgrid: {cv_res['ms_per_step']:.1f} ms/step, {cv_res['value']:.4e} pair-bw-evals/s")
 
     head = results.get("gauss") or next(iter(results.values()))
     line = {
@@ -216,6 +342,8 @@ def main() -> int:
         line["parity"] = cb["parity"]
     else:
         line["cpu_baseline"] = None
+    if cv_res is not None:
+        line["cv_grid"] = cv_res
     print(json.dumps(line), flush=True)
     return 0
 

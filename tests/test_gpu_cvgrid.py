@@ -168,10 +168,11 @@ def test_underflowed_sums_are_rescued(bk, oracle):
             check_scores(lcv[0, ix, iy], lscv[0, ix, iy], want_lcv, want_lscv, bound, w, f"({ix},{iy})")
 
 
-def test_agrees_with_single_bandwidth_path_at_scale(bk):
-    """200k points: the grid's scores equal bk_likelihood_cv / bk_lsq_conv_cv
-    (an independent kernel path) at every checked bandwidth."""
-    pts, w = mixture(200_000, 1004)
+@pytest.mark.parametrize("n", [20_000, 33_333, 200_000])
+def test_agrees_with_single_bandwidth_path_at_scale(bk, n):
+    """One dataset, split over many work items: the grid's scores equal
+    bk_likelihood_cv / bk_lsq_conv_cv (an independent kernel path)."""
+    pts, w = mixture(n, 1004)
     h = np.array([0.01, 0.05, 0.2])
     with bk.CvGrid(GAUSSPROD, [pts], [w]) as cv:
         lcv, lscv = cv.run(h, h)
