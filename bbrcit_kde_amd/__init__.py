@@ -27,7 +27,7 @@ __all__ = [
     "BK_EPANECHNIKOV", "BK_GAUSSIAN", "BK_EPANECHNIKOV_PRODUCT_2D", "BK_GAUSSIAN_PRODUCT_2D",
     "BK_EPANECHNIKOV_CONV_1D", "BK_EPANECHNIKOV_PRODUCT_CONV_2D", "BK_GAUSSIAN_CONV_1D",
     "BK_GAUSSIAN_PRODUCT_CONV_2D", "BK_PREC_F32", "BK_PREC_F64", "BkError", "KernelDesc",
-    "DirectKde", "CvGrid", "load_library", "library_path",
+    "DirectKde", "CvGrid", "Surrogate", "load_library", "library_path",
 ]
 
 # ---- constants (mirror include/bbrcit_hip.h) ---------------------------------
@@ -42,6 +42,8 @@ BK_ERR_ALLOC = -6
 BK_OPT_GAUSS_FORM = 1
 BK_GAUSS_FORM_AUTO = 0
 BK_GAUSS_FORM_DIRECT = 1
+
+BK_MLP_NUM_PARAMS = 33537
 
 BK_CVG_VARIANT = 1
 BK_CVG_GRAPH = 2
@@ -102,6 +104,17 @@ class BkError(RuntimeError):
 _LIB: Optional[ctypes.CDLL] = None
 
 _DBL_P = ctypes.POINTER(ctypes.c_double)
+_FLT_P = ctypes.POINTER(ctypes.c_float)
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("n_samples", ctypes.c_size_t), ("n_fallback", ctypes.c_size_t),
+                ("cells", ctypes.c_size_t), ("cells_failed", ctypes.c_size_t), ("max_sample_err", ctypes.c_double),
+                ("ms_infer", ctypes.c_double), ("ms_check", ctypes.c_double), ("ms_fallback", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def library_path() -> Path:
@@ -150,6 +163,23 @@ def load_library() -> ctypes.CDLL:
         "bk_cv_grid_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "bk_cv_grid_last_timing": (ctypes.c_int, [ctypes.c_void_p, _DBL_P, _DBL_P,
                                                   ctypes.POINTER(ctypes.c_longlong)]),
+        "bk_mlp_create": (ctypes.c_int, [c_ctx_p, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]),
+        "bk_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+        "bk_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _FLT_P, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "bk_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _FLT_P, _DBL_P, _DBL_P, _DBL_P, _DBL_P, _DBL_P]),
+        "bk_mlp_set_training_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _DBL_P, _DBL_P,
+                                                    ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "bk_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_ulonglong, _DBL_P, _DBL_P]),
+        "bk_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _DBL_P, _DBL_P, _DBL_P, _FLT_P]),
+        "bk_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _DBL_P, _DBL_P]),
+        "bk_mlp_infer_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
+        "bk_mlp_serve_device": (ctypes.c_int, [ctypes.c_void_p, c_ctx_p, ctypes.POINTER(KernelDesc),
+                                               ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_ulonglong, ctypes.POINTER(ServeStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -418,3 +448,97 @@ class CvGrid:
         m, t, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
         _check(self._lib.bk_cv_grid_last_timing(self._plan, ctypes.byref(m), ctypes.byref(t), ctypes.byref(r)))
         return m.value, t.value, r.value
+
+
+class Surrogate:
+    """Learned density surrogate (``bk_mlp_*``): an MLP 2-128-128-128-1 with
+    BF16 weights, trained by the library's own HIP kernels on densities of
+    the exact direct sum and served with a sampled exact check."""
+
+    def __init__(self, seed: int = 0, *, device: int = 0):
+        lib = load_library()
+        self._lib = lib
+        self._ctx = ctypes.c_void_p()
+        self._mlp = ctypes.c_void_p()
+        _check(lib.bk_create(ctypes.byref(self._ctx), int(device), BK_PREC_F32))
+        try:
+            _check(lib.bk_mlp_create(self._ctx, int(seed), ctypes.byref(self._mlp)))
+        except Exception:
+            self.close()
+            raise
+
+    def close(self) -> None:
+        if getattr(self, "_mlp", None) is not None and self._mlp.value:
+            self._lib.bk_mlp_destroy(self._mlp)
+            self._mlp = ctypes.c_void_p()
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._lib.bk_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self):
+        """(theta float32[BK_MLP_NUM_PARAMS], lo, hi, mu, sigma, eps)."""
+        th = np.empty(BK_MLP_NUM_PARAMS, dtype=np.float32)
+        vals = [ctypes.c_double() for _ in range(5)]
+        _check(self._lib.bk_mlp_get_params(self._mlp, th.ctypes.data_as(_FLT_P), *[ctypes.byref(v) for v in vals]))
+        return (th,) + tuple(v.value for v in vals)
+
+    def set_params(self, theta, lo, hi, mu, sigma, eps) -> None:
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float32))
+        if th.shape != (BK_MLP_NUM_PARAMS,):
+            raise ValueError(f"expected {BK_MLP_NUM_PARAMS} parameters")
+        _check(self._lib.bk_mlp_set_params(self._mlp, th.ctypes.data_as(_FLT_P), float(lo), float(hi), float(mu),
+                                           float(sigma), float(eps)))
+
+    def set_training_data(self, q, f, lo: float, hi: float, eps: float) -> None:
+        q = _as_points(q)
+        f = _opt_vec(f, len(q))
+        _check(self._lib.bk_mlp_set_training_data(self._mlp, len(q), _dptr(q), _dptr(f), float(lo), float(hi),
+                                                  float(eps)))
+
+    def train(self, steps: int, batch: int, lr_max: float, lr_min: float, seed: int = 0):
+        """Returns (mean loss of the last batches, device milliseconds)."""
+        loss, ms = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.bk_mlp_train(self._mlp, int(steps), int(batch), float(lr_max), float(lr_min), int(seed),
+                                      ctypes.byref(loss), ctypes.byref(ms)))
+        return loss.value, ms.value
+
+    def loss_grad(self, x, t):
+        x = _as_points(x)
+        t = _opt_vec(t, len(x))
+        g = np.empty(BK_MLP_NUM_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(self._lib.bk_mlp_loss_grad(self._mlp, len(x), _dptr(x), _dptr(t), ctypes.byref(loss),
+                                          g.ctypes.data_as(_FLT_P)))
+        return loss.value, g
+
+    def forward(self, x) -> np.ndarray:
+        x = _as_points(x)
+        y = np.empty(len(x), dtype=np.float64)
+        _check(self._lib.bk_mlp_forward(self._mlp, len(x), _dptr(x), _dptr(y)))
+        return y
+
+    def infer_device(self, n: int, qx_ptr: int, qy_ptr: int, out_ptr: int) -> None:
+        _check(self._lib.bk_mlp_infer_device(self._mlp, int(n), ctypes.c_void_p(qx_ptr), ctypes.c_void_p(qy_ptr),
+                                             ctypes.c_void_p(out_ptr)))
+
+    def serve_device(self, kde: "DirectKde", desc: KernelDesc, n: int, qx_ptr: int, qy_ptr: int, out_ptr: int, *,
+                     sample_fraction: float = 0.01, rel_tol: float = 1e-3, floor: float, cells: int = 64,
+                     seed: int = 0) -> dict:
+        st = ServeStats()
+        _check(self._lib.bk_mlp_serve_device(self._mlp, kde._ctx, ctypes.byref(desc), int(n), ctypes.c_void_p(qx_ptr),
+                                             ctypes.c_void_p(qy_ptr), ctypes.c_void_p(out_ptr),
+                                             float(sample_fraction), float(rel_tol), float(floor), int(cells),
+                                             int(seed), ctypes.byref(st)))
+        return st.as_dict()

@@ -557,6 +557,48 @@ double reduce_to_host(bk_ctx *ctx, int mode, const double *w, const double *u, c
 
 }  // namespace
 
+namespace bk {
+// Normalised direct sums over every reference of an FP32 2-D context for
+// nq device-resident FP32 queries (qx, qy), written to out (device), all on
+// stream s (used by the surrogate's sampled check and tile fallback).
+void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, const float *qy, size_t nq,
+                      double *out, hipStream_t s) {
+  if (nq == 0) return;
+  const EvalParams p = params_of(desc);
+  if (ctx->precision != BK_PREC_F32 || ctx->refs.dim != 2 || p.dim != 2)
+    fail(BK_ERR_UNSUPPORTED, "device queries need an FP32 context with 2-D references");
+  if (ctx->refs.n == 0) fail(BK_ERR_STATE, "no reference points uploaded");
+  const hipStream_t saved = ctx->stream;
+  ctx->stream = s;
+  try {
+    ensure_features(ctx, desc, p);
+    const Path path = path_of(ctx, p);
+    const long long nr = (long long)ctx->refs.n;
+    const int splits = choose_splits((long long)nq, nr, queries_per_item(path), tile_of(path));
+    double *dst = out;
+    if (splits > 1) {
+      ctx->partial.reserve((size_t)splits * nq * sizeof(double));
+      dst = ctx->partial.as<double>();
+    }
+    const double scale = path_scale(ctx, path, p, 1);
+    hipError_t e;
+    if (path == Path::GAUSS32)
+      e = launch_gauss(p, qx, qy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr, 0, 0,
+                       nr, 0, (long long)nq, splits, dst, scale, 0,
+                       ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? GAUSS_DIRECT : GAUSS_AUTO, s);
+    else
+      e = launch_epan(p, qx, qy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr, 0, (long long)nq,
+                      splits, dst, scale, 0, s);
+    hip_check(e, "direct kernel launch");
+    if (splits > 1) hip_check(launch_reduce_splits(dst, splits, (long long)nq, out, s), "reduce launch");
+  } catch (...) {
+    ctx->stream = saved;
+    throw;
+  }
+  ctx->stream = saved;
+}
+}  // namespace bk
+
 // ---- C ABI -----------------------------------------------------------------
 
 extern "C" {

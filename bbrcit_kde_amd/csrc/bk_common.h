@@ -96,4 +96,9 @@ struct DeviceGuard {
 int ctx_device(const bk_ctx *ctx);
 hipStream_t ctx_stream(const bk_ctx *ctx);
 
+// Normalised direct sums over all references of an FP32 2-D context for nq
+// device-resident FP32 queries, into device memory `out`, on stream s.
+void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, const float *qy, size_t nq,
+                      double *out, hipStream_t s);
+
 }  // namespace bk

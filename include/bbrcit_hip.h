@@ -284,6 +284,87 @@ int bk_cv_grid_get_option(const bk_cv_grid *plan, int option, int *value);
 int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
                            double *total_ms, long long *rescued);
 
+/* ---- learned density surrogate ------------------------------------------ */
+/* No counterpart in the reference.  An MLP 2 -> 128 -> 128 -> 128 -> 1 with
+ * SiLU activations maps a 2-D query q to y, and the surrogate density is
+ *     f(q) = max(exp(mu + sigma y(q)) - eps, 0)
+ * with inputs mapped onto [-1, 1]^2 from the box [lo, hi]^2.  It is trained on
+ * (query, density) pairs produced by the exact direct sum (bk_eval_range),
+ * with the loss mean (y - t)^2, t = (ln(f + eps) - mu) / sigma.  Weights are
+ * BF16 (FP32 master copies for Adam), biases FP32, accumulation FP32.
+ * Parameter order (floats): W1[128][2], b1[128], W2[128][128], b2[128],
+ * W3[128][128], b3[128], W4[128], b4 (W[out][in]).  The exact direct sum is
+ * both the fallback and the judge of the surrogate (bk_mlp_serve_device). */
+#define BK_MLP_NUM_PARAMS 33537
+
+typedef struct bk_mlp bk_mlp;
+
+/* Statistics of one bk_mlp_serve_device call. */
+typedef struct bk_serve_stats {
+  size_t n;              /* queries served                                  */
+  size_t n_samples;      /* queries checked against the exact sum           */
+  size_t n_fallback;     /* queries re-evaluated exactly (failing cells)    */
+  size_t cells;          /* cells of the check grid                         */
+  size_t cells_failed;   /* cells whose sampled error exceeded rel_tol or
+                            that held no sample                             */
+  double max_sample_err; /* largest sampled error of the surrogate          */
+  double ms_infer, ms_check, ms_fallback, ms_total;  /* HIP-event times     */
+} bk_serve_stats;
+
+/* Create a surrogate on the context's device with Glorot-uniform weights
+ * drawn from `seed` and zero biases. */
+int bk_mlp_create(bk_ctx *ctx, unsigned long long seed, bk_mlp **out);
+int bk_mlp_destroy(bk_mlp *mlp);
+
+/* Load / save the FP32 master parameters (BK_MLP_NUM_PARAMS floats) and the
+ * input box and output normalisation.  Any output pointer may be NULL. */
+int bk_mlp_set_params(bk_mlp *mlp, const float *theta, double lo, double hi,
+                      double mu, double sigma, double eps);
+int bk_mlp_get_params(const bk_mlp *mlp, float *theta, double *lo,
+                      double *hi, double *mu, double *sigma, double *eps);
+
+/* Training set: n queries q (row-major x, y) with exact densities f >= 0;
+ * sets lo, hi, eps and mu, sigma (mean and deviation of ln(f + eps)). */
+int bk_mlp_set_training_data(bk_mlp *mlp, size_t n, const double *q,
+                             const double *f, double lo, double hi,
+                             double eps);
+
+/* `steps` Adam steps (beta 0.9 / 0.999) on batches of `batch` samples drawn
+ * from the training set by a seeded hash, learning rate cosine-decayed from
+ * lr_max to lr_min.  mean_loss_out: mean loss of the last 16 batches;
+ * ms_out: device time.  Either may be NULL. */
+int bk_mlp_train(bk_mlp *mlp, size_t steps, size_t batch, double lr_max,
+                 double lr_min, unsigned long long seed,
+                 double *mean_loss_out, double *ms_out);
+
+/* Loss and its gradient with respect to the parameters (straight through
+ * the BF16 rounding) for n samples x (row-major, already on the network's
+ * [-1, 1] input scale) with targets t: the training kernels' own forward and
+ * backward pass.  n must be a multiple of 256 (at most 65536). */
+int bk_mlp_loss_grad(bk_mlp *mlp, size_t n, const double *x,
+                     const double *t, double *loss_out, float *grad_out);
+
+/* Network outputs y for n inputs x on the [-1, 1] scale (serving kernel). */
+int bk_mlp_forward(bk_mlp *mlp, size_t n, const double *x, double *y_out);
+
+/* Surrogate densities for n device-resident FP32 queries into device
+ * memory out (FP64). */
+int bk_mlp_infer_device(bk_mlp *mlp, size_t n, const float *qx,
+                        const float *qy, double *out);
+
+/* Serve n device-resident queries: surrogate densities for all, then a
+ * deterministic sample of about sample_fraction of them is evaluated
+ * exactly by the direct sum of `ctx` (its FP32 2-D reference set, kernel
+ * desc) and replaces the surrogate there; the error of a sample is
+ * |s - e| / max(e, floor_density).  The box [lo, hi]^2 is cut into
+ * cells_per_axis^2 cells; every query of a cell whose largest sampled error
+ * exceeds rel_tol, or which holds no sample, is evaluated exactly. */
+int bk_mlp_serve_device(bk_mlp *mlp, bk_ctx *ctx, const bk_kernel_desc *desc,
+                        size_t n, const float *qx, const float *qy,
+                        double *out, double sample_fraction, double rel_tol,
+                        double floor_density, int cells_per_axis,
+                        unsigned long long seed, bk_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,14 @@ ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
+# Tests that hand device buffers to the library allocate them with torch.
+# torch must load its HIP runtime before libbbrcit_hip.so pulls in the system
+# one; in the other order torch finds no device in the same process.
+try:
+    import torch  # noqa: F401
+except Exception:  # torch is optional for the CPU tests
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (runs the HIP library)")
