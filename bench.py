@@ -54,6 +54,10 @@ PEAK_VALU_LANE_INSTR = 256 * 4 * 16 * 2.4e9
 CV_SIZES = (50_000, 200_000, 500_000, 1_000_000)
 CV_SEED = 10040
 CV_GRID = 0.01 * 50.0 ** (np.arange(8) / 7.0)
+SUR_TRAIN, SUR_SERVE = 4_000_000, 10_000_000
+SUR_STEPS, SUR_BATCH = 16_000, 16_384
+MLP_FLOP_PER_QUERY = 66_304     # 2 * (2*128 + 128*128 + 128*128 + 128)
+PEAK_BF16_TFLOPS = 2500.0       # MI355X dense BF16 MFMA
 
 
 def parse_args():
@@ -63,7 +67,8 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
-    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid", help="comma list: gauss, epanprod, cvgrid")
+    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate",
+                    help="comma list: gauss, epanprod, cvgrid, surrogate")
     ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
                     help="Gaussian exponent form (auto: tile-centred on compact tiles)")
     return ap.parse_args()
@@ -260,6 +265,81 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
     return res
 
 
+def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int):
+    import torch
+
+    from bbrcit_kde_amd import datasets as ds
+
+    desc = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS)
+    floor = 1e-10 * (1.0 / (2.0 * np.pi * H_GAUSS ** 2)) * float(ds.round_f32(mass).sum())
+    qt = ds.uniform_box(SUR_TRAIN, -4.0, 4.0, ds.SEED_MIXTURE_1M + 3)
+    t0 = time.perf_counter()
+    with bk.DirectKde(pts, qt, mass) as lab:
+        ft = lab.eval(desc, 0, len(pts) - 1, 0, len(qt) - 1)
+    t_label = time.perf_counter() - t0
+    qs = ds.uniform_box(SUR_SERVE, -4.0, 4.0, ds.SEED_MIXTURE_1M + 4)
+    qx = torch.tensor(qs[:, 0], dtype=torch.float32, device="cuda")
+    qy = torch.tensor(qs[:, 1], dtype=torch.float32, device="cuda")
+    out = torch.empty(SUR_SERVE, dtype=torch.float64, device="cuda")
+    raw = torch.empty(SUR_SERVE, dtype=torch.float64, device="cuda")
+    res = {}
+    with bk.DirectKde(pts, None, mass) as kde, bk.Surrogate(seed=1) as sur:
+        sur.set_training_data(qt, ft, -4.0, 4.0, 1e-3 * floor)
+        loss, train_ms = sur.train(SUR_STEPS, SUR_BATCH, 3e-3, 1e-5, seed=11)
+        sigma = sur.params()[4]
+        # raw inference, timed with events on the default stream around the call
+        sur.infer_device(SUR_SERVE, qx.data_ptr(), qy.data_ptr(), raw.data_ptr())
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        sur.infer_device(SUR_SERVE, qx.data_ptr(), qy.data_ptr(), raw.data_ptr())
+        ev1.record()
+        torch.cuda.synchronize()
+        infer_ms = ev0.elapsed_time(ev1)
+        for _ in range(warmup):
+            sur.serve_device(kde, desc, SUR_SERVE, qx.data_ptr(), qy.data_ptr(), out.data_ptr(), floor=floor,
+                             sample_fraction=0.01, rel_tol=1e-3, cells=64, seed=7)
+        stats = []
+        for _ in range(steps):
+            stats.append(sur.serve_device(kde, desc, SUR_SERVE, qx.data_ptr(), qy.data_ptr(), out.data_ptr(),
+                                          floor=floor, sample_fraction=0.01, rel_tol=1e-3, cells=64, seed=7))
+        torch.cuda.synchronize()
+        served = out.cpu().numpy()
+        rawv = raw.cpu().numpy()
+    ms = float(np.mean([st["ms_total"] for st in stats]))
+    st = stats[-1]
+    # independent audit: every 1000th query, exact direct sum
+    idx = np.arange(0, SUR_SERVE, 1000)
+    with bk.DirectKde(pts, qs[idx], mass) as ex:
+        fe = ex.eval(desc, 0, len(pts) - 1, 0, len(idx) - 1)
+    audit = np.abs(served[idx] - fe) / np.maximum(fe, floor)
+    raw_err = np.abs(rawv[idx] - fe) / np.maximum(fe, floor)
+    qps = SUR_SERVE / (ms * 1e-3)
+    raw_qps = SUR_SERVE / (infer_ms * 1e-3)
+    res.update({
+        "workload": "MLP 2-128-128-128-1 (SiLU, BF16 weights, FP32 accumulation) trained on 4M (query, density) "
+                    "pairs from the exact 1M-reference sum (h=0.05); serve 10M uniform queries on [-4,4]^2, 1% "
+                    "exact sample, exact fallback of 64x64 cells with sampled rel. error > 1e-3",
+        "unit": "queries/s", "value": qps, "ms_per_step": ms, "steps": steps,
+        "vs_direct": qps / direct_qps, "direct_queries_per_s": direct_qps,
+        "raw_inference_queries_per_s": raw_qps, "raw_vs_direct": raw_qps / direct_qps,
+        "label_s": t_label, "train_ms": train_ms, "train_steps": SUR_STEPS, "train_batch": SUR_BATCH,
+        "final_loss": loss, "log_rmse": float(np.sqrt(loss) * sigma),
+        "cells_failed": st["cells_failed"], "cells": st["cells"], "fallback_queries": st["n_fallback"],
+        "samples": st["n_samples"], "ms_infer": st["ms_infer"], "ms_check": st["ms_check"],
+        "ms_fallback": st["ms_fallback"],
+        "audit": {"queries": int(len(idx)), "served_frac_within_1e-3": float(np.mean(audit <= 1e-3)),
+                  "served_max_err": float(audit.max()), "surrogate_alone_frac_within_1e-3":
+                  float(np.mean(raw_err <= 1e-3)), "floor": floor},
+        "roofline": {"bound": "mfma", "kernel": "mlp_infer_kernel (BF16 MFMA 16x16x32, hi+lo activations)",
+                     "achieved": raw_qps * MLP_FLOP_PER_QUERY / 1e12, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": raw_qps * MLP_FLOP_PER_QUERY / 1e12 / PEAK_BF16_TFLOPS,
+                     "flop_per_query": MLP_FLOP_PER_QUERY,
+                     "note": "issued MFMA work is 2x the algorithmic FLOP (hi and lo activation halves)"},
+    })
+    return res
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -279,7 +359,7 @@ def main() -> int:
         if args.gauss_form == "direct":
             kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         for k in kinds:
-            if k == "cvgrid":
+            if k in ("cvgrid", "surrogate"):
                 continue
             if k == "gauss":
                 desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS
@@ -301,9 +381,17 @@ def main() -> int:
         cv_res = bench_cv_grid(bk, min(args.steps, 2), args.warmup, not args.no_cpu_baseline, args.cpu_seconds)
         log(f"cvgrid: {cv_res['ms_per_step']:.1f} ms/step, {cv_res['value']:.4e} pair-bw-evals/s")
 
+    sur_res = None
+    if "surrogate" in kinds and "gauss" in results:
+        log("surrogate: labels, training, serving")
+        sur_res = bench_surrogate(bk, pts, mass, results["gauss"]["pair_evals_per_s"] / N_REF, min(args.steps, 3),
+                                  args.warmup)
+        log(f"surrogate: {sur_res['ms_per_step']:.1f} ms per 10M, {sur_res['value']:.4e} q/s "
+            f"({sur_res['vs_direct']:.2f}x direct)")
+
     head = results.get("gauss") or next(iter(results.values()))
     line = {
-        "metric": "pair-evals/s 1Mx1M 2D FP32 vs FP64 oracle (GaussianKernel h=0.05)",
+        "metric": "pair-evals/s 1M\u00d71M 2D FP32 vs FP64 oracle; surrogate queries/s at \u22641e-3 rel",
         "value": head["pair_evals_per_s"],
         "unit": "pair-evals/s",
         "n_gpus": 1,
@@ -344,6 +432,8 @@ def main() -> int:
         line["cpu_baseline"] = None
     if cv_res is not None:
         line["cv_grid"] = cv_res
+    if sur_res is not None:
+        line["surrogate"] = sur_res
     print(json.dumps(line), flush=True)
     return 0
 
