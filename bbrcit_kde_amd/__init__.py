@@ -42,6 +42,9 @@ BK_ERR_ALLOC = -6
 BK_OPT_GAUSS_FORM = 1
 BK_GAUSS_FORM_AUTO = 0
 BK_GAUSS_FORM_DIRECT = 1
+BK_OPT_HIGH_DIM_FORM = 2
+BK_HIGH_DIM_AUTO = 0
+BK_HIGH_DIM_DIRECT = 1
 
 BK_MLP_NUM_PARAMS = 33537
 

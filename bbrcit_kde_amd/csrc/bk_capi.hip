@@ -64,6 +64,8 @@ struct bk_ctx {
 
   // options (bk_set_option)
   int gauss_form = BK_GAUSS_FORM_AUTO;
+  int high_dim_form = BK_HIGH_DIM_AUTO;
+  int feat_path = -1;  // Path of the cached features
 
   // staging
   DevBuf partial, out, items, work_a, work_b, weights, scratch, scalar;
@@ -332,11 +334,11 @@ bool same_desc(const bk_kernel_desc &a, const bk_kernel_desc &b) {
 }
 
 // Which kernel family evaluates a descriptor in this context.
-enum class Path { F64, GAUSS32, EPAN64, GENERAL };
+enum class Path { F64, GAUSS32, EPAN64, GENERAL, GRAM };
 
 Path path_of(const bk_ctx *ctx, const EvalParams &p) {
   if (ctx->precision == BK_PREC_F64) return Path::F64;
-  if (p.dim > 2) return Path::GENERAL;
+  if (p.dim > 2) return (p.dim >= 8 && ctx->high_dim_form == BK_HIGH_DIM_AUTO) ? Path::GRAM : Path::GENERAL;
   return p.family == bk::FAM_GAUSS ? Path::GAUSS32 : Path::EPAN64;
 }
 
@@ -348,18 +350,21 @@ int queries_per_item(Path path) {
 
 int tile_of(Path path) {
   if (path == Path::GAUSS32) return bk::kGaussTile;
-  return path == Path::GENERAL ? 128 : 256;
+  return (path == Path::GENERAL || path == Path::GRAM) ? 128 : 256;
 }
 
 // Make sure the per-reference features of `desc` are on the device.
 void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &p) {
   PointSet &r = ctx->refs;
-  if (ctx->feat_valid && same_desc(ctx->feat_desc, *desc) && ctx->feat_version == r.version) return;
+  const Path want = path_of(ctx, p);
+  if (ctx->feat_valid && same_desc(ctx->feat_desc, *desc) && ctx->feat_version == r.version &&
+      ctx->feat_path == (int)want)
+    return;
   ctx->feat_valid = false;
   const size_t n = r.n;
   if (n == 0) return;
   const hipStream_t s = ctx->stream;
-  switch (path_of(ctx, p)) {
+  switch (want) {
     case Path::F64:
       ctx->f0.reserve(n * sizeof(double4));
       ctx->f1.reserve(n * sizeof(double4));
@@ -390,7 +395,14 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
       hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, ctx->f0.as<double2>(), s),
                 "prep_general launch");
       break;
+    case Path::GRAM:
+      ctx->f0.reserve(n * sizeof(double4));
+      hip_check(bk::launch_prep_gram(p, r.coords.as<float>(), r.abw.as<float>(), r.m.as<float>(), n,
+                                     ctx->f0.as<double4>(), s),
+                "prep_gram launch");
+      break;
   }
+  ctx->feat_path = (int)want;
   ctx->feat_desc = *desc;
   ctx->feat_version = r.version;
   ctx->feat_valid = true;
@@ -452,6 +464,9 @@ hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkIt
     case Path::EPAN64:
       return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<double4>(), ctx->f1.as<double4>(),
                              items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
+    case Path::GRAM:
+      return bk::launch_gram(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double4>(), items, n_items,
+                             r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
     case Path::GENERAL:
     default:
       return bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double2>(),
@@ -661,6 +676,11 @@ int bk_set_option(bk_ctx *ctx, int option, int value) {
         if (value != BK_GAUSS_FORM_AUTO && value != BK_GAUSS_FORM_DIRECT)
           fail(BK_ERR_INVALID_ARG, "BK_OPT_GAUSS_FORM takes BK_GAUSS_FORM_AUTO or BK_GAUSS_FORM_DIRECT");
         ctx->gauss_form = value;
+        break;
+      case BK_OPT_HIGH_DIM_FORM:
+        if (value != BK_HIGH_DIM_AUTO && value != BK_HIGH_DIM_DIRECT)
+          fail(BK_ERR_INVALID_ARG, "BK_OPT_HIGH_DIM_FORM takes BK_HIGH_DIM_AUTO or BK_HIGH_DIM_DIRECT");
+        ctx->high_dim_form = value;
         break;
       default:
         fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));

@@ -119,6 +119,14 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
 hipError_t launch_prep_general(const EvalParams &p, const float *abw,
                                const float *m, size_t n, double2 *wm,
                                hipStream_t s);
+// Gram form for 8 <= D <= 16 (FP64 MFMA dot products); rw per reference:
+// (|r|^2, g / a^2, m, 0).
+hipError_t launch_prep_gram(const EvalParams &p, const float *rc, const float *abw, const float *m, size_t n,
+                            double4 *rw, hipStream_t s);
+hipError_t launch_gram(const EvalParams &p, const float *qc, const float *rc, const double4 *rw,
+                       const WorkItem *items, int n_items, long long r_i, long long r_end, long long q_i,
+                       long long nq, int splits, double *dst, double out_scale, int self_excl, hipStream_t s);
+
 hipError_t launch_direct_general(const EvalParams &p, const float *qc,
                                  const float *rc, const double2 *wm,
                                  const WorkItem *items, int n_items,

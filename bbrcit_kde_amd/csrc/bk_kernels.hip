@@ -716,6 +716,139 @@ __global__ void prep_general_kernel(EvalParams p, const float *__restrict__ abw,
   wm[i] = make_double2(p.gx / (a * a), (double)m[i]);
 }
 
+// ---- Gram form for 8 <= D <= 16: FP64 MFMA dot products ----------------------
+// |q - r|^2 = |q|^2 + |r|^2 - 2 q.r, with the q.r of 16 references x 16
+// queries per v_mfma_f64_16x16x4f64 chain (K = padded D in steps of 4).
+// FP64 products of FP32 coordinates are exact and the accumulation is FP64,
+// so the Gram form loses nothing to cancellation (an FP32 Gram matrix leaves
+// ~1e-5 relative error in the terms at d = 16); the matrix pipe runs the
+// dot products beside the VALU work of the terms.  Layout of the FP64
+// 16x16x4 MFMA (gfx950): lane l gives A[m = l%16][k = l/16] and
+// B[k = l/16][n = l%16] and receives D[l/16 + 4i][l%16], i = 0..3 (the FP32
+// and BF16 16x16 forms return rows 4(l/16) + i instead).  References sit on
+// M, queries on N, so a lane's four results belong to its own query.
+typedef __attribute__((__vector_size__(4 * sizeof(double)))) double v4d;
+
+// 2^e for e <= 0 over the FP64 range: FP32 exp2 of the fraction, exact
+// power of two by ldexp in FP64.
+__device__ __forceinline__ double exp2_wide(double e) {
+  const double n = __builtin_fmax(__builtin_floor(e), -2000.0);
+  return __builtin_ldexp((double)__builtin_amdgcn_exp2f((float)(e - n)), (int)n);
+}
+
+constexpr int kGramGroups = 4;  // groups of 16 queries per wavefront (64 queries)
+
+template <int DP, int FAM>
+__device__ __forceinline__ void gram_item(const float *__restrict__ qc, const float *__restrict__ rc,
+                                          const double4 *__restrict__ rw, int D, long long r_begin,
+                                          long long r_end, long long q_begin, int q_count,
+                                          double *__restrict__ dst, double out_scale, int self_excl) {
+  constexpr int KS = DP / 4;
+  __shared__ double sr[kTileG][DP + 1];
+  __shared__ double4 sw[kTileG];
+  const int t = threadIdx.x, wave = t / 64, lane = t % 64, g16 = lane / 16, n16 = lane % 16;
+  double bq[kGramGroups][KS], qn[kGramGroups], acc[kGramGroups];
+  long long qidx[kGramGroups];
+  bool qok[kGramGroups];
+#pragma unroll
+  for (int c = 0; c < kGramGroups; ++c) {
+    const int qi = wave * 64 + c * 16 + n16;
+    qok[c] = qi < q_count;
+    const long long q = q_begin + (qok[c] ? qi : 0);
+    qidx[c] = q;
+    double nrm = 0.0;
+    for (int k = 0; k < D; ++k) {
+      const double v = (double)qc[q * D + k];
+      nrm = __builtin_fma(v, v, nrm);
+    }
+    qn[c] = nrm;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + g16;
+      bq[c][s] = k < D ? (double)qc[q * D + k] : 0.0;
+    }
+    acc[c] = 0.0;
+  }
+  for (long long tile = r_begin; tile < r_end; tile += kTileG) {
+    const int cnt = (int)((r_end - tile) < kTileG ? (r_end - tile) : kTileG);
+    for (int e = t; e < kTileG * DP; e += kBlockG) {
+      const int j = e / DP, k = e % DP;
+      sr[j][k] = (j < cnt && k < D) ? (double)rc[(tile + j) * D + k] : 0.0;
+    }
+    for (int e = t; e < kTileG; e += kBlockG) sw[e] = e < cnt ? rw[tile + e] : make_double4(0.0, 0.0, 0.0, 0.0);
+    __syncthreads();
+    for (int sub = 0; sub < kTileG; sub += 16) {
+      double a[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) a[s] = sr[sub + n16][4 * s + g16];
+#pragma unroll
+      for (int c = 0; c < kGramGroups; ++c) {
+        v4d d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bq[c][s], d, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int j = sub + g16 + 4 * i;
+          const double4 w = sw[j];  // (|r|^2, g, m, 0)
+          const double d2 = __builtin_fmax(__builtin_fma(-2.0, d[i], qn[c] + w.x), 0.0);
+          double v;
+          if constexpr (FAM == FAM_GAUSS)
+            v = w.z * exp2_wide(-w.y * d2);
+          else
+            v = w.z * __builtin_fmax(__builtin_fma(-w.y, d2, 1.0), 0.0);
+          if (!(self_excl && tile + j == qidx[c])) acc[c] += v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < kGramGroups; ++c) {
+    double v = acc[c];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (g16 == 0 && qok[c]) dst[wave * 64 + c * 16 + n16] = v * out_scale;
+  }
+}
+
+template <int DP, int FAM>
+__global__ __launch_bounds__(kBlockG) void gram_range_kernel(
+    const float *__restrict__ qc, const float *__restrict__ rc, const double4 *__restrict__ rw, int D,
+    long long r_i, long long r_end, long long split_len, long long q_i, long long nq, double *__restrict__ dst,
+    double out_scale, int self_excl) {
+  const long long qb = (long long)blockIdx.x * kBlockG;
+  const long long rb = r_i + (long long)blockIdx.y * split_len;
+  long long re = rb + split_len;
+  if (re > r_end) re = r_end;
+  const long long left = nq - qb;
+  const int q_count = (int)(left < kBlockG ? left : kBlockG);
+  gram_item<DP, FAM>(qc, rc, rw, D, rb, re, q_i + qb, q_count, dst + (long long)blockIdx.y * nq + qb, out_scale,
+                     self_excl);
+}
+
+template <int DP, int FAM>
+__global__ __launch_bounds__(kBlockG) void gram_list_kernel(
+    const float *__restrict__ qc, const float *__restrict__ rc, const double4 *__restrict__ rw, int D,
+    const WorkItem *__restrict__ items, double *__restrict__ dst, double out_scale, int self_excl) {
+  const WorkItem it = items[blockIdx.x];
+  gram_item<DP, FAM>(qc, rc, rw, D, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index, out_scale,
+                     self_excl);
+}
+
+// Per reference: (|r|^2 in FP64 of the FP32 coordinates, g / a^2, m, 0).
+__global__ void prep_gram_kernel(EvalParams p, const float *__restrict__ rc, const float *__restrict__ abw,
+                                 const float *__restrict__ m, long long n, int D, double4 *__restrict__ rw) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double nrm = 0.0;
+  for (int k = 0; k < D; ++k) {
+    const double v = (double)rc[i * D + k];
+    nrm = __builtin_fma(v, v, nrm);
+  }
+  const double a = (double)abw[i];
+  rw[i] = make_double4(nrm, p.gx / (a * a), (double)m[i], 0.0);
+}
+
 // ---- FP64 rescue of underflowed Gaussian sums ---------------------------------
 __global__ void zero_int_kernel(int *p) { *p = 0; }
 
@@ -1031,6 +1164,57 @@ hipError_t launch_direct_general(const EvalParams &p, const float *qc,
       return hipErrorInvalidValue;
   }
 #undef BK_GD
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_gram(const EvalParams &p, const float *rc, const float *abw, const float *m, size_t n,
+                            double4 *rw, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int th = 256;
+  prep_gram_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, rc, abw, m, (long long)n, p.dim, rw);
+  return hipGetLastError();
+}
+
+namespace {
+template <int DP, int FAM>
+void gram_dispatch(const EvalParams &p, const float *qc, const float *rc, const double4 *rw, const WorkItem *items,
+                   int n_items, long long r_i, long long r_end, long long q_i, long long nq, int splits, double *dst,
+                   double out_scale, int self_excl, hipStream_t s) {
+  if (items) {
+    gram_list_kernel<DP, FAM><<<n_items, kBlockG, 0, s>>>(qc, rc, rw, p.dim, items, dst, out_scale, self_excl);
+  } else {
+    const long long len = split_length(r_i, r_end, splits, kTileG);
+    const dim3 grid((unsigned)((nq + kBlockG - 1) / kBlockG), (unsigned)splits);
+    gram_range_kernel<DP, FAM><<<grid, kBlockG, 0, s>>>(qc, rc, rw, p.dim, r_i, r_end, len, q_i, nq, dst, out_scale,
+                                                        self_excl);
+  }
+}
+}  // namespace
+
+hipError_t launch_gram(const EvalParams &p, const float *qc, const float *rc, const double4 *rw,
+                       const WorkItem *items, int n_items, long long r_i, long long r_end, long long q_i,
+                       long long nq, int splits, double *dst, double out_scale, int self_excl, hipStream_t s) {
+  if ((items && n_items <= 0) || (!items && nq <= 0)) return hipSuccess;
+  if (p.dim < 8 || p.dim > 16) return hipErrorInvalidValue;
+  const int dp = (p.dim + 3) / 4 * 4;
+  const bool gauss = p.family == FAM_GAUSS;
+#define BK_GR(DPV)                                                                                              \
+  case DPV:                                                                                                     \
+    if (gauss)                                                                                                  \
+      gram_dispatch<DPV, FAM_GAUSS>(p, qc, rc, rw, items, n_items, r_i, r_end, q_i, nq, splits, dst, out_scale, \
+                                    self_excl, s);                                                              \
+    else                                                                                                        \
+      gram_dispatch<DPV, FAM_EPAN>(p, qc, rc, rw, items, n_items, r_i, r_end, q_i, nq, splits, dst, out_scale,  \
+                                   self_excl, s);                                                               \
+    break
+  switch (dp) {
+    BK_GR(8);
+    BK_GR(12);
+    BK_GR(16);
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef BK_GR
   return hipGetLastError();
 }
 

@@ -75,6 +75,9 @@ extern "C" {
 #define BK_OPT_GAUSS_FORM      1  /* how FP32 Gaussian exponents are formed:         */
 #define BK_GAUSS_FORM_AUTO     0  /*   tile-centred form on compact tiles (default)  */
 #define BK_GAUSS_FORM_DIRECT   1  /*   differences q - r on every tile               */
+#define BK_OPT_HIGH_DIM_FORM   2  /* distances for 8 <= D <= 16 (FP32 contexts):      */
+#define BK_HIGH_DIM_AUTO       0  /*   Gram form, dot products on FP64 MFMA (default) */
+#define BK_HIGH_DIM_DIRECT     1  /*   differences q - r on the FP64 VALU             */
 
 /* ---- POD types -------------------------------------------------------- */
 typedef struct bk_ctx bk_ctx;

@@ -154,19 +154,46 @@ def test_invariants_linearity_permutation(bk):
     np.testing.assert_allclose(sp[big], s12[big], rtol=3e-6)
 
 
-@pytest.mark.parametrize("kind,dim", [(GAUSS, 3), (GAUSS, 8), (GAUSS, 16), (EPAN, 5), (EPAN, 16)])
-def test_general_dimension(bk, oracle, kind, dim):
+@pytest.mark.parametrize("form", ["auto", "direct"])
+@pytest.mark.parametrize("kind,dim", [(GAUSS, 3), (GAUSS, 8), (GAUSS, 9), (GAUSS, 12), (GAUSS, 16), (EPAN, 5),
+                                      (EPAN, 11), (EPAN, 16)])
+def test_general_dimension(bk, oracle, kind, dim, form):
+    # D >= 8 runs the Gram form (FP64 MFMA dot products) by default and the
+    # FP64 difference form with BK_HIGH_DIM_DIRECT; D < 8 always the latter.
     rng = np.random.default_rng(dim)
-    pts = rng.standard_normal((2000, dim))
-    q = rng.standard_normal((700, dim))
+    pts = rng.standard_normal((2001, dim))
+    q = rng.standard_normal((701, dim))
     mass = np.full(len(pts), 1.0 / len(pts))
     abw = rng.uniform(0.8, 1.25, len(pts))
     h = 0.5 if kind == GAUSS else 2.0
     desc = bk.KernelDesc(kind, dim, h)
     with bk.DirectKde(pts, q, mass, abw) as kde:
+        if form == "direct":
+            kde.set_option(bk.BK_OPT_HIGH_DIM_FORM, bk.BK_HIGH_DIM_DIRECT)
         got = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        sub = kde.eval(desc, 17, 1500, 3, 650)
     want = oracle.direct(kind, h, h, f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
-    assert_parity(got, want, oracle.normalization(kind, dim, h, h) * mass.sum(), what=f"D={dim}")
+    bound = oracle.normalization(kind, dim, h, h) * mass.sum()
+    assert_parity(got, want, bound, what=f"D={dim} {form}")
+    want_sub = oracle.direct(kind, h, h, f32(pts[17:1501]), f32(q[3:651]), mass=f32(mass[17:1501]),
+                             abw=f32(abw[17:1501]), normalize=True)
+    assert_parity(sub, want_sub, bound, what=f"D={dim} {form} sub-range")
+
+
+@pytest.mark.parametrize("dim", [8, 16])
+def test_gram_leave_one_out_and_forms_agree(bk, oracle, dim):
+    rng = np.random.default_rng(100 + dim)
+    pts = rng.standard_normal((3000, dim))
+    mass = np.full(len(pts), 1.0 / len(pts))
+    desc = bk.KernelDesc(GAUSS, dim, 0.5)
+    with bk.DirectKde(pts, None, mass) as kde:
+        loo = kde.self_eval(desc, exclude_self=True, normalize=True)
+        kde.set_option(bk.BK_OPT_HIGH_DIM_FORM, bk.BK_HIGH_DIM_DIRECT)
+        loo_direct = kde.self_eval(desc, exclude_self=True, normalize=True)
+    want = oracle.direct(GAUSS, 0.5, 0.5, f32(pts), f32(pts), mass=f32(mass), exclude_self=True, normalize=True)
+    bound = oracle.normalization(GAUSS, dim, 0.5, 0.5)
+    assert_parity(loo, want, bound, what=f"Gram LOO D={dim}")
+    assert_parity(loo_direct, want, bound, what=f"direct LOO D={dim}")
 
 
 def test_mixture_1m_refs_sampled_queries(bk, oracle):
