@@ -738,6 +738,30 @@ __device__ __forceinline__ double exp2_wide(double e) {
 
 constexpr int kGramGroups = 4;  // groups of 16 queries per wavefront (64 queries)
 
+// Terms of one 16 x 16 block of (reference, query) pairs from the dot
+// products d (lane layout above).  Gaussian: w = (|r|^2, g / a^2, log2 m):
+// m 2^(-g d2) = exp2(log2 m - g d2), the exponent formed in FP64 and rounded
+// to FP32 once (|error| <= 2^-24 |e|: 2.6e-6 relative at e = -128, below
+// which a term is < 2^-126 m and far under any density floor).
+// Epanechnikov: w = (|r|^2, g / a^2, m): m max(1 - g d2, 0) in FP64.
+template <int FAM, bool DIAG>
+__device__ __forceinline__ void gram_terms(const v4d &d, const double4 (&ws)[4], int jb, double qn, long long tile,
+                                           long long q, double &acc) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = jb + 4 * i;
+    const double4 w = ws[i];
+    const double d2 = __builtin_fma(-2.0, d[i], qn + w.x);
+    double v;
+    if constexpr (FAM == FAM_GAUSS)
+      v = (double)__builtin_amdgcn_exp2f((float)__builtin_fma(-w.y, d2, w.z));
+    else
+      v = w.z * __builtin_fmax(__builtin_fma(-w.y, d2, 1.0), 0.0);
+    if (DIAG) v = (tile + j == q) ? 0.0 : v;
+    acc += v;
+  }
+}
+
 template <int DP, int FAM>
 __device__ __forceinline__ void gram_item(const float *__restrict__ qc, const float *__restrict__ rc,
                                           const double4 *__restrict__ rw, int D, long long r_begin,
@@ -769,35 +793,35 @@ __device__ __forceinline__ void gram_item(const float *__restrict__ qc, const fl
     }
     acc[c] = 0.0;
   }
+  const double4 pad = FAM == FAM_GAUSS ? make_double4(0.0, 0.0, -__builtin_inf(), 0.0)
+                                       : make_double4(0.0, 0.0, 0.0, 0.0);
+  const long long q_end = q_begin + q_count;
   for (long long tile = r_begin; tile < r_end; tile += kTileG) {
     const int cnt = (int)((r_end - tile) < kTileG ? (r_end - tile) : kTileG);
     for (int e = t; e < kTileG * DP; e += kBlockG) {
       const int j = e / DP, k = e % DP;
       sr[j][k] = (j < cnt && k < D) ? (double)rc[(tile + j) * D + k] : 0.0;
     }
-    for (int e = t; e < kTileG; e += kBlockG) sw[e] = e < cnt ? rw[tile + e] : make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int e = t; e < kTileG; e += kBlockG) sw[e] = e < cnt ? rw[tile + e] : pad;
     __syncthreads();
+    const bool diag = self_excl && tile < q_end && q_begin < tile + kTileG;
     for (int sub = 0; sub < kTileG; sub += 16) {
       double a[KS];
 #pragma unroll
       for (int s = 0; s < KS; ++s) a[s] = sr[sub + n16][4 * s + g16];
+      // the four references of this lane's result rows, shared by the query groups
+      double4 ws[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ws[i] = sw[sub + g16 + 4 * i];
 #pragma unroll
       for (int c = 0; c < kGramGroups; ++c) {
         v4d d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < KS; ++s) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bq[c][s], d, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int j = sub + g16 + 4 * i;
-          const double4 w = sw[j];  // (|r|^2, g, m, 0)
-          const double d2 = __builtin_fmax(__builtin_fma(-2.0, d[i], qn[c] + w.x), 0.0);
-          double v;
-          if constexpr (FAM == FAM_GAUSS)
-            v = w.z * exp2_wide(-w.y * d2);
-          else
-            v = w.z * __builtin_fmax(__builtin_fma(-w.y, d2, 1.0), 0.0);
-          if (!(self_excl && tile + j == qidx[c])) acc[c] += v;
-        }
+        if (diag)
+          gram_terms<FAM, true>(d, ws, sub + g16, qn[c], tile, qidx[c], acc[c]);
+        else
+          gram_terms<FAM, false>(d, ws, sub + g16, qn[c], tile, qidx[c], acc[c]);
       }
     }
     __syncthreads();
@@ -835,7 +859,8 @@ __global__ __launch_bounds__(kBlockG) void gram_list_kernel(
                      self_excl);
 }
 
-// Per reference: (|r|^2 in FP64 of the FP32 coordinates, g / a^2, m, 0).
+// Per reference: (|r|^2 in FP64 of the FP32 coordinates, g / a^2, w, 0)
+// with w = log2 m (Gaussian; -inf for m = 0) or m (Epanechnikov).
 __global__ void prep_gram_kernel(EvalParams p, const float *__restrict__ rc, const float *__restrict__ abw,
                                  const float *__restrict__ m, long long n, int D, double4 *__restrict__ rw) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -846,7 +871,9 @@ __global__ void prep_gram_kernel(EvalParams p, const float *__restrict__ rc, con
     nrm = __builtin_fma(v, v, nrm);
   }
   const double a = (double)abw[i];
-  rw[i] = make_double4(nrm, p.gx / (a * a), (double)m[i], 0.0);
+  const double mi = (double)m[i];
+  const double wv = p.family == FAM_GAUSS ? (mi > 0.0 ? log2(mi) : -__builtin_inf()) : mi;
+  rw[i] = make_double4(nrm, p.gx / (a * a), wv, 0.0);
 }
 
 // ---- FP64 rescue of underflowed Gaussian sums ---------------------------------
