@@ -58,6 +58,9 @@ SUR_TRAIN, SUR_SERVE = 4_000_000, 10_000_000
 SUR_STEPS, SUR_BATCH = 16_000, 16_384
 MLP_FLOP_PER_QUERY = 66_304     # 2 * (2*128 + 128*128 + 128*128 + 128)
 PEAK_BF16_TFLOPS = 2500.0       # MI355X dense BF16 MFMA
+PEAK_FP64_TFLOPS = 78.6         # MI355X FP64 (vector and matrix)
+G16_N, G16_D, G16_H = 200_000, 16, 0.5
+G16_FLOP_PER_PAIR = 3 * G16_D + 4   # 16 differences, 16 squares+adds, scale, exp, fma (exp as 1)
 
 
 def parse_args():
@@ -67,8 +70,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
-    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate",
-                    help="comma list: gauss, epanprod, cvgrid, surrogate")
+    ap.add_argument("--sur-steps", type=int, default=16_000, help="training steps of the surrogate")
+    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate,gauss16,adaptive",
+                    help="comma list: gauss, epanprod, cvgrid, surrogate, gauss16, adaptive")
     ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
                     help="Gaussian exponent form (auto: tile-centred on compact tiles)")
     return ap.parse_args()
@@ -265,7 +269,7 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
     return res
 
 
-def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int):
+def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int, train_steps: int = 16_000):
     import torch
 
     from bbrcit_kde_amd import datasets as ds
@@ -285,7 +289,7 @@ def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int):
     res = {}
     with bk.DirectKde(pts, None, mass) as kde, bk.Surrogate(seed=1) as sur:
         sur.set_training_data(qt, ft, -4.0, 4.0, 1e-3 * floor)
-        loss, train_ms = sur.train(SUR_STEPS, SUR_BATCH, 3e-3, 1e-5, seed=11)
+        loss, train_ms = sur.train(train_steps, SUR_BATCH, 3e-3, 1e-5, seed=11)
         sigma = sur.params()[4]
         # raw inference, timed with events on the default stream around the call
         sur.infer_device(SUR_SERVE, qx.data_ptr(), qy.data_ptr(), raw.data_ptr())
@@ -323,8 +327,8 @@ def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int):
         "unit": "queries/s", "value": qps, "ms_per_step": ms, "steps": steps,
         "vs_direct": qps / direct_qps, "direct_queries_per_s": direct_qps,
         "raw_inference_queries_per_s": raw_qps, "raw_vs_direct": raw_qps / direct_qps,
-        "label_s": t_label, "train_ms": train_ms, "train_steps": SUR_STEPS, "train_batch": SUR_BATCH,
-        "final_loss": loss, "log_rmse": float(np.sqrt(loss) * sigma),
+        "label_s": t_label, "train_ms": train_ms, "train_batch": SUR_BATCH,
+        "final_loss": loss, "log_rmse": float(np.sqrt(loss) * sigma), "train_steps_run": train_steps,
         "cells_failed": st["cells_failed"], "cells": st["cells"], "fallback_queries": st["n_fallback"],
         "samples": st["n_samples"], "ms_infer": st["ms_infer"], "ms_check": st["ms_check"],
         "ms_fallback": st["ms_fallback"],
@@ -337,6 +341,112 @@ def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int):
                      "flop_per_query": MLP_FLOP_PER_QUERY,
                      "note": "issued MFMA work is 2x the algorithmic FLOP (hi and lo activation halves)"},
     })
+    return res
+
+
+def bench_gauss16(bk, steps: int, warmup: int, cpu: bool, seconds: float):
+    import torch
+
+    from bbrcit_kde_amd import datasets as ds
+    from oracle.oracle import Oracle, default_threads
+
+    R = ds.standard_normal(G16_N, G16_D, 1004)
+    Q = ds.standard_normal(G16_N, G16_D, 1005)
+    m = np.full(G16_N, 1.0 / G16_N)
+    desc = bk.KernelDesc(bk.BK_GAUSSIAN, G16_D, G16_H)
+    out = torch.empty(G16_N, dtype=torch.float64, device="cuda")
+    res = {"workload": "d=16 GaussianKernel h=0.5, 200,000 x 200,000 standard-normal points (seeds 1004/1005)",
+           "unit": "pair-evals/s"}
+    pairs = float(G16_N) * G16_N
+    vals = {}
+    with bk.DirectKde(R, Q, m) as kde:
+        stream = torch.cuda.Stream()
+        kde.set_stream(stream.cuda_stream)
+        for name, form in (("gram_fp64_mfma", bk.BK_HIGH_DIM_AUTO), ("difference_fp64_valu", bk.BK_HIGH_DIM_DIRECT)):
+            kde.set_option(bk.BK_OPT_HIGH_DIM_FORM, form)
+            for _ in range(warmup):
+                kde.eval_device(desc, 0, G16_N - 1, 0, G16_N - 1, out.data_ptr(), True)
+            torch.cuda.synchronize()
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+            for e0, e1 in ev:
+                e0.record(stream)
+                kde.eval_device(desc, 0, G16_N - 1, 0, G16_N - 1, out.data_ptr(), True)
+                e1.record(stream)
+            torch.cuda.synchronize()
+            ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+            vals[name] = out.cpu().numpy()
+            res[name] = {"ms_per_step": ms, "pair_evals_per_s": pairs / (ms * 1e-3)}
+        kde.set_stream(0)
+    g = res["gram_fp64_mfma"]
+    res["value"] = g["pair_evals_per_s"]
+    res["ms_per_step"] = g["ms_per_step"]
+    res["speedup_vs_difference_form"] = g["pair_evals_per_s"] / res["difference_fp64_valu"]["pair_evals_per_s"]
+    res["roofline"] = {"bound": "mfma", "pipe": "FP64 MFMA (dot products) beside FP64/FP32 VALU (terms)",
+                       "achieved": g["pair_evals_per_s"] * 2 * G16_D / 1e12, "peak": PEAK_FP64_TFLOPS,
+                       "unit": "TFLOP/s", "frac": g["pair_evals_per_s"] * 2 * G16_D / 1e12 / PEAK_FP64_TFLOPS,
+                       "flop_per_pair_mfma": 2 * G16_D, "flop_per_pair_algorithmic": G16_FLOP_PER_PAIR,
+                       "algorithmic_tflops": g["pair_evals_per_s"] * G16_FLOP_PER_PAIR / 1e12}
+    # parity on 128 sampled queries against the FP64 oracle
+    idx = np.linspace(0, G16_N - 1, 128).astype(np.int64)
+    orc = Oracle()
+    t0 = time.perf_counter()
+    want = orc.direct(bk.BK_GAUSSIAN, G16_H, G16_H, ds.round_f32(R), ds.round_f32(Q[idx]), mass=ds.round_f32(m),
+                      normalize=True, threads=default_threads())
+    dt = time.perf_counter() - t0
+    norm = (2 * np.pi) ** (-G16_D / 2) * G16_H ** (-G16_D)
+    floor = 1e-10 * norm * float(ds.round_f32(m).sum())
+    above = want >= floor
+    res["parity"] = {"queries": int(len(idx)), "floor": floor, "frac_below_floor": float(1 - above.mean()),
+                     "tolerance": 1e-5}
+    for name, v in vals.items():
+        rel = np.abs(v[idx][above] - want[above]) / want[above]
+        res["parity"][f"max_rel_err_{name}"] = float(rel.max())
+    if cpu:
+        threads = default_threads()
+        rate = len(idx) * G16_N / dt
+        nq = int(min(G16_N, max(128, rate * seconds / G16_N)))
+        t0 = time.perf_counter()
+        orc.direct(bk.BK_GAUSSIAN, G16_H, G16_H, ds.round_f32(R), ds.round_f32(Q[:nq]), mass=ds.round_f32(m),
+                   normalize=True, threads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": nq * G16_N / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+                               "sample": f"{nq} queries x {G16_N} refs, FP64 restatement, {threads} threads (the "
+                                         "reference evaluates d=16 on the CPU only)"}
+    return res
+
+
+def bench_adaptive(bk, pts, mass, q):
+    from bbrcit_kde_amd import datasets as ds
+
+    import torch
+
+    desc = bk.KernelDesc(bk.BK_GAUSSIAN_PRODUCT_2D, 2, 0.05, 0.05)
+    n, nq = len(pts), len(q)
+    res = {"workload": "1M mixture points: pilot GaussianProductKernel2d (0.05, 0.05) with self term, "
+                       "lambda_i = (f_i/g)^-0.5 with FP64 geometric mean g, adaptive evaluation at 1M uniform "
+                       "queries on [-4,4]^2, leave-one-out LCV of the adapted density", "unit": "pair-evals/s"}
+    out = torch.empty(nq, dtype=torch.float64, device="cuda")
+    with bk.DirectKde(pts, None, mass) as kde:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g, lam = kde.adapt(desc, 0.5, mass)
+        t1 = time.perf_counter()
+        lcv = kde.likelihood_cv(desc, mass)
+        t2 = time.perf_counter()
+        a_mass = mass * lam ** -2.0
+    with bk.DirectKde(pts, q, a_mass, lam) as ev:
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        ev.eval_device(desc, 0, n - 1, 0, nq - 1, out.data_ptr(), True)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+    pairs = 2.0 * n * n + float(n) * nq
+    total = (t1 - t0) + (t2 - t1) + (t4 - t3)
+    res.update({"value": pairs / total, "pairs": pairs, "s_adapt": t1 - t0, "s_lcv": t2 - t1,
+                "s_adaptive_eval": t4 - t3, "g": g, "lcv": lcv,
+                "lambda_range": [float(lam.min()), float(lam.max())],
+                "timing": "host wall clock around each synchronous C-ABI call (includes the FP64 reductions and "
+                          "the host mass update of the adaptive pass)"})
     return res
 
 
@@ -359,7 +469,7 @@ def main() -> int:
         if args.gauss_form == "direct":
             kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         for k in kinds:
-            if k in ("cvgrid", "surrogate"):
+            if k in ("cvgrid", "surrogate", "gauss16", "adaptive"):
                 continue
             if k == "gauss":
                 desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS
@@ -385,9 +495,19 @@ def main() -> int:
     if "surrogate" in kinds and "gauss" in results:
         log("surrogate: labels, training, serving")
         sur_res = bench_surrogate(bk, pts, mass, results["gauss"]["pair_evals_per_s"] / N_REF, min(args.steps, 3),
-                                  args.warmup)
+                                  args.warmup, args.sur_steps)
         log(f"surrogate: {sur_res['ms_per_step']:.1f} ms per 10M, {sur_res['value']:.4e} q/s "
             f"({sur_res['vs_direct']:.2f}x direct)")
+
+    g16_res = ad_res = None
+    if "gauss16" in kinds:
+        log("timing gauss16 (d=16, 200k x 200k)")
+        g16_res = bench_gauss16(bk, min(args.steps, 3), args.warmup, not args.no_cpu_baseline, args.cpu_seconds)
+        log(f"gauss16: {g16_res['ms_per_step']:.2f} ms/step, {g16_res['value']:.4e} pair-evals/s")
+    if "adaptive" in kinds:
+        log("adaptive pass + LOO LCV + adaptive evaluation (1M)")
+        ad_res = bench_adaptive(bk, pts, mass, q)
+        log(f"adaptive: {ad_res['value']:.4e} pair-evals/s")
 
     head = results.get("gauss") or next(iter(results.values()))
     line = {
@@ -434,6 +554,10 @@ def main() -> int:
         line["cv_grid"] = cv_res
     if sur_res is not None:
         line["surrogate"] = sur_res
+    if g16_res is not None:
+        line["gauss16"] = g16_res
+    if ad_res is not None:
+        line["adaptive"] = ad_res
     print(json.dumps(line), flush=True)
     return 0
 
