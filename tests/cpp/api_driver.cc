@@ -7,6 +7,7 @@
 #include <random>
 #include <vector>
 
+#include <CvGrid.h>
 #include <HipDirectKde.h>
 #include <KernelDensity.h>
 #include <Kernels/ConvKernelAssociator.h>
@@ -342,6 +343,59 @@ int api_hip_direct_range(int kind, int dim, int kf, double hx, double hy, const 
       }
       return 0;
     });
+  });
+}
+
+// Grid search over (hx, hy) pairs with the product kernels (kind 3 or 4),
+// kernel_float 0: scores from lsq_convolution_cross_validate_grid /
+// likelihood_cross_validate_grid (one batched device pipeline) and, for
+// comparison, from the per-bandwidth KernelDensity calls grid_search makes.
+int api_cv_grid(int kind, const double *pts, const double *w, size_t n, int leaf_nmax, const double *hx,
+                const double *hy, size_t npairs, int adapt, double alpha, double *lcv_grid, double *lscv_grid,
+                double *lcv_loop, double *lscv_loop) {
+  return guarded([&] {
+    auto fn = [&](auto kernel_tag) -> int {
+      using K = decltype(kernel_tag);
+      using KDE = KernelDensity<2, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      if (adapt) {
+        kde.kernel().set_bandwidths(hx[0], hy[0]);
+        kde.adapt_density(alpha);
+      }
+      std::vector<std::pair<double, double>> grid;
+      for (size_t k = 0; k < npairs; ++k) grid.emplace_back(hx[k], hy[k]);
+      const std::vector<double> l = likelihood_cross_validate_grid(kde, grid);
+      const std::vector<double> q = lsq_convolution_cross_validate_grid(kde, grid);
+      for (size_t k = 0; k < npairs; ++k) {
+        lcv_grid[k] = l[k];
+        lscv_grid[k] = q[k];
+        kde.kernel().set_bandwidths(hx[k], hy[k]);
+        lcv_loop[k] = kde.likelihood_cross_validate();
+        lscv_loop[k] = kde.lsq_convolution_cross_validate();
+      }
+      return 0;
+    };
+    if (kind == 3) return fn(EpanechnikovProductKernel2d<double>());
+    if (kind == 4) return fn(GaussianProductKernel2d<double>());
+    throw std::invalid_argument("api_cv_grid: kind must be 3 or 4");
+  });
+}
+
+// The reference's two bandwidth-grid layouts (grid_search/cv_utils_impl.h).
+int api_grid_layouts(const double *v1, size_t n1, const double *v2, size_t n2, double sx, double ex, int nx,
+                     double sy, double ey, int ny, double *crossed, double *generated) {
+  return guarded([&] {
+    const auto c = make_crossed_vector_pairs(std::vector<double>(v1, v1 + n1), std::vector<double>(v2, v2 + n2));
+    for (size_t k = 0; k < c.size(); ++k) {
+      crossed[2 * k] = c[k].first;
+      crossed[2 * k + 1] = c[k].second;
+    }
+    const auto g = generate_2d_bandwidth_grid(sx, ex, nx, sy, ey, ny);
+    for (size_t k = 0; k < g.size(); ++k) {
+      generated[2 * k] = g[k].first;
+      generated[2 * k + 1] = g[k].second;
+    }
+    return 0;
   });
 }
 

@@ -50,6 +50,8 @@ def api():
     lib.api_adapt.restype = ctypes.c_long
     lib.api_adapt.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, _D, _DP, _DP, _DP, _DP, _SZ, _DP, _DP]
     lib.api_hip_direct_range.argtypes = [_I, _I, _I, _D, _D, _DP, _SZ, _DP, _SZ, _SZ, _SZ, _SZ, _SZ, _I, _DP]
+    lib.api_cv_grid.argtypes = [_I, _DP, _DP, _SZ, _I, _DP, _DP, _SZ, _I, _D, _DP, _DP, _DP, _DP]
+    lib.api_grid_layouts.argtypes = [_DP, _SZ, _DP, _SZ, _D, _D, _I, _D, _D, _I, _DP, _DP]
     return lib
 
 
@@ -229,3 +231,41 @@ def test_hip_direct_kde_ranges_and_errors(api, oracle, kf):
                                     _p(out)) == -102  # std::out_of_range
     assert api.api_hip_direct_range(GAUSSPROD, 2, kf, 0.2, 0.3, _p(pts), 3000, _p(q), 1200, 5, 4, 0, 10, 0,
                                     _p(out)) == -101  # std::invalid_argument
+
+
+def test_bandwidth_grid_layouts(api):
+    """make_crossed_vector_pairs: v1 outer, v2 inner; generate_2dgrid: end
+    included, y outer (grid_search/cv_utils_impl.h:15-25, 43-59)."""
+    v1, v2 = np.array([0.1, 0.2]), np.array([0.3, 0.4, 0.5])
+    crossed = np.empty((6, 2))
+    gen = np.empty((6, 2))
+    assert api.api_grid_layouts(_p(v1), 2, _p(v2), 3, 0.0, 1.0, 2, 0.0, 1.0, 1, _p(crossed), _p(gen)) == 0
+    np.testing.assert_array_equal(crossed, [[0.1, 0.3], [0.1, 0.4], [0.1, 0.5], [0.2, 0.3], [0.2, 0.4], [0.2, 0.5]])
+    np.testing.assert_array_equal(gen, [[0.0, 0.0], [0.5, 0.0], [1.0, 0.0], [0.0, 1.0], [0.5, 1.0], [1.0, 1.0]])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("adapt", [0, 1])
+@pytest.mark.parametrize("kind", [EPANPROD, GAUSSPROD])
+def test_grid_search_matches_per_bandwidth_calls(api, kind, adapt):
+    """The batched grid equals the per-bandwidth KernelDensity cross-validations
+    grid_search makes (FP64 device sums) within the FP32 grid's bound, and the
+    reference's own scores at the fixture bandwidths."""
+    f = np.load(GOLDEN / "cv_ref.npz")
+    pts, w = f["pts"], f["w"]
+    case = [i for i, c in enumerate(f["cases"]) if int(c[0]) == kind][0]
+    hx0, hy0 = float(f["cases"][case][2]), float(f["cases"][case][3])
+    pairs = [(hx0, hy0), (0.2, 0.25), (0.2, 0.6), (0.5, 0.25), (hx0, 0.6)]
+    hx = np.array([a for a, _ in pairs])
+    hy = np.array([b for _, b in pairs])
+    out = [np.empty(len(pairs)) for _ in range(4)]
+    assert api.api_cv_grid(kind, _p(pts), _p(w), len(pts), 16, _p(hx), _p(hy), len(pairs), adapt, 0.5,
+                           *[_p(o) for o in out]) == 0, _err(api)
+    lcv_g, lscv_g, lcv_l, lscv_l = out
+    finite = np.isfinite(lcv_l)
+    np.testing.assert_array_equal(np.isfinite(lcv_g), finite)
+    np.testing.assert_allclose(lcv_g[finite], lcv_l[finite], rtol=0, atol=1e-5)
+    np.testing.assert_allclose(lscv_g, lscv_l, rtol=1e-5, atol=1e-7)
+    if not adapt:
+        assert lcv_g[0] == pytest.approx(float(f[f"lcv_{case}"]), rel=1e-5, abs=1e-6)
+        assert lscv_g[0] == pytest.approx(float(f[f"lscv_{case}"]), rel=1e-5, abs=1e-7)
