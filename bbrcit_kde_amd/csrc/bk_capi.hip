@@ -376,7 +376,7 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
       const size_t ntiles = (n + bk::kGaussTile - 1) / bk::kGaussTile;
       ctx->f0.reserve(n * sizeof(float4));
       ctx->f1.reserve(n * sizeof(float4));
-      ctx->tiles.reserve(ntiles * sizeof(float4));
+      ctx->tiles.reserve(ntiles * 5 * sizeof(float4));  // per tile: meta + 4 sub-tile centres
       hip_check(bk::launch_prep_gauss(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
                                       mass_scale(r), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
                                       ctx->tiles.as<float4>(), s),
@@ -425,13 +425,33 @@ void check_range(const bk_ctx *ctx, size_t r_i, size_t r_j, size_t q_i, size_t q
   if (r_i > r_j || q_i > q_j) fail(BK_ERR_INVALID_ARG, "must have r_i <= r_j and q_i <= q_j");
 }
 
-int choose_splits(long long nq, long long nr, int queries_per_block, int tile) {
+// Reference splits of a range evaluation: about kSplitTarget workgroups per
+// CU.  With `slots` resident workgroups known, the count is moved (within
+// +-25%) to the value whose last round of equal workgroups is fullest.
+int choose_splits(long long nq, long long nr, int queries_per_block, int tile, long long slots = 0) {
   const long long qblocks = (nq + queries_per_block - 1) / queries_per_block;
   const long long target = (long long)bk::kSplitTarget * bk::device_cu_count();
   long long splits = (target + qblocks - 1) / qblocks;
-  const long long max_splits = std::max(1LL, nr / (4LL * tile));
+  const long long max_splits = std::min(65535LL, std::max(1LL, nr / (4LL * tile)));
   splits = std::max(1LL, std::min(splits, max_splits));
-  splits = std::min(splits, 65535LL);
+  if (slots > 0 && splits > 4) {
+    long long best = splits;
+    double best_eff = 0.0;
+    for (long long s = splits - splits / 4; s <= std::min(max_splits, splits + splits / 4); ++s) {
+      // non-empty splits after tile alignment
+      long long len = (nr + s - 1) / s;
+      len = (len + tile - 1) / tile * tile;
+      const long long eff_s = (nr + len - 1) / len;
+      const double items = (double)qblocks * (double)eff_s;
+      const double rounds = std::ceil(items / (double)slots);
+      const double eff = items / (rounds * (double)slots);
+      if (eff > best_eff + 1e-9) {
+        best_eff = eff;
+        best = s;
+      }
+    }
+    splits = best;
+  }
   return (int)splits;
 }
 
@@ -486,7 +506,9 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
   const long long nr = (long long)(r_j - r_i + 1);
   const long long r_end = (long long)r_j + 1;
   const double scale = path_scale(ctx, path, p, normalize);
-  const int splits = choose_splits(nq, nr, queries_per_item(path), tile_of(path));
+  const long long slots =
+      path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
+  const int splits = choose_splits(nq, nr, queries_per_item(path), tile_of(path), slots);
   double *dst = out;
   if (splits > 1) {
     ctx->partial.reserve((size_t)splits * nq * sizeof(double));
@@ -589,7 +611,9 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
     ensure_features(ctx, desc, p);
     const Path path = path_of(ctx, p);
     const long long nr = (long long)ctx->refs.n;
-    const int splits = choose_splits((long long)nq, nr, queries_per_item(path), tile_of(path));
+    const long long slots =
+        path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
+    const int splits = choose_splits((long long)nq, nr, queries_per_item(path), tile_of(path), slots);
     double *dst = out;
     if (splits > 1) {
       ctx->partial.reserve((size_t)splits * nq * sizeof(double));

@@ -46,10 +46,10 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
 #define BK_QPT 4
 #endif
 #ifndef BK_GAUSS_QPT
-#define BK_GAUSS_QPT 2
+#define BK_GAUSS_QPT 1
 #endif
 #ifndef BK_GAUSS_UNROLL
-#define BK_GAUSS_UNROLL 4
+#define BK_GAUSS_UNROLL 2
 #endif
 constexpr int kQpt = BK_QPT;       // queries held in registers per thread (FP64 kernels)
 constexpr int kGaussQpt = BK_GAUSS_QPT;  // queries per thread of the FP32 Gaussian kernel
@@ -60,7 +60,7 @@ constexpr int kTile = 256;         // references per LDS stage (tiles are aligne
 #define BK_GAUSS_TILE 256
 #endif
 #ifndef BK_SPLIT_TARGET
-#define BK_SPLIT_TARGET 256
+#define BK_SPLIT_TARGET 512
 #endif
 constexpr int kGaussTile = BK_GAUSS_TILE;  // references per LDS stage of the FP32 Gaussian kernel
 constexpr int kSplitTarget = BK_SPLIT_TARGET;  // workgroups per CU aimed at by reference splitting
@@ -74,7 +74,8 @@ enum GaussMode : int {
 
 // FP32 Gaussian path, D <= 2 ---------------------------------------------
 // Per reference: fd = direct-form features, fg = centred-form features; per
-// aligned tile of kGaussTile references: tiles = (centre x, centre y, form, W).
+// aligned tile of kGaussTile references: tiles holds 5 float4 per tile,
+// (centre x, centre y, form, W) and the centres of its 4 sub-tiles.
 hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
                              const float *m, const float *abw, size_t n,
                              double m_scale, float4 *fd, float4 *fg,
@@ -172,5 +173,7 @@ hipError_t launch_adapt_lambda(const double *S, long long n, double norm,
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
+// Workgroups of the FP32 Gaussian range kernel resident per CU.
+int gauss_resident_blocks();
 
 }  // namespace bk

@@ -41,12 +41,15 @@ constexpr float kGramMaxRho2 = 1.0f;      // compactness bound of the Gram form
 #define BK_FACTORED_RHO2 9.0f
 #endif
 constexpr float kFactoredMaxRho2 = BK_FACTORED_RHO2;  // compactness bound of the factored form
-constexpr float kFarExp = 100.0f;         // W Q2 beyond which a tile's terms are < 2^-64
+constexpr float kFarExp = 100.0f;         // W Q2 beyond which a factored tile's terms are < 2^-49
 
 // Evaluation form of an aligned tile (tiles[t].z).
 constexpr float kFormDirect = 0.0f;     // differences q - r
 constexpr float kFormGram = 1.0f;       // tile-centred, per-reference W
 constexpr float kFormFactored = 2.0f;   // tile-centred, one W for the tile (tiles[t].w)
+constexpr float kFormFactored4 = 3.0f;  // factored about the centres of the tile's four sub-tiles
+constexpr int kTileMeta = 5;            // float4 per tile: (cx, cy, form, W), then 4 sub-tile centres
+constexpr int kSubTile = kGaussTile / 4;
 
 __device__ __forceinline__ float clamp01(float v) {
   return __builtin_fminf(__builtin_fmaxf(v, 0.0f), 1.0f);
@@ -88,7 +91,13 @@ __device__ __forceinline__ void split_bounds(long long r_i, long long r_end,
 //             is computed per (query, tile) in FP64 (exact differences, split
 //             exp2).  C + A.a = exponent + W Q2 <= W Q2, so the partial sums
 //             stay finite for W Q2 <= kFarExp; farther queries take factor 0
-//             (all their terms in the tile are below 2^-64 of the mass scale).
+//             (all their terms in the tile are below 2^-49 of the mass scale:
+//             the nearest reference is at least (10 - 3)^2 = 49 exponent units
+//             away); a wavefront whose queries all take factor 0 skips the tile;
+//   factored4: the same about the centres of the tile's four aligned
+//             sub-tiles of kSubTile references (one factor per query and
+//             sub-tile), for tiles too wide for one centre whose sub-tiles
+//             each satisfy rho^2 <= kFactoredMaxRho2.
 // The rounding error of a centred exponent grows like (rho + distance)^2 in
 // bandwidth units, which bounds rho for the centred forms.
 
@@ -196,7 +205,7 @@ __device__ __forceinline__ void gauss_item(
   const long long q_self0 = q_begin + t;
 
   long long tile = (r_begin / T) * T;
-  float4 meta = tiles[tile / T];
+  float4 meta = tiles[(tile / T) * kTileMeta];
   bool centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
   float4 nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
   if (t < T) {
@@ -213,7 +222,7 @@ __device__ __forceinline__ void gauss_item(
       const long long nt = tile + T;
       nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
       if (nt < r_end) {
-        meta = tiles[nt / T];
+        meta = tiles[(nt / T) * kTileMeta];
         centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
         const long long r = nt + t;
         if (t < T && r < r_end) nf = centred ? fg[r] : fd[r];
@@ -229,6 +238,36 @@ __device__ __forceinline__ void gauss_item(
       fac[k] = 1.0;
       ax[k] = ay[k] = aq[k] = 0.0f;
     }
+    if (form == 3) {
+      // factored about each sub-tile's centre: one FP64 factor per query and sub-tile
+      const float4 *subc = tiles + (tile / T) * kTileMeta + 1;
+      for (int sb = 0; sb * kSubTile < cnt; ++sb) {
+        const int c0 = sb * kSubTile;
+        const int cn = (cnt - c0) < kSubTile ? (cnt - c0) : kSubTile;
+        const float4 sc = subc[sb];
+        double sfac[kGaussQpt];
+        bool any_near = false;
+#pragma unroll
+        for (int k = 0; k < kGaussQpt; ++k) {
+          const double dx = (double)px[k] - (double)sc.x;
+          const double dy = (double)py[k] - (double)sc.y;
+          const double e = (double)cur.w * __builtin_fma(gy * dy, dy, gx * dx * dx);
+          const bool near = e <= (double)kFarExp;
+          any_near |= near;
+          sfac[k] = near ? exp2_split(-e) : 0.0;
+          ax[k] = near ? (float)dx : 0.0f;
+          ay[k] = near ? (float)dy : 0.0f;
+          s[k] = 0.0f;
+        }
+        // a factor of zero for every query of the wavefront: the sub-tile adds nothing
+        if (__any(any_near)) gauss_tile<2, ISO>(sf[buf] + c0, cn, tile + c0, diag, q_self0, px, py, ax, ay, aq, rho, s);
+#pragma unroll
+        for (int k = 0; k < kGaussQpt; ++k) acc[k] = __builtin_fma((double)s[k], sfac[k], acc[k]);
+      }
+      buf ^= 1;
+      continue;
+    }
+    bool any_near = false;
     if (form != 0) {
 #pragma unroll
       for (int k = 0; k < kGaussQpt; ++k) {
@@ -241,6 +280,7 @@ __device__ __forceinline__ void gauss_item(
         if (form == 2) {
           const double e = (double)cur.w * q2;
           const bool near = e <= (double)kFarExp;
+          any_near |= near;
           fac[k] = near ? exp2_split(-e) : 0.0;
           ax[k] = near ? ax[k] : 0.0f;
           ay[k] = near ? ay[k] : 0.0f;
@@ -248,7 +288,8 @@ __device__ __forceinline__ void gauss_item(
       }
     }
     if (form == 2) {
-      gauss_tile<2, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+      // a factor of zero for every query of the wavefront: the tile adds nothing
+      if (__any(any_near)) gauss_tile<2, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
     } else if (form == 1) {
       gauss_tile<1, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
     } else {
@@ -265,8 +306,15 @@ __device__ __forceinline__ void gauss_item(
   }
 }
 
+// Minimum waves per SIMD of the range kernel: caps its registers at 64 so
+// eight workgroups fit on a CU (measured best at the 1M x 1M workload, with
+// one query per thread and the reference loop unrolled twice).
+#ifndef BK_GAUSS_WAVES
+#define BK_GAUSS_WAVES 8
+#endif
+
 template <bool ISO>
-__global__ __launch_bounds__(kBlock) void gauss_range_kernel(
+__global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_i, long long r_end,
@@ -310,14 +358,25 @@ __device__ float block_reduce_f(float v, float *sh) {
   return r;
 }
 
-// One workgroup of kGaussTile threads per aligned tile.
+__device__ __forceinline__ float wave_reduce_f(float v, bool is_max) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const float u = __shfl_xor(v, o, 64);
+    v = is_max ? __builtin_fmaxf(v, u) : __builtin_fminf(v, u);
+  }
+  return v;
+}
+
+// One workgroup of kGaussTile threads per aligned tile; each wavefront is one
+// sub-tile of kSubTile references.
 __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
     EvalParams p, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ m, const float *__restrict__ abw, long long n,
     double inv_m_scale, float4 *__restrict__ fd, float4 *__restrict__ fg,
     float4 *__restrict__ tiles) {
   constexpr int T = kGaussTile;
+  static_assert(kSubTile == 64, "one wavefront per sub-tile");
   __shared__ float sh[T];
+  __shared__ float s_form;
   const long long i = (long long)blockIdx.x * T + threadIdx.x;
   const bool valid = i < n;
   const float xi = valid ? x[i] : 0.0f;
@@ -329,34 +388,56 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
   const float ymax = block_reduce_f<T, true>(valid ? yi : -big, sh);
   const float cx = 0.5f * (xmin + xmax);
   const float cy = 0.5f * (ymin + ymax);
-  float rho2 = 0.0f;
+  // sub-tile centre (a sub-tile without points gets the tile's centre)
+  const float sxmin = wave_reduce_f(valid ? xi : big, false), sxmax = wave_reduce_f(valid ? xi : -big, true);
+  const float symin = wave_reduce_f(valid ? yi : big, false), symax = wave_reduce_f(valid ? yi : -big, true);
+  const bool sub_empty = sxmin > sxmax;
+  const float scx = sub_empty ? cx : 0.5f * (sxmin + sxmax);
+  const float scy = sub_empty ? cy : 0.5f * (symin + symax);
+  float rho2 = 0.0f, rho2_sub = 0.0f;
   float w_lo = big, w_hi = -big;
+  double W = 1.0, lm = 0.0, mass = 0.0;
+  float Wf = 1.0f;
   if (valid) {
     const double a = (double)abw[i];
-    const float Wf = (float)(1.0 / (a * a));
-    const double W = (double)Wf;
-    const double mass = (double)m[i] * inv_m_scale;
-    const double lm = mass > 0.0 ? log2(mass) : (double)kMaskedLog;
+    Wf = (float)(1.0 / (a * a));
+    W = (double)Wf;
+    mass = (double)m[i] * inv_m_scale;
+    lm = mass > 0.0 ? log2(mass) : (double)kMaskedLog;
     fd[i] = make_float4(xi, yi, (float)(p.gx * W), (float)lm);
-    const double rx = (double)xi - (double)cx;
-    const double ry = (double)yi - (double)cy;
-    const double q = p.gx * rx * rx + p.gy * ry * ry;
-    fg[i] = make_float4((float)(2.0 * p.gx * W * rx), (float)(2.0 * p.gy * W * ry), Wf,
-                        mass > 0.0 ? (float)(lm - W * q) : kMaskedLog);
-    rho2 = (float)(W * q);
+    const double rx = (double)xi - (double)cx, ry = (double)yi - (double)cy;
+    rho2 = (float)(W * (p.gx * rx * rx + p.gy * ry * ry));
+    const double sx = (double)xi - (double)scx, sy = (double)yi - (double)scy;
+    rho2_sub = (float)(W * (p.gx * sx * sx + p.gy * sy * sy));
     w_lo = w_hi = Wf;
   }
   const float rmax = block_reduce_f<T, true>(rho2, sh);
+  const float rsub = block_reduce_f<T, true>(rho2_sub, sh);
   const float wmin = block_reduce_f<T, false>(w_lo, sh);
   const float wmax = block_reduce_f<T, true>(w_hi, sh);
   if (threadIdx.x == 0) {
     float form = kFormDirect;
     if (wmin == wmax && rmax <= kFactoredMaxRho2) {
       form = kFormFactored;
+    } else if (wmin == wmax && rsub <= kFactoredMaxRho2) {
+      form = kFormFactored4;
     } else if (rmax <= kGramMaxRho2) {
       form = kFormGram;
     }
-    tiles[blockIdx.x] = make_float4(cx, cy, form, wmax);
+    s_form = form;
+    tiles[blockIdx.x * kTileMeta] = make_float4(cx, cy, form, wmax);
+  }
+  if (threadIdx.x % kSubTile == 0)
+    tiles[blockIdx.x * kTileMeta + 1 + threadIdx.x / kSubTile] = make_float4(scx, scy, 0.0f, 0.0f);
+  __syncthreads();
+  if (valid) {
+    // centred features about the tile's or the sub-tile's centre
+    const bool sub = s_form == kFormFactored4;
+    const double rx = (double)xi - (double)(sub ? scx : cx);
+    const double ry = (double)yi - (double)(sub ? scy : cy);
+    const double q = p.gx * rx * rx + p.gy * ry * ry;
+    fg[i] = make_float4((float)(2.0 * p.gx * W * rx), (float)(2.0 * p.gy * W * ry), Wf,
+                        mass > 0.0 ? (float)(lm - W * q) : kMaskedLog);
   }
 }
 
@@ -1297,6 +1378,21 @@ hipError_t launch_adapt_lambda(const double *S, long long n, double norm,
   adapt_lambda_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(
       S, n, norm, log_g_dev, alpha, lambda);
   return hipGetLastError();
+}
+
+int gauss_resident_blocks() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1;
+  if (cached[dev] == 0) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(gauss_range_kernel<true>),
+                                                     kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
+    cached[dev] = per_cu;
+  }
+  return cached[dev];
 }
 
 int device_cu_count() {
