@@ -235,3 +235,26 @@ def test_gaussian_forms_agree_with_oracle(bk, oracle, kind, hx, hy):
             kde.set_option(bk.BK_OPT_GAUSS_FORM, 7)
     assert_parity(auto, want, mass.sum(), what="tile-centred form")
     assert_parity(direct, want, mass.sum(), what="difference form")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_reference_cpu_case_10k(bk, oracle, reference, precision):
+    """The reference's own CPU-runnable case: 10,000 bimodal references (means
+    (-1,-1), (1,1), sigma 0.5, unit weights), 10,000 uniform queries on
+    [-3,3]^2, GaussianKernel h = 0.1.  Against the reference's
+    KernelDensity<2,GaussianKernel<2,double>>::direct_eval (FP64) and the FP64
+    oracle on the same (FP32-rounded for the FP32 context) inputs."""
+    pts = ds.bimodal_gaussian(10_000, ds.SEED_BIMODAL_10K)
+    q = ds.uniform_box(10_000, -3.0, 3.0, ds.SEED_BIMODAL_10K + 1)
+    w = np.full(len(pts), 1.0 / len(pts))
+    prec = bk.BK_PREC_F64 if precision == "f64" else bk.BK_PREC_F32
+    with bk.DirectKde(pts, q, w, precision=prec) as kde:
+        got = kde.eval(bk.KernelDesc(GAUSS, 2, 0.1), 0, len(pts) - 1, 0, len(q) - 1)
+    want_ref = reference.direct_eval(GAUSS, 0, 0.1, 0.1, pts, np.ones(len(pts)), q, leaf_nmax=1024)
+    bound = oracle.normalization(GAUSS, 2, 0.1, 0.1)
+    if precision == "f64":
+        assert_parity(got, want_ref, bound, rtol=1e-12, what="FP64 vs reference")
+    else:
+        want = oracle.direct(GAUSS, 0.1, 0.1, f32(pts), f32(q), mass=f32(w), normalize=True)
+        assert_parity(got, want, bound, what="FP32 vs oracle")
+        assert_parity(got, want_ref, bound, rtol=2e-5, what="FP32 vs reference (FP64 inputs)")
