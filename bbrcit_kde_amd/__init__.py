@@ -45,6 +45,9 @@ BK_GAUSS_FORM_DIRECT = 1
 BK_OPT_HIGH_DIM_FORM = 2
 BK_HIGH_DIM_AUTO = 0
 BK_HIGH_DIM_DIRECT = 1
+BK_OPT_QUERY_ORDER = 3
+BK_QUERY_ORDER_GIVEN = 0
+BK_QUERY_ORDER_SPATIAL = 1
 
 BK_MLP_NUM_PARAMS = 33537
 
@@ -120,6 +123,14 @@ class ServeStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BoundStats(ctypes.Structure):
+    _fields_ = [("excluded", ctypes.c_ulonglong), ("approximated", ctypes.c_ulonglong),
+                ("evaluated", ctypes.c_ulonglong)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def library_path() -> Path:
     return LIB_PATH
 
@@ -150,6 +161,12 @@ def load_library() -> ctypes.CDLL:
                                                 ctypes.c_int]),
         "bk_eval_leaf_pairs": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.POINTER(RangePair),
                                               ctypes.c_size_t, _DBL_P, ctypes.c_int]),
+        "bk_eval_bounded": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t, ctypes.c_size_t,
+                                           ctypes.c_double, ctypes.c_double, _DBL_P, _DBL_P, ctypes.c_int,
+                                           ctypes.POINTER(BoundStats)]),
+        "bk_eval_bounded_device": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
+                                                  ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                                  ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(BoundStats)]),
         "bk_self_eval": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_int, _DBL_P, ctypes.c_int]),
         "bk_likelihood_cv": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), _DBL_P, _DBL_P]),
         "bk_lsq_conv_cv": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), _DBL_P, _DBL_P]),
@@ -326,6 +343,25 @@ class DirectKde:
             res.append(out[off:off + s])
             off += s
         return res
+
+    def eval_bounded(self, desc: KernelDesc, q_i: int, q_j: int, rel_err: float, abs_err: float,
+                     normalize: bool = True):
+        """Dual-tree bounds over every reference: (lower, upper, stats dict)."""
+        self._check_indices(q_i, q_j)
+        n = max(q_j - q_i + 1, 0)
+        lo = np.empty(max(n, 1))
+        up = np.empty(max(n, 1))
+        st = BoundStats()
+        _check(self._lib.bk_eval_bounded(self._ctx, ctypes.byref(desc), q_i, q_j, rel_err, abs_err, _dptr(lo),
+                                         _dptr(up), int(normalize), ctypes.byref(st)))
+        return lo[:n], up[:n], st.as_dict()
+
+    def eval_bounded_device(self, desc: KernelDesc, q_i: int, q_j: int, rel_err: float, abs_err: float,
+                            lower_dev_ptr: int, upper_dev_ptr: int, normalize: bool = True) -> None:
+        self._check_indices(q_i, q_j)
+        _check(self._lib.bk_eval_bounded_device(self._ctx, ctypes.byref(desc), q_i, q_j, rel_err, abs_err,
+                                                ctypes.c_void_p(lower_dev_ptr), ctypes.c_void_p(upper_dev_ptr),
+                                                int(normalize), None))
 
     def self_eval(self, desc: KernelDesc, exclude_self: bool = False, normalize: bool = True) -> np.ndarray:
         out = np.empty(self.reference_size(), dtype=np.float64)

@@ -33,6 +33,8 @@ struct PointSet {
   int dim = 0;
   bool uniform_abw = true;  // every abw == 1
   double m_max = 0.0;       // largest mass (as stored on the device)
+  double m_sum = 0.0;       // sum of the masses (as stored on the device)
+  float box_lo = 0.0f, box_hi = 1.0f;  // square holding every FP32 point (D <= 2), for spatial sorting
   // FP32 storage: x, y (D <= 2; y == 0 for D == 1) or row-major coords (D > 2)
   DevBuf x, y, coords, m, abw;
   // FP64 storage (D <= 2)
@@ -66,6 +68,12 @@ struct bk_ctx {
   int gauss_form = BK_GAUSS_FORM_AUTO;
   int high_dim_form = BK_HIGH_DIM_AUTO;
   int feat_path = -1;  // Path of the cached features
+  int query_order = BK_QUERY_ORDER_GIVEN;
+  DevBuf ord_list, ord_keys_a, ord_keys_b, ord_tmp, ord_x, ord_y, ord_out;  // spatial query order
+  // bounded evaluation: tile summaries of the reference set (per version),
+  // per-block thresholds, pruning counters, bounds planes
+  DevBuf tsum, bnd_thr, bnd_counts, bnd_gap, bnd_hi;
+  unsigned long long tsum_version = 0;
 
   // staging
   DevBuf partial, out, items, work_a, work_b, weights, scratch, scalar;
@@ -215,6 +223,7 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
   ps.n = n;
   ps.uniform_abw = true;
   ps.m_max = 0.0;
+  ps.m_sum = 0.0;
   ps.version = ++ctx->version_counter;
   if (n == 0) return;
   const hipStream_t s = ctx->stream;
@@ -229,6 +238,7 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
       if (!(m[i] >= 0.0f) || !std::isfinite(m[i]))
         fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
       ps.m_max = std::max(ps.m_max, (double)m[i]);
+      ps.m_sum += (double)m[i];
     }
     ps.m.reserve(n * sizeof(float));
     ps.abw.reserve(n * sizeof(float));
@@ -240,6 +250,13 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
         x[i] = (float)coords[i * dim];
         if (dim == 2) y[i] = (float)coords[i * dim + 1];
       }
+      float lo = x[0], hi = x[0];
+      for (size_t i = 0; i < n; ++i) {
+        lo = std::min(lo, std::min(x[i], y[i]));
+        hi = std::max(hi, std::max(x[i], y[i]));
+      }
+      ps.box_lo = std::isfinite(lo) ? lo : -1.0f;
+      ps.box_hi = std::isfinite(hi) && hi > ps.box_lo ? hi : ps.box_lo + 1.0f;
       ps.x.reserve(n * sizeof(float));
       ps.y.reserve(n * sizeof(float));
       hip_check(hipMemcpyAsync(ps.x.ptr, x.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload x");
@@ -265,6 +282,7 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
       if (!(m[i] >= 0.0) || !std::isfinite(m[i]))
         fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
       ps.m_max = std::max(ps.m_max, m[i]);
+      ps.m_sum += m[i];
     }
     for (DevBuf *b : {&ps.xd, &ps.yd, &ps.md, &ps.abwd}) b->reserve(n * sizeof(double));
     hip_check(hipMemcpyAsync(ps.xd.ptr, x.data(), n * 8, hipMemcpyHostToDevice, s), "upload x");
@@ -286,11 +304,13 @@ void update_refs(bk_ctx *ctx, const double *mass, const double *abw) {
     std::vector<float> buf(n);
     if (mass) {
       ps.m_max = 0.0;
+      ps.m_sum = 0.0;
       for (size_t i = 0; i < n; ++i) {
         buf[i] = (float)mass[i];
         if (!(buf[i] >= 0.0f) || !std::isfinite(buf[i]))
           fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
         ps.m_max = std::max(ps.m_max, (double)buf[i]);
+        ps.m_sum += (double)buf[i];
       }
       hip_check(hipMemcpyAsync(ps.m.ptr, buf.data(), n * 4, hipMemcpyHostToDevice, s), "update masses");
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -308,10 +328,12 @@ void update_refs(bk_ctx *ctx, const double *mass, const double *abw) {
   } else {
     if (mass) {
       ps.m_max = 0.0;
+      ps.m_sum = 0.0;
       for (size_t i = 0; i < n; ++i) {
         if (!(mass[i] >= 0.0) || !std::isfinite(mass[i]))
           fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
         ps.m_max = std::max(ps.m_max, mass[i]);
+        ps.m_sum += mass[i];
       }
       hip_check(hipMemcpyAsync(ps.md.ptr, mass, n * 8, hipMemcpyHostToDevice, s), "update masses");
     }
@@ -376,20 +398,23 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
       const size_t ntiles = (n + bk::kGaussTile - 1) / bk::kGaussTile;
       ctx->f0.reserve(n * sizeof(float4));
       ctx->f1.reserve(n * sizeof(float4));
-      ctx->tiles.reserve(ntiles * 5 * sizeof(float4));  // per tile: meta + 4 sub-tile centres
+      ctx->tiles.reserve(ntiles * bk::kGaussTileMeta * sizeof(float4));
       hip_check(bk::launch_prep_gauss(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
                                       mass_scale(r), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
                                       ctx->tiles.as<float4>(), s),
                 "prep_gauss launch");
       break;
     }
-    case Path::EPAN64:
+    case Path::EPAN64: {
+      const size_t ntiles = (n + bk::kTile - 1) / bk::kTile;
       ctx->f0.reserve(n * sizeof(double4));
       ctx->f1.reserve(n * sizeof(double4));
+      ctx->tiles.reserve(ntiles * bk::kEpanTileMeta * sizeof(double4));
       hip_check(bk::launch_prep_epan(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
-                                     ctx->f0.as<double4>(), ctx->f1.as<double4>(), s),
+                                     ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(), s),
                 "prep_epan launch");
       break;
+    }
     case Path::GENERAL:
       ctx->f0.reserve(n * sizeof(double2));
       hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, ctx->f0.as<double2>(), s),
@@ -483,6 +508,7 @@ hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkIt
                               ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s);
     case Path::EPAN64:
       return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<double4>(), ctx->f1.as<double4>(),
+                             ctx->tiles.as<double4>(),
                              items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
     case Path::GRAM:
       return bk::launch_gram(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double4>(), items, n_items,
@@ -509,6 +535,43 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
   const long long slots =
       path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
   const int splits = choose_splits(nq, nr, queries_per_item(path), tile_of(path), slots);
+  const bool spatial = ctx->query_order == BK_QUERY_ORDER_SPATIAL && !self_excl &&
+                       (path == Path::GAUSS32 || path == Path::EPAN64) && nq > queries_per_item(path) &&
+                       nq < (1LL << 32);
+  if (spatial) {
+    // evaluate the queries in Z-curve order, then scatter back: each workgroup
+    // holds nearby queries and skips the reference tiles out of their reach
+    const hipStream_t s = ctx->stream;
+    const PointSet &qs = query_set(ctx);
+    const float *qx = qs.x.as<float>() + q_i, *qy = qs.y.as<float>() + q_i;
+    ctx->ord_list.reserve((size_t)nq * sizeof(long long));
+    ctx->ord_x.reserve((size_t)nq * sizeof(float));
+    ctx->ord_y.reserve((size_t)nq * sizeof(float));
+    ctx->ord_out.reserve((size_t)nq * sizeof(double));
+    long long *list = ctx->ord_list.as<long long>();
+    bk::spatial_sort(qx, qy, nullptr, nq, nq, qs.box_lo, qs.box_hi, list, ctx->ord_keys_a, ctx->ord_keys_b,
+                     ctx->ord_tmp, s);
+    float *ox = ctx->ord_x.as<float>(), *oy = ctx->ord_y.as<float>();
+    hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
+    double *res = ctx->ord_out.as<double>();
+    double *dst = res;
+    if (splits > 1) {
+      ctx->partial.reserve((size_t)splits * nq * sizeof(double));
+      dst = ctx->partial.as<double>();
+    }
+    hipError_t e;
+    if (path == Path::GAUSS32)
+      e = bk::launch_gauss(p, ox, oy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
+                           0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0,
+                           ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s);
+    else
+      e = bk::launch_epan(p, ox, oy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
+                          nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s);
+    hip_check(e, "direct kernel launch");
+    if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, res, s), "reduce launch");
+    hip_check(bk::launch_scatter_f64(list, nq, res, out, s), "scatter launch");
+    return;
+  }
   double *dst = out;
   if (splits > 1) {
     ctx->partial.reserve((size_t)splits * nq * sizeof(double));
@@ -518,6 +581,153 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                         self_excl),
             "direct kernel launch");
   if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, ctx->stream), "reduce launch");
+}
+
+// Tile summaries (box, mass, abw range per aligned tile of kTile
+// references) of the current reference set.
+void ensure_tile_summary(bk_ctx *ctx) {
+  PointSet &r = ctx->refs;
+  if (ctx->tsum_version == r.version) return;
+  const size_t ntiles = (r.n + bk::kTile - 1) / bk::kTile;
+  ctx->tsum.reserve(std::max<size_t>(ntiles, 1) * bk::kTileSummary * sizeof(double4));
+  if (ctx->precision == BK_PREC_F64)
+    hip_check(bk::launch_tile_summary_f64(r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
+                                          r.abwd.as<double>(), r.n, ctx->tsum.as<double4>(), ctx->stream),
+              "tile summary launch");
+  else
+    hip_check(bk::launch_tile_summary_f32(r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), r.n,
+                                          ctx->tsum.as<double4>(), ctx->stream),
+              "tile summary launch");
+  ctx->tsum_version = r.version;
+}
+
+// Bounded (dual-tree) evaluation of queries q_i..q_j against every
+// reference into device arrays lower / upper.
+void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_t q_j, double rel_err,
+                      double abs_err, double *lower, double *upper, int normalize, bk_bound_stats *stats) {
+  const EvalParams p = params_of(desc);
+  check_compatible(ctx, p);
+  if (ctx->refs.n == 0) fail(BK_ERR_STATE, "no reference points uploaded");
+  check_range(ctx, 0, ctx->refs.n - 1, q_i, q_j);
+  if (!(rel_err >= 0.0) || !(abs_err >= 0.0) || !std::isfinite(rel_err) || !std::isfinite(abs_err))
+    fail(BK_ERR_INVALID_ARG, "rel_err and abs_err must be finite and non-negative");
+  ensure_features(ctx, desc, p);
+  const Path path = path_of(ctx, p);
+  const hipStream_t s = ctx->stream;
+  const long long nq = (long long)(q_j - q_i + 1);
+  const long long nr = (long long)ctx->refs.n;
+  bk_bound_stats st{};
+  if (path == Path::GENERAL || path == Path::GRAM) {
+    // D > 2: exact sums, the bounds coincide
+    eval_range_dev(ctx, desc, 0, ctx->refs.n - 1, q_i, q_j, lower, normalize, 0);
+    hip_check(hipMemcpyAsync(upper, lower, (size_t)nq * sizeof(double), hipMemcpyDeviceToDevice, s), "copy bounds");
+    const long long tile = tile_of(path);
+    st.evaluated = (unsigned long long)(((nq + 255) / 256) * ((nr + tile - 1) / tile));
+    if (stats) *stats = st;
+    return;
+  }
+  ensure_tile_summary(ctx);
+  const long long ntiles = (nr + bk::kTile - 1) / bk::kTile;
+  const double scale = path_scale(ctx, path, p, normalize);
+  const double abs_u = normalize ? abs_err / p.norm : abs_err;
+  const double m_total = ctx->refs.m_sum > 0.0 ? ctx->refs.m_sum : 1.0;
+  const int qpb = path == Path::F64 ? bk::kBlock : queries_per_item(path);
+  const long long nblocks = (nq + qpb - 1) / qpb;
+  ctx->bnd_thr.reserve((size_t)nblocks * sizeof(double));
+  ctx->bnd_counts.reserve(4 * sizeof(unsigned long long));
+  hip_check(hipMemsetAsync(ctx->bnd_counts.ptr, 0, 4 * sizeof(unsigned long long), s), "clear counters");
+  const PointSet &qs = query_set(ctx);
+  // queries: as stored, or gathered in Z-curve order (FP32 contexts)
+  const bool spatial =
+      ctx->query_order == BK_QUERY_ORDER_SPATIAL && path != Path::F64 && nq > qpb && nq < (1LL << 32);
+  const float *qx32 = nullptr, *qy32 = nullptr;
+  const double *qx64 = nullptr, *qy64 = nullptr;
+  long long *list = nullptr;
+  long long q_off = (long long)q_i;
+  if (path == Path::F64) {
+    qx64 = qs.xd.as<double>();
+    qy64 = qs.yd.as<double>();
+  } else if (spatial) {
+    const float *bx = qs.x.as<float>() + q_i, *by = qs.y.as<float>() + q_i;
+    ctx->ord_list.reserve((size_t)nq * sizeof(long long));
+    ctx->ord_x.reserve((size_t)nq * sizeof(float));
+    ctx->ord_y.reserve((size_t)nq * sizeof(float));
+    list = ctx->ord_list.as<long long>();
+    bk::spatial_sort(bx, by, nullptr, nq, nq, qs.box_lo, qs.box_hi, list, ctx->ord_keys_a, ctx->ord_keys_b,
+                     ctx->ord_tmp, s);
+    hip_check(bk::launch_gather_xy(bx, by, list, nq, ctx->ord_x.as<float>(), ctx->ord_y.as<float>(), s),
+              "gather launch");
+    qx32 = ctx->ord_x.as<float>();
+    qy32 = ctx->ord_y.as<float>();
+    q_off = 0;
+  } else {
+    qx32 = qs.x.as<float>();
+    qy32 = qs.y.as<float>();
+  }
+  double *thr = ctx->bnd_thr.as<double>();
+  unsigned long long *counts = ctx->bnd_counts.as<unsigned long long>();
+  hip_check(bk::launch_bound_pass(p, qx32 ? qx32 + q_off : nullptr, qy32 ? qy32 + q_off : nullptr,
+                                  qx64 ? qx64 + q_off : nullptr, qy64 ? qy64 + q_off : nullptr, nq, qpb,
+                                  ctx->tsum.as<double4>(), ntiles, rel_err, abs_u, m_total, thr, counts, s),
+            "bound pass launch");
+  // output planes: lower and gap (in Z-curve order when sorted)
+  double *lo_out = lower;
+  if (spatial) {
+    ctx->ord_out.reserve((size_t)nq * sizeof(double));
+    lo_out = ctx->ord_out.as<double>();
+  }
+  ctx->bnd_gap.reserve((size_t)nq * sizeof(double));
+  double *gap_out = ctx->bnd_gap.as<double>();
+  const long long slots =
+      path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
+  const int splits = choose_splits(nq, nr, qpb, tile_of(path), slots);
+  double *dst = lo_out, *gdst = gap_out;
+  if (splits > 1) {
+    ctx->partial.reserve((size_t)2 * splits * nq * sizeof(double));
+    dst = ctx->partial.as<double>();
+    gdst = dst + (size_t)splits * nq;
+  }
+  bk::BoundArgs ba{};
+  ba.tsum = ctx->tsum.as<double4>();
+  ba.thr = thr;
+  ba.gap = gdst;
+  ba.ms = path == Path::GAUSS32 ? 1.0 / mass_scale(ctx->refs) : 1.0;
+  ba.kx = p.family == bk::FAM_EPANCONV ? p.ix : p.gx;
+  ba.ky = p.family == bk::FAM_EPANCONV ? p.iy : p.gy;
+  hipError_t e;
+  if (path == Path::GAUSS32)
+    e = bk::launch_gauss(p, qx32, qy32, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
+                         0, 0, nr, q_off, nq, splits, dst, scale, 0,
+                         ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, &ba);
+  else if (path == Path::EPAN64)
+    e = bk::launch_epan(p, qx32, qy32, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
+                        nullptr, 0, 0, nr, q_off, nq, splits, dst, scale, 0, s, &ba);
+  else
+    e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr, q_off,
+                              nq, splits, dst, scale, 0, s, &ba);
+  hip_check(e, "bounded kernel launch");
+  if (splits > 1) {
+    hip_check(bk::launch_reduce_splits(dst, splits, nq, lo_out, s), "reduce launch");
+    hip_check(bk::launch_reduce_splits(gdst, splits, nq, gap_out, s), "reduce launch");
+  }
+  if (spatial) {
+    ctx->bnd_hi.reserve((size_t)nq * sizeof(double));
+    double *hi_ord = ctx->bnd_hi.as<double>();
+    hip_check(bk::launch_add_planes(lo_out, gap_out, nq, hi_ord, s), "bounds launch");
+    hip_check(bk::launch_scatter_f64(list, nq, lo_out, lower, s), "scatter launch");
+    hip_check(bk::launch_scatter_f64(list, nq, hi_ord, upper, s), "scatter launch");
+  } else {
+    hip_check(bk::launch_add_planes(lo_out, gap_out, nq, upper, s), "bounds launch");
+  }
+  if (stats) {
+    unsigned long long c[4] = {0, 0, 0, 0};
+    hip_check(hipMemcpyAsync(c, counts, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "read counts");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    st.excluded = c[0];
+    st.approximated = c[1];
+    st.evaluated = c[2];
+    *stats = st;
+  }
 }
 
 void copy_to_host(bk_ctx *ctx, double *host, const double *dev, size_t n) {
@@ -626,7 +836,8 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
                        nr, 0, (long long)nq, splits, dst, scale, 0,
                        ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? GAUSS_DIRECT : GAUSS_AUTO, s);
     else
-      e = launch_epan(p, qx, qy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr, 0, (long long)nq,
+      e = launch_epan(p, qx, qy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(), nullptr, 0,
+                      0, nr, 0, (long long)nq,
                       splits, dst, scale, 0, s);
     hip_check(e, "direct kernel launch");
     if (splits > 1) hip_check(launch_reduce_splits(dst, splits, (long long)nq, out, s), "reduce launch");
@@ -677,7 +888,9 @@ int bk_destroy(bk_ctx *ctx) {
       ctx->refs.release();
       ctx->queries.release();
       for (DevBuf *b : {&ctx->f0, &ctx->f1, &ctx->tiles, &ctx->partial, &ctx->out, &ctx->items, &ctx->work_a, &ctx->work_b,
-                        &ctx->weights, &ctx->scratch, &ctx->scalar})
+                        &ctx->weights, &ctx->scratch, &ctx->scalar, &ctx->ord_list, &ctx->ord_keys_a,
+                        &ctx->ord_keys_b, &ctx->ord_tmp, &ctx->ord_x, &ctx->ord_y, &ctx->ord_out, &ctx->tsum,
+                        &ctx->bnd_thr, &ctx->bnd_counts, &ctx->bnd_gap, &ctx->bnd_hi})
         b->release();
       if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }
@@ -705,6 +918,11 @@ int bk_set_option(bk_ctx *ctx, int option, int value) {
         if (value != BK_HIGH_DIM_AUTO && value != BK_HIGH_DIM_DIRECT)
           fail(BK_ERR_INVALID_ARG, "BK_OPT_HIGH_DIM_FORM takes BK_HIGH_DIM_AUTO or BK_HIGH_DIM_DIRECT");
         ctx->high_dim_form = value;
+        break;
+      case BK_OPT_QUERY_ORDER:
+        if (value != BK_QUERY_ORDER_GIVEN && value != BK_QUERY_ORDER_SPATIAL)
+          fail(BK_ERR_INVALID_ARG, "BK_OPT_QUERY_ORDER takes BK_QUERY_ORDER_GIVEN or BK_QUERY_ORDER_SPATIAL");
+        ctx->query_order = value;
         break;
       default:
         fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));
@@ -774,6 +992,32 @@ int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, si
     if (!out_dev) fail(BK_ERR_INVALID_ARG, "null output array");
     DeviceGuard g(ctx->device);
     eval_range_dev(ctx, desc, r_i, r_j, q_i, q_j, out_dev, normalize, 0);
+  });
+}
+
+int bk_eval_bounded_device(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_t q_j, double rel_err,
+                           double abs_err, double *lower_dev, double *upper_dev, int normalize,
+                           bk_bound_stats *stats) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!lower_dev || !upper_dev) fail(BK_ERR_INVALID_ARG, "null output array");
+    DeviceGuard g(ctx->device);
+    eval_bounded_dev(ctx, desc, q_i, q_j, rel_err, abs_err, lower_dev, upper_dev, normalize, stats);
+  });
+}
+
+int bk_eval_bounded(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_t q_j, double rel_err,
+                    double abs_err, double *lower_host, double *upper_host, int normalize, bk_bound_stats *stats) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!lower_host || !upper_host) fail(BK_ERR_INVALID_ARG, "null output array");
+    DeviceGuard g(ctx->device);
+    if (ctx->refs.n > 0) check_range(ctx, 0, ctx->refs.n - 1, q_i, q_j);
+    const size_t nq = q_j >= q_i ? q_j - q_i + 1 : 0;
+    double *lo = staging_out(ctx, 2 * std::max<size_t>(nq, 1));
+    eval_bounded_dev(ctx, desc, q_i, q_j, rel_err, abs_err, lo, lo + nq, normalize, stats);
+    copy_to_host(ctx, lower_host, lo, nq);
+    copy_to_host(ctx, upper_host, lo + nq, nq);
   });
 }
 

@@ -72,10 +72,44 @@ enum GaussMode : int {
   GAUSS_DIRECT = 1,  // direct form everywhere
 };
 
+// Bounded (dual-tree) evaluation, D <= 2 -------------------------------------
+// Per aligned tile of kTile references, kTileSummary double4: the box
+// (x0, x1, y0, y1) and (mass sum, smallest abw, largest abw, 0).  For a block
+// of queries and a tile, du / dl bound the kernel U between any query of the
+// block and any reference of the tile (U at the boxes' nearest per-axis
+// distances with the largest abw, at the farthest with the smallest).  The
+// block's tiles with du - dl <= thr are not evaluated: they add mass * dl to
+// the lower bound and mass * (du - dl) to the gap between the bounds.  With
+// thr = max(rel * L, abs) / (total mass), L = sum over tiles of mass * dl (a
+// lower bound of every query's sum), the gap stays <= max(rel * lower, abs).
+constexpr int kTileSummary = 2;
+struct BoundArgs {
+  const double4 *tsum;  // tile summaries
+  const double *thr;    // per query block
+  double *gap;          // second output plane, laid out like dst
+  double ms;            // accumulator units per unit of mass
+  double kx, ky;        // the family's per-axis constants (gx, gy or ix, iy)
+};
+hipError_t launch_tile_summary_f32(const float *x, const float *y, const float *m, const float *abw, size_t n,
+                                   double4 *tsum, hipStream_t s);
+hipError_t launch_tile_summary_f64(const double *x, const double *y, const double *m, const double *abw, size_t n,
+                                   double4 *tsum, hipStream_t s);
+// One workgroup per block of qpb queries: thr[b] as above, and counts[0..2]
+// += tiles passed over with du == 0, approximated with du > 0, evaluated.
+// queries are FP32 (qx32, qy32) or FP64 (qx64, qy64; the other pair null).
+hipError_t launch_bound_pass(const EvalParams &p, const float *qx32, const float *qy32, const double *qx64,
+                             const double *qy64, long long nq, int qpb, const double4 *tsum, long long ntiles,
+                             double rel, double abs_u, double m_total, double *thr, unsigned long long *counts,
+                             hipStream_t s);
+// hi[k] = lo[k] + gap[k] for k < n.
+hipError_t launch_add_planes(const double *lo, const double *gap, long long n, double *hi, hipStream_t s);
+
 // FP32 Gaussian path, D <= 2 ---------------------------------------------
 // Per reference: fd = direct-form features, fg = centred-form features; per
-// aligned tile of kGaussTile references: tiles holds 5 float4 per tile,
-// (centre x, centre y, form, W) and the centres of its 4 sub-tiles.
+// aligned tile of kGaussTile references: tiles holds kGaussTileMeta float4
+// per tile: (centre x, centre y, form, W), the centres of its 4 sub-tiles,
+// and (half width x, half width y, smallest W, 0) of its bounding box.
+constexpr int kGaussTileMeta = 6;
 hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
                              const float *m, const float *abw, size_t n,
                              double m_scale, float4 *fd, float4 *fg,
@@ -84,23 +118,27 @@ hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y
 // items == nullptr: range mode, grid (query blocks) x (reference splits),
 // dst holds `splits` consecutive arrays of nq partial sums.
 // items != nullptr: one workgroup per work item.
+// ba != nullptr (range mode only, r_i aligned to kTile): bounded evaluation.
 hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
                         const float4 *fd, const float4 *fg, const float4 *tiles,
                         const WorkItem *items, int n_items, long long r_i,
                         long long r_end, long long q_i, long long nq,
                         int splits, double *dst, double out_scale,
-                        int self_excl, int mode, hipStream_t s);
+                        int self_excl, int mode, hipStream_t s, const BoundArgs *ba = nullptr);
 
 // FP64 Epanechnikov-family path, D <= 2, on FP32 point data ----------------
+// tb: 2 double4 per aligned tile of kTile references (box and smallest
+// scales, for workgroup-level culling).
+constexpr int kEpanTileMeta = 2;
 hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
                             const float *m, const float *abw, size_t n,
-                            double4 *f0, double4 *f1, hipStream_t s);
+                            double4 *f0, double4 *f1, double4 *tb, hipStream_t s);
 hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
-                       const double4 *f0, const double4 *f1,
+                       const double4 *f0, const double4 *f1, const double4 *tb,
                        const WorkItem *items, int n_items, long long r_i,
                        long long r_end, long long q_i, long long nq, int splits,
                        double *dst, double out_scale, int self_excl,
-                       hipStream_t s);
+                       hipStream_t s, const BoundArgs *ba = nullptr);
 
 // FP64 context path, D <= 2 ------------------------------------------------
 hipError_t launch_prep_f64(const EvalParams &p, const double *x,
@@ -113,7 +151,7 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              int n_items, long long r_i, long long r_end,
                              long long q_i, long long nq, int splits,
                              double *dst, double out_scale, int self_excl,
-                             hipStream_t s);
+                             hipStream_t s, const BoundArgs *ba = nullptr);
 
 // General dimension, 3 <= D <= 16 (FAM_GAUSS and FAM_EPAN) ---------------
 // FP32 point data, FP64 distances.  wm = (w = g / abw^2, mass) per point.
@@ -170,6 +208,23 @@ hipError_t launch_weighted_reduce(int mode, const double *w, const double *u,
 hipError_t launch_adapt_lambda(const double *S, long long n, double norm,
                                const double *log_g_dev, double alpha,
                                double *lambda, hipStream_t s);
+
+// Spatial order ------------------------------------------------------------
+struct DevBuf;  // bk_common.h
+// Sorts list[0, n) (indices into qx, qy, each below index_limit <= 2^32;
+// list == nullptr means the identity 0..n-1, written into list_out) by the
+// Morton code of the point on a 1024 x 1024 grid over [lo, hi]^2 (points
+// outside are clamped onto the border), ties broken by the index, so the
+// order is deterministic.  The result goes to list_out (may equal list).
+// keys and tmp are workspaces the function grows.
+void spatial_sort(const float *qx, const float *qy, const long long *list, long long n, long long index_limit,
+                  float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
+                  hipStream_t s);
+// gx[k] = qx[list[k]], gy[k] = qy[list[k]] for k < n.
+hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
+                            float *gy, hipStream_t s);
+// out[list[k]] = vals[k] for k < n.
+hipError_t launch_scatter_f64(const long long *list, long long n, const double *vals, double *out, hipStream_t s);
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();

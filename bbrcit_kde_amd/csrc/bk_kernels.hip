@@ -48,7 +48,7 @@ constexpr float kFormDirect = 0.0f;     // differences q - r
 constexpr float kFormGram = 1.0f;       // tile-centred, per-reference W
 constexpr float kFormFactored = 2.0f;   // tile-centred, one W for the tile (tiles[t].w)
 constexpr float kFormFactored4 = 3.0f;  // factored about the centres of the tile's four sub-tiles
-constexpr int kTileMeta = 5;            // float4 per tile: (cx, cy, form, W), then 4 sub-tile centres
+constexpr int kTileMeta = kGaussTileMeta;  // float4 per tile: (cx, cy, form, W), 4 sub-tile centres, box
 constexpr int kSubTile = kGaussTile / 4;
 
 __device__ __forceinline__ float clamp01(float v) {
@@ -69,6 +69,91 @@ __device__ __forceinline__ void split_bounds(long long r_i, long long r_end,
   re = rb + len;
   if (rb < r_i) rb = r_i;
   if (re > r_end) re = r_end;
+}
+
+__device__ __forceinline__ float wave_reduce_f(float v, bool is_max) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const float u = __shfl_xor(v, o, 64);
+    v = is_max ? __builtin_fmaxf(v, u) : __builtin_fminf(v, u);
+  }
+  return v;
+}
+
+// Bounding box (x0, x1, y0, y1) of the queries held by a workgroup of kBlock
+// threads, NQ per thread (padding lanes repeat a real query).  Every thread
+// gets the same four values.
+template <int NQ, typename T>
+__device__ __forceinline__ void block_query_box(const T *px, const T *py, float (&sbox)[4][kBlock / 64], float &bx0,
+                                                float &bx1, float &by0, float &by1) {
+  const int t = threadIdx.x;
+  float x0 = (float)px[0], x1 = x0, y0 = (float)py[0], y1 = y0;
+#pragma unroll
+  for (int k = 1; k < NQ; ++k) {
+    x0 = __builtin_fminf(x0, (float)px[k]);
+    x1 = __builtin_fmaxf(x1, (float)px[k]);
+    y0 = __builtin_fminf(y0, (float)py[k]);
+    y1 = __builtin_fmaxf(y1, (float)py[k]);
+  }
+  x0 = wave_reduce_f(x0, false);
+  x1 = wave_reduce_f(x1, true);
+  y0 = wave_reduce_f(y0, false);
+  y1 = wave_reduce_f(y1, true);
+  if ((t & 63) == 0) {
+    sbox[0][t >> 6] = x0;
+    sbox[1][t >> 6] = x1;
+    sbox[2][t >> 6] = y0;
+    sbox[3][t >> 6] = y1;
+  }
+  __syncthreads();
+  bx0 = sbox[0][0];
+  bx1 = sbox[1][0];
+  by0 = sbox[2][0];
+  by1 = sbox[3][0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) {
+    bx0 = __builtin_fminf(bx0, sbox[0][w]);
+    bx1 = __builtin_fmaxf(bx1, sbox[1][w]);
+    by0 = __builtin_fminf(by0, sbox[2][w]);
+    by1 = __builtin_fmaxf(by1, sbox[3][w]);
+  }
+}
+
+// ---- bounds of a tile's contribution (bounded evaluation) ---------------------
+// U at per-axis distances dx, dy >= 0 with bandwidth factor a, in the form the
+// evaluation kernels use (kx, ky: gx, gy of FAM_GAUSS / FAM_EPAN /
+// FAM_EPANPROD, ix, iy of FAM_EPANCONV).  U(0) = 1; U decreases with each
+// distance and increases with a.
+template <int FAM>
+__device__ __forceinline__ double kernel_u_bound(double dx, double dy, double a, double kx, double ky) {
+  const double ia2 = 1.0 / (a * a);
+  if constexpr (FAM == FAM_GAUSS) {
+    return exp2(-(kx * dx * dx + ky * dy * dy) * ia2);
+  } else if constexpr (FAM == FAM_EPAN) {
+    return fmax(1.0 - kx * (dx * dx + dy * dy) * ia2, 0.0);
+  } else if constexpr (FAM == FAM_EPANPROD) {
+    return fmax(1.0 - kx * dx * dx * ia2, 0.0) * fmax(1.0 - ky * dy * dy * ia2, 0.0);
+  } else {
+    const double ux = dx * kx / a, uy = dy * ky / a;
+    const double cx = fmax(2.0 - ux, 0.0), cy = fmax(2.0 - uy, 0.0);
+    return (cx * cx * cx * (ux * ux + 6.0 * ux + 4.0) / 32.0) * (cy * cy * cy * (uy * uy + 6.0 * uy + 4.0) / 32.0);
+  }
+}
+
+// Mass of tile ti and the bounds du >= U >= dl over every (query in the box,
+// reference in the tile) pair.
+template <int FAM>
+__device__ __forceinline__ void tile_bounds(const double4 *__restrict__ tsum, long long ti, double bx0, double bx1,
+                                            double by0, double by1, double kx, double ky, double &mt, double &du,
+                                            double &dl) {
+  const double4 b = tsum[kTileSummary * ti];
+  const double4 w = tsum[kTileSummary * ti + 1];
+  const double nx = fmax(0.0, fmax(b.x - bx1, bx0 - b.y));
+  const double ny = fmax(0.0, fmax(b.z - by1, by0 - b.w));
+  const double fx = fmax(b.y - bx0, bx1 - b.x);
+  const double fy = fmax(b.w - by0, by1 - b.z);
+  mt = w.x;
+  du = kernel_u_bound<FAM>(nx, ny, w.z, kx, ky);
+  dl = kernel_u_bound<FAM>(fx, fy, w.y, kx, ky);
 }
 
 // ---- FP32 Gaussian ----------------------------------------------------------
@@ -100,6 +185,14 @@ __device__ __forceinline__ void split_bounds(long long r_i, long long r_end,
 //             each satisfy rho^2 <= kFactoredMaxRho2.
 // The rounding error of a centred exponent grows like (rho + distance)^2 in
 // bandwidth units, which bounds rho for the centred forms.
+// Workgroup-level culling: a tile whose bounding box lies farther than
+// W_min Q2 = kFarExp from the bounding box of the workgroup's queries is
+// skipped whole (not even staged in LDS).  For the factored forms this only
+// skips tiles whose factors are all zero anyway (the box distance is at most
+// the centre distance), so no result changes; for the direct and Gram forms
+// it drops terms below 2^-100 of the largest mass.  Queries in random order
+// give workgroup boxes that span the data and cull nothing; spatially sorted
+// queries (the surrogate's sampled check and fallback) cull most tiles.
 
 __device__ __forceinline__ float gauss_direct_term(float qx, float qy, float4 f, float rho) {
   const float dx = qx - f.x;
@@ -177,20 +270,26 @@ __device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cn
     gauss_tile_loop<FORM, ISO, false>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
 }
 
-template <bool ISO>
+// BOUNDED: tiles with du - dl <= thr_b are passed over and enter the bounds
+// (lower in dst, gap in gap_dst) through their mass; the others are summed.
+template <bool ISO, bool BOUNDED>
 __device__ __forceinline__ void gauss_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_begin, long long r_end,
     long long q_begin, int q_count, double *__restrict__ dst, double out_scale,
-    int self_excl, float rho, double gx, double gy, int mode) {
+    int self_excl, float rho, double gx, double gy, int mode, const BoundArgs &ba, double thr_b,
+    double *__restrict__ gap_dst) {
   constexpr int T = kGaussTile;
+  static_assert(!BOUNDED || kGaussTile == kTile, "tile summaries are per kTile references");
   __shared__ float4 sf[2][T];
   const int t = threadIdx.x;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
+    if constexpr (BOUNDED) write_zeros(gap_dst, q_count);
     return;
   }
+  __shared__ float sbox[4][kBlock / 64];
   float px[kGaussQpt], py[kGaussQpt];
   double acc[kGaussQpt];
 #pragma unroll
@@ -204,22 +303,51 @@ __device__ __forceinline__ void gauss_item(
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
   const long long q_self0 = q_begin + t;
 
-  long long tile = (r_begin / T) * T;
-  float4 meta = tiles[(tile / T) * kTileMeta];
-  bool centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
+  float bx0, bx1, by0, by1;
+  block_query_box<kGaussQpt>(px, py, sbox, bx0, bx1, by0, by1);
+  double lo_add = 0.0, gap_add = 0.0;  // bounded: passed-over tiles
+  // first tile at or after `tl` that is not culled (or passed over), or r_end
+  auto live_from = [&](long long tl) {
+    for (; tl < r_end; tl += T) {
+      if constexpr (BOUNDED) {
+        double mt, du, dl;
+        tile_bounds<FAM_GAUSS>(ba.tsum, tl / T, bx0, bx1, by0, by1, gx, gy, mt, du, dl);
+        if (!(du - dl <= thr_b)) break;
+        lo_add += ba.ms * mt * dl;
+        gap_add += ba.ms * mt * (du - dl);
+      } else {
+        const float4 *m = tiles + (tl / T) * kTileMeta;
+        const float4 c = m[0], b = m[kTileMeta - 1];
+        const double dx =
+            fmax(0.0, fmax((double)c.x - (double)b.x - (double)bx1, (double)bx0 - (double)c.x - (double)b.x));
+        const double dy =
+            fmax(0.0, fmax((double)c.y - (double)b.y - (double)by1, (double)by0 - (double)c.y - (double)b.y));
+        if ((double)b.z * __builtin_fma(gy * dy, dy, gx * dx * dx) <= (double)kFarExp) break;
+      }
+    }
+    return tl;
+  };
+
+  long long tile = live_from((r_begin / T) * T);
+  float4 meta = make_float4(0.f, 0.f, kFormDirect, 1.f);
+  bool centred = false;
   float4 nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
-  if (t < T) {
+  if (tile < r_end) {
+    meta = tiles[(tile / T) * kTileMeta];
+    centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
     const long long r = tile + t;
-    if (r >= r_begin && r < r_end) nf = centred ? fg[r] : fd[r];
+    if (t < T && r >= r_begin && r < r_end) nf = centred ? fg[r] : fd[r];
   }
   int buf = 0;
-  for (; tile < r_end; tile += T) {
+  long long next_tile = r_end;
+  for (; tile < r_end; tile = next_tile) {
     if (t < T) sf[buf][t] = nf;
     __syncthreads();
     const float4 cur = meta;
     const int form = centred ? (int)cur.z : 0;
     {
-      const long long nt = tile + T;
+      next_tile = live_from(tile + T);
+      const long long nt = next_tile;
       nf = make_float4(0.f, 0.f, 0.f, kMaskedLog);
       if (nt < r_end) {
         meta = tiles[(nt / T) * kTileMeta];
@@ -302,7 +430,14 @@ __device__ __forceinline__ void gauss_item(
 #pragma unroll
   for (int k = 0; k < kGaussQpt; ++k) {
     const int qi = t + k * kBlock;
-    if (qi < q_count) dst[qi] = acc[k] * out_scale;
+    if (qi < q_count) {
+      if constexpr (BOUNDED) {
+        dst[qi] = (acc[k] + lo_add) * out_scale;
+        gap_dst[qi] = gap_add * out_scale;
+      } else {
+        dst[qi] = acc[k] * out_scale;
+      }
+    }
   }
 }
 
@@ -313,20 +448,21 @@ __device__ __forceinline__ void gauss_item(
 #define BK_GAUSS_WAVES 8
 #endif
 
-template <bool ISO>
+template <bool ISO, bool BOUNDED>
 __global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_i, long long r_end,
     long long split_len, long long q_i, long long nq, double *__restrict__ dst,
-    double out_scale, int self_excl, float rho, double gx, double gy, int mode) {
+    double out_scale, int self_excl, float rho, double gx, double gy, int mode, BoundArgs ba) {
   const long long qb = (long long)blockIdx.x * kGaussQueriesPerBlock;
   long long rb, re;
   split_bounds<kGaussTile>(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
   const long long left = nq - qb;
   const int q_count = (int)(left < kGaussQueriesPerBlock ? left : kGaussQueriesPerBlock);
-  gauss_item<ISO>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count,
-                  dst + (long long)blockIdx.y * nq + qb, out_scale, self_excl, rho, gx, gy, mode);
+  const long long off = (long long)blockIdx.y * nq + qb;
+  gauss_item<ISO, BOUNDED>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
+                           gx, gy, mode, ba, BOUNDED ? ba.thr[blockIdx.x] : 0.0, BOUNDED ? ba.gap + off : nullptr);
 }
 
 template <bool ISO>
@@ -337,8 +473,8 @@ __global__ __launch_bounds__(kBlock) void gauss_list_kernel(
     double *__restrict__ dst, double out_scale, int self_excl, float rho,
     double gx, double gy, int mode) {
   const WorkItem it = items[blockIdx.x];
-  gauss_item<ISO>(qx, qy, fd, fg, tiles, it.r_begin, it.r_end, it.q_begin, it.q_count,
-                  dst + it.out_index, out_scale, self_excl, rho, gx, gy, mode);
+  gauss_item<ISO, false>(qx, qy, fd, fg, tiles, it.r_begin, it.r_end, it.q_begin, it.q_count,
+                         dst + it.out_index, out_scale, self_excl, rho, gx, gy, mode, BoundArgs{}, 0.0, nullptr);
 }
 
 // Block-wide min / max over N threads (one value per thread).
@@ -356,14 +492,6 @@ __device__ float block_reduce_f(float v, float *sh) {
   const float r = sh[0];
   __syncthreads();
   return r;
-}
-
-__device__ __forceinline__ float wave_reduce_f(float v, bool is_max) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const float u = __shfl_xor(v, o, 64);
-    v = is_max ? __builtin_fmaxf(v, u) : __builtin_fminf(v, u);
-  }
-  return v;
 }
 
 // One workgroup of kGaussTile threads per aligned tile; each wavefront is one
@@ -429,6 +557,17 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
   }
   if (threadIdx.x % kSubTile == 0)
     tiles[blockIdx.x * kTileMeta + 1 + threadIdx.x / kSubTile] = make_float4(scx, scy, 0.0f, 0.0f);
+  if (threadIdx.x == 0) {
+    // half widths about (cx, cy), rounded up so the box holds every point
+    auto half = [](float lo, float hi, float c) {
+      const double hw = fmax((double)hi - (double)c, (double)c - (double)lo);
+      float f = (float)hw;
+      if ((double)f < hw) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f >= 0)
+      return f;
+    };
+    tiles[blockIdx.x * kTileMeta + kTileMeta - 1] =
+        make_float4(half(xmin, xmax, cx), half(ymin, ymax, cy), wmin, 0.0f);
+  }
   __syncthreads();
   if (valid) {
     // centred features about the tile's or the sub-tile's centre
@@ -473,18 +612,45 @@ __device__ __forceinline__ double epan_term(double qx, double qy, double4 a, dou
   }
 }
 
+// Per aligned tile of kTile references, two double4: the bounding box
+// (x0, x1, y0, y1) and the smallest per-reference scales (min f0.z, min
+// f0.w) of the tile (EPANPROD and EPANCONV use both, EPAN the first).
+// A tile is culled for a workgroup when every term is exactly zero for the
+// whole box of its queries.  The test evaluates the term's own support
+// condition at the per-axis gaps between the boxes with the tile's smallest
+// scales; every FP64 operation on the way is monotone, so a culled tile's
+// terms are zero as computed, and culling changes no result.
 template <int FAM>
+__device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, long long tile_idx, float bx0, float bx1,
+                                              float by0, float by1) {
+  const double4 b = tb[2 * tile_idx];
+  const double4 w = tb[2 * tile_idx + 1];
+  const double gx = fmax(0.0, fmax(b.x - (double)bx1, (double)bx0 - b.y));
+  const double gy = fmax(0.0, fmax(b.z - (double)by1, (double)by0 - b.w));
+  if constexpr (FAM == FAM_EPAN) {
+    return __builtin_fma(-w.x, __builtin_fma(gy, gy, gx * gx), 1.0) <= 0.0;
+  } else if constexpr (FAM == FAM_EPANPROD) {
+    return __builtin_fma(-(w.x * gx), gx, 1.0) <= 0.0 || __builtin_fma(-(w.y * gy), gy, 1.0) <= 0.0;
+  } else {
+    return gx * w.x >= 2.0 || gy * w.y >= 2.0;
+  }
+}
+
+template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void epan_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
-    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_begin, long long r_end, long long q_begin, int q_count,
-    double *__restrict__ dst, double out_scale, int self_excl) {
+    double *__restrict__ dst, double out_scale, int self_excl, const BoundArgs &ba, double thr_b,
+    double *__restrict__ gap_dst) {
   constexpr int kNf = EpanNf<FAM>::kNf;
   __shared__ double4 s0[2][kTile];
   __shared__ double4 s1[2][kNf == 2 ? kTile : 1];
+  __shared__ float sbox[4][kBlock / 64];
   const int t = threadIdx.x;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
+    if constexpr (BOUNDED) write_zeros(gap_dst, q_count);
     return;
   }
   double px[kQpt], py[kQpt], acc[kQpt];
@@ -496,11 +662,28 @@ __device__ __forceinline__ void epan_item(
     py[k] = (double)qy[q];
     acc[k] = 0.0;
   }
+  float bx0, bx1, by0, by1;
+  block_query_box<kQpt>(px, py, sbox, bx0, bx1, by0, by1);
+  double lo_add = 0.0, gap_add = 0.0;  // bounded: passed-over tiles
+  auto live_from = [&](long long tl) {
+    for (; tl < r_end; tl += kTile) {
+      if constexpr (BOUNDED) {
+        double mt, du, dl;
+        tile_bounds<FAM>(ba.tsum, tl / kTile, bx0, bx1, by0, by1, ba.kx, ba.ky, mt, du, dl);
+        if (!(du - dl <= thr_b)) break;
+        lo_add += ba.ms * mt * dl;
+        gap_add += ba.ms * mt * (du - dl);
+      } else {
+        if (!epan_tile_far<FAM>(tb, tl / kTile, bx0, bx1, by0, by1)) break;
+      }
+    }
+    return tl;
+  };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
   const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
-  long long tile = (r_begin / kTile) * kTile;
+  long long tile = live_from((r_begin / kTile) * kTile);
   double4 n0 = zero, n1 = zero;
-  {
+  if (tile < r_end) {
     const long long r = tile + t;
     if (r >= r_begin && r < r_end) {
       n0 = f0[r];
@@ -508,12 +691,14 @@ __device__ __forceinline__ void epan_item(
     }
   }
   int buf = 0;
-  for (; tile < r_end; tile += kTile) {
+  long long next_tile = r_end;
+  for (; tile < r_end; tile = next_tile) {
     s0[buf][t] = n0;
     if constexpr (kNf == 2) s1[buf][t] = n1;
     __syncthreads();
     {
-      const long long r = tile + kTile + t;
+      next_tile = live_from(tile + kTile);
+      const long long r = next_tile + t;
       n0 = zero;
       n1 = zero;
       if (r < r_end) {
@@ -548,34 +733,42 @@ __device__ __forceinline__ void epan_item(
 #pragma unroll
   for (int k = 0; k < kQpt; ++k) {
     const int qi = t + k * kBlock;
-    if (qi < q_count) dst[qi] = acc[k] * out_scale;
+    if (qi < q_count) {
+      if constexpr (BOUNDED) {
+        dst[qi] = (acc[k] + lo_add) * out_scale;
+        gap_dst[qi] = gap_add * out_scale;
+      } else {
+        dst[qi] = acc[k] * out_scale;
+      }
+    }
   }
 }
 
-template <int FAM>
+template <int FAM, bool BOUNDED>
 __global__ __launch_bounds__(kBlock) void epan_range_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
-    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_i, long long r_end, long long split_len, long long q_i,
-    long long nq, double *__restrict__ dst, double out_scale, int self_excl) {
+    long long nq, double *__restrict__ dst, double out_scale, int self_excl, BoundArgs ba) {
   const long long qb = (long long)blockIdx.x * kQueriesPerBlock;
   long long rb, re;
   split_bounds<kTile>(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
   const long long left = nq - qb;
   const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
-  epan_item<FAM>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + (long long)blockIdx.y * nq + qb,
-                 out_scale, self_excl);
+  const long long off = (long long)blockIdx.y * nq + qb;
+  epan_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, ba,
+                          BOUNDED ? ba.thr[blockIdx.x] : 0.0, BOUNDED ? ba.gap + off : nullptr);
 }
 
 template <int FAM>
 __global__ __launch_bounds__(kBlock) void epan_list_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
-    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     const WorkItem *__restrict__ items, double *__restrict__ dst,
     double out_scale, int self_excl) {
   const WorkItem it = items[blockIdx.x];
-  epan_item<FAM>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
-                 out_scale, self_excl);
+  epan_item<FAM, false>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
+                        out_scale, self_excl, BoundArgs{}, 0.0, nullptr);
 }
 
 __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
@@ -605,6 +798,34 @@ __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
   }
 }
 
+// One workgroup of kTile threads per aligned tile: the tile's box and
+// smallest scales (see epan_tile_far).
+__global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const double4 *__restrict__ f0, long long n,
+                                                                 double4 *__restrict__ tb) {
+  __shared__ float sh[kTile];
+  const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
+  const bool valid = i < n;
+  const double4 a = valid ? f0[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const float big = 3.0e38f;
+  // coordinates are FP32 values, so the FP32 reductions are exact
+  const float x0 = block_reduce_f<kTile, false>(valid ? (float)a.x : big, sh);
+  const float x1 = block_reduce_f<kTile, true>(valid ? (float)a.x : -big, sh);
+  const float y0 = block_reduce_f<kTile, false>(valid ? (float)a.y : big, sh);
+  const float y1 = block_reduce_f<kTile, true>(valid ? (float)a.y : -big, sh);
+  // smallest scales, rounded down to FP32 so the test never overstates them
+  auto down = [](double v) {
+    float f = (float)v;
+    if ((double)f > v) f = __uint_as_float(__float_as_uint(f) - 1u);  // v > 0
+    return f;
+  };
+  const float wz = block_reduce_f<kTile, false>(valid ? down(a.z) : big, sh);
+  const float ww = block_reduce_f<kTile, false>(valid ? down(a.w) : big, sh);
+  if (threadIdx.x == 0) {
+    tb[2 * blockIdx.x] = make_double4(x0, x1, y0, y1);
+    tb[2 * blockIdx.x + 1] = make_double4(wz, ww, 0.0, 0.0);
+  }
+}
+
 // ---- FP64 context path ---------------------------------------------------
 constexpr int kBlock64 = 256;
 constexpr int kTile64 = 256;
@@ -623,13 +844,38 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   }
 }
 
-template <int FAM>
+// Bounding box of the workgroup's FP64 queries (one per thread).
+__device__ __forceinline__ void block_query_box_d(double px, double py, double (&sbox)[4][kBlock / 64], double &bx0,
+                                                  double &bx1, double &by0, double &by1) {
+  const int t = threadIdx.x;
+  double v[4] = {px, -px, py, -py};  // min of each (max as min of the negation)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    for (int o = 32; o > 0; o >>= 1) v[i] = fmin(v[i], __shfl_xor(v[i], o, 64));
+    if ((t & 63) == 0) sbox[i][t >> 6] = v[i];
+  }
+  __syncthreads();
+  double r[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r[i] = sbox[i][0];
+    for (int w = 1; w < kBlock / 64; ++w) r[i] = fmin(r[i], sbox[i][w]);
+  }
+  bx0 = r[0];
+  bx1 = -r[1];
+  by0 = r[2];
+  by1 = -r[3];
+}
+
+template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void direct_f64_item(
     const double *__restrict__ qx, const double *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1,
     long long r_begin, long long r_end, long long q_begin, int q_count,
-    double *__restrict__ dst, double out_scale, int self_excl, double rho) {
+    double *__restrict__ dst, double out_scale, int self_excl, double rho, const BoundArgs &ba, double thr_b,
+    double *__restrict__ gap_dst) {
   constexpr int kNf = (FAM == FAM_EPANPROD || FAM == FAM_EPANCONV) ? 2 : 1;
+  static_assert(kBlock64 == kBlock && kTile64 == kTile, "one query per thread, tile summaries per kTile");
   __shared__ double4 s0[kTile64];
   __shared__ double4 s1[kNf == 2 ? kTile64 : 1];
   const int t = threadIdx.x;
@@ -637,7 +883,23 @@ __device__ __forceinline__ void direct_f64_item(
   const long long q = q_begin + (valid ? t : 0);
   const double px = qx[q], py = qy[q];
   double sum = 0.0;
-  for (long long tile = r_begin; tile < r_end; tile += kTile64) {
+  double lo_add = 0.0, gap_add = 0.0;
+  __shared__ double sbox[4][kBlock / 64];
+  double bx0 = 0.0, bx1 = 0.0, by0 = 0.0, by1 = 0.0;
+  if constexpr (BOUNDED) block_query_box_d(px, py, sbox, bx0, bx1, by0, by1);
+  auto live_from = [&](long long tl) {
+    if constexpr (BOUNDED) {
+      for (; tl < r_end; tl += kTile64) {
+        double mt, du, dl;
+        tile_bounds<FAM>(ba.tsum, tl / kTile64, bx0, bx1, by0, by1, ba.kx, ba.ky, mt, du, dl);
+        if (!(du - dl <= thr_b)) break;
+        lo_add += ba.ms * mt * dl;
+        gap_add += ba.ms * mt * (du - dl);
+      }
+    }
+    return tl;
+  };
+  for (long long tile = live_from(r_begin); tile < r_end; tile = live_from(tile + kTile64)) {
     const long long r = tile + t;
     if (r < r_end) {
       s0[t] = f0[r];
@@ -653,25 +915,32 @@ __device__ __forceinline__ void direct_f64_item(
     }
     __syncthreads();
   }
-  if (valid) dst[t] = sum * out_scale;
+  if (valid) {
+    if constexpr (BOUNDED) {
+      dst[t] = (sum + lo_add) * out_scale;
+      gap_dst[t] = gap_add * out_scale;
+    } else {
+      dst[t] = sum * out_scale;
+    }
+  }
 }
 
-template <int FAM>
+template <int FAM, bool BOUNDED>
 __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
     const double *__restrict__ qx, const double *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1,
     long long r_i, long long r_end, long long split_len, long long q_i,
     long long nq, double *__restrict__ dst, double out_scale, int self_excl,
-    double rho) {
+    double rho, BoundArgs ba) {
   const long long qb = (long long)blockIdx.x * kBlock64;
   const long long rb = r_i + (long long)blockIdx.y * split_len;
   long long re = rb + split_len;
   if (re > r_end) re = r_end;
   const long long left = nq - qb;
   const int q_count = (int)(left < kBlock64 ? left : kBlock64);
-  direct_f64_item<FAM>(qx, qy, f0, f1, rb, re, q_i + qb, q_count,
-                       dst + (long long)blockIdx.y * nq + qb, out_scale,
-                       self_excl, rho);
+  const long long off = (long long)blockIdx.y * nq + qb;
+  direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
+                                ba, BOUNDED ? ba.thr[blockIdx.x] : 0.0, BOUNDED ? ba.gap + off : nullptr);
 }
 
 template <int FAM>
@@ -681,9 +950,8 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_list_kernel(
     const WorkItem *__restrict__ items, double *__restrict__ dst,
     double out_scale, int self_excl, double rho) {
   const WorkItem it = items[blockIdx.x];
-  direct_f64_item<FAM>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin,
-                       it.q_count, dst + it.out_index, out_scale, self_excl,
-                       rho);
+  direct_f64_item<FAM, false>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
+                              out_scale, self_excl, rho, BoundArgs{}, 0.0, nullptr);
 }
 
 __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
@@ -1072,9 +1340,153 @@ long long split_length(long long r_i, long long r_end, int splits, int tile) {
   return len;
 }
 
+// ---- bounded evaluation: tile summaries, first pass, bounds planes ----------------
+// Block-wide (kTile threads) FP64 reduction: min, or sum.
+template <bool SUM>
+__device__ double block_reduce_d(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kTile / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = SUM ? sh[threadIdx.x] + sh[threadIdx.x + o]
+                                                    : fmin(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTile) void tile_summary_kernel(const T *__restrict__ x, const T *__restrict__ y,
+                                                             const T *__restrict__ m, const T *__restrict__ abw,
+                                                             long long n, double4 *__restrict__ tsum) {
+  __shared__ double sh[kTile];
+  const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
+  const bool valid = i < n;
+  const double inf = __builtin_inf();
+  const double xi = valid ? (double)x[i] : 0.0, yi = valid ? (double)y[i] : 0.0;
+  const double ai = valid ? (double)abw[i] : 0.0;
+  const double x0 = block_reduce_d<false>(valid ? xi : inf, sh);
+  const double x1 = -block_reduce_d<false>(valid ? -xi : inf, sh);
+  const double y0 = block_reduce_d<false>(valid ? yi : inf, sh);
+  const double y1 = -block_reduce_d<false>(valid ? -yi : inf, sh);
+  const double a0 = block_reduce_d<false>(valid ? ai : inf, sh);
+  const double a1 = -block_reduce_d<false>(valid ? -ai : inf, sh);
+  const double mt = block_reduce_d<true>(valid ? (double)m[i] : 0.0, sh);
+  if (threadIdx.x == 0) {
+    tsum[kTileSummary * blockIdx.x] = make_double4(x0, x1, y0, y1);
+    tsum[kTileSummary * blockIdx.x + 1] = make_double4(mt, a0, a1, 0.0);
+  }
+}
+
+// One workgroup of kTile threads per block of qpb queries.
+template <int FAM, typename T>
+__global__ __launch_bounds__(kTile) void bound_pass_kernel(const T *__restrict__ qx, const T *__restrict__ qy,
+                                                           long long nq, int qpb, const double4 *__restrict__ tsum,
+                                                           long long ntiles, double kx, double ky, double rel,
+                                                           double abs_u, double m_total, double *__restrict__ thr,
+                                                           unsigned long long *__restrict__ counts) {
+  __shared__ double sh[kTile];
+  const long long q0 = (long long)blockIdx.x * qpb;
+  const long long q1 = (q0 + qpb < nq) ? q0 + qpb : nq;
+  const double inf = __builtin_inf();
+  double x0 = inf, nx1 = inf, y0 = inf, ny1 = inf;
+  for (long long q = q0 + threadIdx.x; q < q1; q += kTile) {
+    const double xq = (double)qx[q], yq = (double)qy[q];
+    x0 = fmin(x0, xq);
+    nx1 = fmin(nx1, -xq);
+    y0 = fmin(y0, yq);
+    ny1 = fmin(ny1, -yq);
+  }
+  const double bx0 = block_reduce_d<false>(x0, sh), bx1 = -block_reduce_d<false>(nx1, sh);
+  const double by0 = block_reduce_d<false>(y0, sh), by1 = -block_reduce_d<false>(ny1, sh);
+  double L = 0.0;
+  for (long long ti = threadIdx.x; ti < ntiles; ti += kTile) {
+    double mt, du, dl;
+    tile_bounds<FAM>(tsum, ti, bx0, bx1, by0, by1, kx, ky, mt, du, dl);
+    L += mt * dl;
+  }
+  L = block_reduce_d<true>(L, sh);
+  const double th = fmax(rel * L, abs_u) / m_total;
+  double c_ex = 0.0, c_ap = 0.0, c_ev = 0.0;
+  for (long long ti = threadIdx.x; ti < ntiles; ti += kTile) {
+    double mt, du, dl;
+    tile_bounds<FAM>(tsum, ti, bx0, bx1, by0, by1, kx, ky, mt, du, dl);
+    if (du - dl <= th) {
+      if (du == 0.0)
+        c_ex += 1.0;
+      else
+        c_ap += 1.0;
+    } else {
+      c_ev += 1.0;
+    }
+  }
+  c_ex = block_reduce_d<true>(c_ex, sh);
+  c_ap = block_reduce_d<true>(c_ap, sh);
+  c_ev = block_reduce_d<true>(c_ev, sh);
+  if (threadIdx.x == 0) {
+    thr[blockIdx.x] = th;
+    atomicAdd(counts + 0, (unsigned long long)c_ex);
+    atomicAdd(counts + 1, (unsigned long long)c_ap);
+    atomicAdd(counts + 2, (unsigned long long)c_ev);
+  }
+}
+
+__global__ void add_planes_kernel(const double *__restrict__ lo, const double *__restrict__ gap, long long n,
+                                  double *__restrict__ hi) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) hi[k] = lo[k] + gap[k];
+}
+
 }  // namespace
 
 // ---- launchers -----------------------------------------------------------------
+
+hipError_t launch_tile_summary_f32(const float *x, const float *y, const float *m, const float *abw, size_t n,
+                                   double4 *tsum, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  tile_summary_kernel<float><<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(x, y, m, abw, (long long)n, tsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_summary_f64(const double *x, const double *y, const double *m, const double *abw, size_t n,
+                                   double4 *tsum, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  tile_summary_kernel<double><<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(x, y, m, abw, (long long)n,
+                                                                                     tsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_bound_pass(const EvalParams &p, const float *qx32, const float *qy32, const double *qx64,
+                             const double *qy64, long long nq, int qpb, const double4 *tsum, long long ntiles,
+                             double rel, double abs_u, double m_total, double *thr, unsigned long long *counts,
+                             hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((nq + qpb - 1) / qpb);
+  const bool conv = p.family == FAM_EPANCONV;
+  const double kx = conv ? p.ix : p.gx, ky = conv ? p.iy : p.gy;
+#define BK_BP(F)                                                                                                  \
+  if (qx64)                                                                                                       \
+    bound_pass_kernel<F, double><<<grid, kTile, 0, s>>>(qx64, qy64, nq, qpb, tsum, ntiles, kx, ky, rel, abs_u,    \
+                                                        m_total, thr, counts);                                    \
+  else                                                                                                            \
+    bound_pass_kernel<F, float><<<grid, kTile, 0, s>>>(qx32, qy32, nq, qpb, tsum, ntiles, kx, ky, rel, abs_u,     \
+                                                       m_total, thr, counts)
+  switch (p.family) {
+    case FAM_GAUSS: BK_BP(FAM_GAUSS); break;
+    case FAM_EPAN: BK_BP(FAM_EPAN); break;
+    case FAM_EPANPROD: BK_BP(FAM_EPANPROD); break;
+    default: BK_BP(FAM_EPANCONV); break;
+  }
+#undef BK_BP
+  return hipGetLastError();
+}
+
+hipError_t launch_add_planes(const double *lo, const double *gap, long long n, double *hi, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  add_planes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(lo, gap, n, hi);
+  return hipGetLastError();
+}
 
 hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
                              const float *m, const float *abw, size_t n,
@@ -1091,7 +1503,7 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
                         const WorkItem *items, int n_items, long long r_i,
                         long long r_end, long long q_i, long long nq,
                         int splits, double *dst, double out_scale,
-                        int self_excl, int mode, hipStream_t s) {
+                        int self_excl, int mode, hipStream_t s, const BoundArgs *ba) {
   const float rho = (float)(p.gy / p.gx);
   const double gx = p.gx, gy = p.gy;
   const bool iso = p.iso != 0;
@@ -1107,34 +1519,47 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
     if (nq <= 0) return hipSuccess;
     const long long len = split_length(r_i, r_end, splits, kGaussTile);
     const dim3 grid((unsigned)((nq + kGaussQueriesPerBlock - 1) / kGaussQueriesPerBlock), (unsigned)splits);
-    if (iso)
-      gauss_range_kernel<true><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst,
-                                                       out_scale, self_excl, rho, gx, gy, mode);
-    else
-      gauss_range_kernel<false><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst,
-                                                        out_scale, self_excl, rho, gx, gy, mode);
+    const BoundArgs b = ba ? *ba : BoundArgs{};
+#define BK_GR(I, B)                                                                                              \
+  gauss_range_kernel<I, B><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst, out_scale, \
+                                                   self_excl, rho, gx, gy, mode, b)
+    if (ba) {
+      if (iso)
+        BK_GR(true, true);
+      else
+        BK_GR(false, true);
+    } else {
+      if (iso)
+        BK_GR(true, false);
+      else
+        BK_GR(false, false);
+    }
+#undef BK_GR
   }
   return hipGetLastError();
 }
 
 hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
                             const float *m, const float *abw, size_t n,
-                            double4 *f0, double4 *f1, hipStream_t s) {
+                            double4 *f0, double4 *f1, double4 *tb, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int th = 256;
   prep_epan_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, x, y, m, abw, (long long)n, f0, f1);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  prep_epan_tiles_kernel<<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(f0, (long long)n, tb);
   return hipGetLastError();
 }
 
 hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
-                       const double4 *f0, const double4 *f1,
+                       const double4 *f0, const double4 *f1, const double4 *tb,
                        const WorkItem *items, int n_items, long long r_i,
                        long long r_end, long long q_i, long long nq, int splits,
                        double *dst, double out_scale, int self_excl,
-                       hipStream_t s) {
+                       hipStream_t s, const BoundArgs *ba) {
   if (items) {
     if (n_items <= 0) return hipSuccess;
-#define BK_EL(F) epan_list_kernel<F><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, items, dst, out_scale, self_excl)
+#define BK_EL(F) epan_list_kernel<F><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale, self_excl)
     switch (p.family) {
       case FAM_EPAN: BK_EL(FAM_EPAN); break;
       case FAM_EPANPROD: BK_EL(FAM_EPANPROD); break;
@@ -1145,13 +1570,22 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
     if (nq <= 0) return hipSuccess;
     const long long len = split_length(r_i, r_end, splits, kTile);
     const dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
-#define BK_ER(F)                                                                                 \
-  epan_range_kernel<F><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, r_i, r_end, len, q_i, nq, dst, out_scale, \
-                                               self_excl)
-    switch (p.family) {
-      case FAM_EPAN: BK_ER(FAM_EPAN); break;
-      case FAM_EPANPROD: BK_ER(FAM_EPANPROD); break;
-      default: BK_ER(FAM_EPANCONV); break;
+    const BoundArgs b = ba ? *ba : BoundArgs{};
+#define BK_ER(F, B)                                                                                 \
+  epan_range_kernel<F, B><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, \
+                                                  self_excl, b)
+    if (ba) {
+      switch (p.family) {
+        case FAM_EPAN: BK_ER(FAM_EPAN, true); break;
+        case FAM_EPANPROD: BK_ER(FAM_EPANPROD, true); break;
+        default: BK_ER(FAM_EPANCONV, true); break;
+      }
+    } else {
+      switch (p.family) {
+        case FAM_EPAN: BK_ER(FAM_EPAN, false); break;
+        case FAM_EPANPROD: BK_ER(FAM_EPANPROD, false); break;
+        default: BK_ER(FAM_EPANCONV, false); break;
+      }
     }
 #undef BK_ER
   }
@@ -1175,7 +1609,7 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              int n_items, long long r_i, long long r_end,
                              long long q_i, long long nq, int splits,
                              double *dst, double out_scale, int self_excl,
-                             hipStream_t s) {
+                             hipStream_t s, const BoundArgs *ba) {
   const double rho = p.gy / p.gx;
   if (items) {
     if (n_items <= 0) return hipSuccess;
@@ -1194,14 +1628,24 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
     long long len = (r_end - r_i + splits - 1) / splits;
     len = (len + kTile64 - 1) / kTile64 * kTile64;
     const dim3 grid((unsigned)((nq + kBlock64 - 1) / kBlock64), (unsigned)splits);
-#define BK_R64(F)                                                             \
-  direct_f64_range_kernel<F><<<grid, kBlock64, 0, s>>>(                       \
-      qx, qy, f0, f1, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho)
-    switch (p.family) {
-      case FAM_GAUSS: BK_R64(FAM_GAUSS); break;
-      case FAM_EPAN: BK_R64(FAM_EPAN); break;
-      case FAM_EPANPROD: BK_R64(FAM_EPANPROD); break;
-      default: BK_R64(FAM_EPANCONV); break;
+    const BoundArgs b = ba ? *ba : BoundArgs{};
+#define BK_R64(F, B)                                                          \
+  direct_f64_range_kernel<F, B><<<grid, kBlock64, 0, s>>>(                    \
+      qx, qy, f0, f1, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho, b)
+    if (ba) {
+      switch (p.family) {
+        case FAM_GAUSS: BK_R64(FAM_GAUSS, true); break;
+        case FAM_EPAN: BK_R64(FAM_EPAN, true); break;
+        case FAM_EPANPROD: BK_R64(FAM_EPANPROD, true); break;
+        default: BK_R64(FAM_EPANCONV, true); break;
+      }
+    } else {
+      switch (p.family) {
+        case FAM_GAUSS: BK_R64(FAM_GAUSS, false); break;
+        case FAM_EPAN: BK_R64(FAM_EPAN, false); break;
+        case FAM_EPANPROD: BK_R64(FAM_EPANPROD, false); break;
+        default: BK_R64(FAM_EPANCONV, false); break;
+      }
     }
 #undef BK_R64
   }
@@ -1386,7 +1830,7 @@ int gauss_resident_blocks() {
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1;
   if (cached[dev] == 0) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(gauss_range_kernel<true>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(gauss_range_kernel<true, false>),
                                                      kBlock, 0) != hipSuccess ||
         per_cu <= 0)
       per_cu = 1;

@@ -423,13 +423,16 @@ __global__ void check_samples_kernel(const float *__restrict__ qx, const float *
   out[i] = e;
 }
 
-// A cell falls back if a sample in it failed or if it holds no sample.
+// A cell falls back if a sample in it failed or if it holds no sample; its
+// sampled queries already hold their exact sums and are left out.
 __global__ void select_fallback_kernel(const float *__restrict__ qx, const float *__restrict__ qy, long long n,
                                        const unsigned int *__restrict__ cell_err,
                                        const unsigned int *__restrict__ cell_cnt, float tol, float lo, float inv_w,
-                                       int G, int *__restrict__ count, long long *__restrict__ list, long long cap) {
+                                       int G, unsigned long long seed, unsigned long long period,
+                                       int *__restrict__ count, long long *__restrict__ list, long long cap) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (splitmix64(seed ^ (unsigned long long)i) % period == 0) return;
   const int c = cell_of(qx[i], qy[i], lo, inv_w, G);
   if (__uint_as_float(cell_err[c]) > tol || cell_cnt[c] == 0u) {
     const int k = atomicAdd(count, 1);
@@ -478,7 +481,7 @@ struct bk_mlp {
   int ws_batch = 0;
   DevBuf X, T, Z1, H1, Z2, H2, Z3, H3, dZ3, dZ2, dZ1, dH, sq, dy, part;
   // serving workspace
-  DevBuf slist, flist, cnt, cell_err, gx, gy, exact, fx, fy, fexact;
+  DevBuf slist, flist, cnt, cell_err, gx, gy, exact, fx, fy, fexact, keys_a, keys_b, sort_tmp;
   long long serve_cap = 0, fb_cap = 0;
   long long steps_done = 0;
 };
@@ -665,7 +668,7 @@ int bk_mlp_destroy(bk_mlp *p) {
       for (DevBuf *b : {&p->theta, &p->m, &p->v, &p->grad, &p->wq, &p->pk, &p->state, &p->lr, &p->xin, &p->tin,
                         &p->X, &p->T, &p->Z1, &p->H1, &p->Z2, &p->H2, &p->Z3, &p->H3, &p->dZ3, &p->dZ2, &p->dZ1,
                         &p->dH, &p->sq, &p->dy, &p->part, &p->slist, &p->flist, &p->cnt, &p->cell_err, &p->gx,
-                        &p->gy, &p->exact, &p->fx, &p->fy, &p->fexact})
+                        &p->gy, &p->exact, &p->fx, &p->fy, &p->fexact, &p->keys_a, &p->keys_b, &p->sort_tmp})
         b->release();
       for (hipEvent_t e : p->ev)
         if (e) (void)hipEventDestroy(e);
@@ -920,13 +923,20 @@ int bk_mlp_serve_device(bk_mlp *p, bk_ctx *ctx, const bk_kernel_desc *desc, size
     // sampled check
     hipLaunchKernelGGL(select_samples_kernel, dim3(grid_for(N, 256)), dim3(256), 0, s, N, seed, period, cnt,
                        p->slist.as<long long>(), scap);
-    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(scap, 256)), dim3(256), 0, s, qx, qy, p->slist.as<long long>(),
-                       cnt, scap, p->gx.as<float>(), p->gy.as<float>());
     hip_check(hipGetLastError(), "sample launch");
     int ns = 0;
     hip_check(hipMemcpyAsync(&ns, cnt, sizeof(int), hipMemcpyDeviceToHost, s), "read sample count");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (ns > scap) fail(BK_ERR_STATE, "internal: sample list overflow");
+    // Z-curve order: each workgroup of the exact sum gets nearby queries and
+    // skips the reference tiles beyond the kernel's reach; the list is sorted
+    // before the gather, so exact[k] belongs to query slist[k]
+    if (N < (1LL << 32))
+      bk::spatial_sort(qx, qy, p->slist.as<long long>(), ns, N, (float)p->lo, (float)p->hi, p->slist.as<long long>(),
+                       p->keys_a, p->keys_b, p->sort_tmp, s);
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(scap, 256)), dim3(256), 0, s, qx, qy, p->slist.as<long long>(),
+                       cnt, scap, p->gx.as<float>(), p->gy.as<float>());
+    hip_check(hipGetLastError(), "gather launch");
     bk::eval_queries_dev(ctx, desc, p->gx.as<float>(), p->gy.as<float>(), (size_t)ns, p->exact.as<double>(), s);
     hipLaunchKernelGGL(check_samples_kernel, dim3(grid_for(std::max(ns, 1), 256)), dim3(256), 0, s, qx, qy,
                        p->slist.as<long long>(), cnt, scap, p->exact.as<double>(), out, floor_density, lo, inv_w, G,
@@ -935,7 +945,7 @@ int bk_mlp_serve_device(bk_mlp *p, bk_ctx *ctx, const bk_kernel_desc *desc, size
     hip_check(hipEventRecord(p->ev[2], s), "hipEventRecord");
     // fallback of failing cells
     hipLaunchKernelGGL(select_fallback_kernel, dim3(grid_for(N, 256)), dim3(256), 0, s, qx, qy, N, cerr, ccnt,
-                       (float)rel_tol, lo, inv_w, G, cnt + 1, p->flist.as<long long>(), N);
+                       (float)rel_tol, lo, inv_w, G, seed, period, cnt + 1, p->flist.as<long long>(), N);
     hip_check(hipGetLastError(), "fallback selection launch");
     int nf = 0;
     unsigned int maxbits = 0;
@@ -943,6 +953,9 @@ int bk_mlp_serve_device(bk_mlp *p, bk_ctx *ctx, const bk_kernel_desc *desc, size
     hip_check(hipMemcpyAsync(&maxbits, cerr + G * G, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "read error");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (nf > 0) {
+      if (N < (1LL << 32))
+        bk::spatial_sort(qx, qy, p->flist.as<long long>(), nf, N, (float)p->lo, (float)p->hi, p->flist.as<long long>(),
+                       p->keys_a, p->keys_b, p->sort_tmp, s);
       hipLaunchKernelGGL(gather_kernel, dim3(grid_for(nf, 256)), dim3(256), 0, s, qx, qy, p->flist.as<long long>(),
                          cnt + 1, N, p->gx.as<float>(), p->gy.as<float>());
       hip_check(hipGetLastError(), "gather launch");

@@ -1,0 +1,112 @@
+// Spatial ordering of query lists: a device radix sort of Morton keys.
+//
+// A workgroup of the FP32 Gaussian kernel skips every reference tile whose
+// bounding box lies beyond the kernel's reach from the bounding box of the
+// workgroup's queries.  That only pays when a workgroup's queries are close
+// together, so callers that evaluate scattered query subsets (the
+// surrogate's sampled check and its fallback) sort them along a Z curve
+// first and scatter the results back through the same list.
+#include "bk_common.h"
+#include "bk_internal.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+namespace bk {
+
+namespace {
+
+constexpr int kMortonBits = 10;  // cells per axis: 2^kMortonBits
+
+__device__ __forceinline__ unsigned int spread_bits(unsigned int v) {
+  v &= 0x0000FFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+__global__ void morton_keys_kernel(const float *__restrict__ qx, const float *__restrict__ qy,
+                                   const long long *__restrict__ list, long long n, float lo, float scale,
+                                   int index_bits, unsigned long long *__restrict__ keys) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long long i = list ? list[k] : k;
+  constexpr float top = (float)((1 << kMortonBits) - 1);
+  // fminf/fmaxf return the number when the other operand is NaN
+  const float fx = __builtin_fminf(__builtin_fmaxf((qx[i] - lo) * scale, 0.0f), top);
+  const float fy = __builtin_fminf(__builtin_fmaxf((qy[i] - lo) * scale, 0.0f), top);
+  const unsigned int code = spread_bits((unsigned int)fx) | (spread_bits((unsigned int)fy) << 1);
+  keys[k] = ((unsigned long long)code << index_bits) | (unsigned long long)i;
+}
+
+__global__ void keys_to_list_kernel(const unsigned long long *__restrict__ keys, long long n,
+                                    unsigned long long mask, long long *__restrict__ list) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) list[k] = (long long)(keys[k] & mask);
+}
+
+__global__ void gather_xy_kernel(const float *__restrict__ qx, const float *__restrict__ qy,
+                                 const long long *__restrict__ list, long long n, float *__restrict__ gx,
+                                 float *__restrict__ gy) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long long i = list[k];
+  gx[k] = qx[i];
+  gy[k] = qy[i];
+}
+
+__global__ void scatter_f64_kernel(const long long *__restrict__ list, long long n, const double *__restrict__ vals,
+                                   double *__restrict__ out) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[list[k]] = vals[k];
+}
+
+}  // namespace
+
+hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
+                            float *gy, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  gather_xy_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(qx, qy, list, n, gx, gy);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_f64(const long long *list, long long n, const double *vals, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  scatter_f64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(list, n, vals, out);
+  return hipGetLastError();
+}
+
+void spatial_sort(const float *qx, const float *qy, const long long *list, long long n, long long index_limit,
+                  float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
+                  hipStream_t s) {
+  if (n <= 0) return;
+  if (!list) index_limit = n;
+  if (index_limit < n || index_limit > (1LL << 32))
+    fail(BK_ERR_INVALID_ARG, "spatial_sort: indices must lie in [0, 2^32) and cover the list");
+  if (!(hi > lo)) fail(BK_ERR_INVALID_ARG, "spatial_sort: empty box");
+  // the key holds the index in its low bits: enough bits for every index
+  int index_bits = 1;
+  while ((1LL << index_bits) < index_limit) ++index_bits;
+  const int end_bit = index_bits + 2 * kMortonBits;
+  keys_a.reserve((size_t)n * sizeof(unsigned long long));
+  keys_b.reserve((size_t)n * sizeof(unsigned long long));
+  size_t tmp_bytes = 0;
+  hip_check(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys_a.as<unsigned long long>(),
+                                              keys_b.as<unsigned long long>(), (int)n, 0, end_bit, s),
+            "radix sort size query");
+  tmp.reserve(tmp_bytes + 256);
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  const float scale = (float)((double)(1 << kMortonBits) / ((double)hi - (double)lo));
+  morton_keys_kernel<<<grid, 256, 0, s>>>(qx, qy, list, n, lo, scale, index_bits, keys_a.as<unsigned long long>());
+  hip_check(hipGetLastError(), "morton key launch");
+  hip_check(hipcub::DeviceRadixSort::SortKeys(tmp.ptr, tmp_bytes, keys_a.as<unsigned long long>(),
+                                              keys_b.as<unsigned long long>(), (int)n, 0, end_bit, s),
+            "radix sort");
+  keys_to_list_kernel<<<grid, 256, 0, s>>>(keys_b.as<unsigned long long>(), n, (1ULL << index_bits) - 1ULL,
+                                           list_out);
+  hip_check(hipGetLastError(), "key unpack launch");
+}
+
+}  // namespace bk

@@ -11,9 +11,9 @@
 // reference GPU callers' type), double the FP64 path.  block_size is accepted
 // for source compatibility; launch shapes are chosen by the library.
 //
-// Extension: unnormalized_eval_pairs evaluates many (reference range, query
-// range) pairs in one launch, which the dual-tree traversal uses instead of
-// one synchronous call per leaf pair.
+// Extensions: unnormalized_eval_pairs evaluates many (reference range, query
+// range) pairs in one launch; unnormalized_eval_bounded runs the whole
+// dual-tree evaluation (bounds within a tolerance) on the device.
 #ifndef BBRCIT_AMD_HIPDIRECTKDE_H_
 #define BBRCIT_AMD_HIPDIRECTKDE_H_
 
@@ -139,6 +139,23 @@ class HipDirectKde {
     if (pairs.empty()) return;
     const bk_kernel_desc d = kernel_.desc();
     hip_detail::check(bk_eval_leaf_pairs(ctx_, &d, pairs.data(), pairs.size(), out.data(), 0));
+  }
+
+  // Bounds of the unnormalised sums over every reference for queries
+  // q_i..q_j: upper - lower <= max(abs_err, rel_err * lower)
+  // (bk_eval_bounded).
+  void unnormalized_eval_bounded(size_t q_i, size_t q_j, double rel_err, double abs_err, std::vector<double> &lower,
+                                 std::vector<double> &upper) {
+    require_ctx();
+    if (q_j >= query_size() || reference_size() == 0)
+      throw std::out_of_range("HipDirectKde<>: unnormalized_eval_bounded(): j_q should be less than the number of "
+                              "query points. ");
+    if (q_i > q_j) throw std::invalid_argument("HipDirectKde<>: unnormalized_eval_bounded(): must have i_q <= j_q. ");
+    lower.resize(q_j - q_i + 1);
+    upper.resize(q_j - q_i + 1);
+    const bk_kernel_desc d = kernel_.desc();
+    hip_detail::check(
+        bk_eval_bounded(ctx_, &d, q_i, q_j, rel_err, abs_err, lower.data(), upper.data(), 0, nullptr));
   }
 
   bk_ctx *context() const { return ctx_; }

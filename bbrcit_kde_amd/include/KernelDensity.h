@@ -11,15 +11,14 @@
 //     lower/upper bounds of each query's attributes, value() = midpoint;
 //   * direct_eval keeps the query order; eval(vector) returns the queries in
 //     the query tree's order (sorted, duplicates merged, partitioned);
-//   * the dual-tree evaluation prunes with the reference's bounds and
-//     tolerance rules (681-1110).
+//   * the dual-tree evaluation meets the reference's bounds and tolerance
+//     contract (681-1110).
 // Device work goes through the C ABI (bbrcit_hip.h):
 //   * every direct sum is one GPU evaluation;
-//   * the dual tree's leaf-leaf base cases are collected during the host
-//     traversal and evaluated together by bk_eval_leaf_pairs, instead of one
-//     synchronous launch per leaf pair; pruning decisions therefore see the
-//     node bounds without the pending base cases, which are looser but valid,
-//     so the tree prunes no more than the reference does;
+//   * the dual-tree evaluation is one bounded GPU evaluation
+//     (bk_eval_bounded): the pruning is decided per (query block, reference
+//     tile) on the device against the same bounds contract, instead of a
+//     host recursion with one synchronous launch per leaf pair;
 //   * cross-validation scores and the adaptive pass are exact direct sums on
 //     the GPU with FP64 reductions (the leave-one-out pair excluded by index);
 //     rel_err / abs_err are accepted and met trivially.
@@ -313,170 +312,26 @@ class KernelDensity {
   }
 
   // ---- dual tree (KernelDensityImpl.h:681-1110) ------------------------------------
-  // Base cases collected during the traversal, evaluated on the GPU in one
-  // launch per batch and applied to the per-query bounds afterwards.
-  template <typename K>
-  struct LeafBatch {
-    Device<K> *dev;
-    KdtreeType *qtree;
-    std::vector<bk_range_pair> pairs;
-    std::vector<FloatT> dmass;  // mass of the reference leaf of each pair
-    size_t pending = 0;
-    static constexpr size_t kMaxPending = size_t(1) << 24;  // queries per flush
-
-    void add(const TreeNodeType *Dn, const TreeNodeType *Qn) {
-      bk_range_pair p;
-      p.r_i = Dn->start_idx_;
-      p.r_j = Dn->end_idx_;
-      p.q_i = Qn->start_idx_;
-      p.q_j = Qn->end_idx_;
-      pairs.push_back(p);
-      dmass.push_back(Dn->attr_.mass());
-      pending += Qn->size();
-      if (pending >= kMaxPending) flush();
-    }
-
-    // per query: upper += S - M_D, lower += S, lower <= upper
-    // (the GPU base case of KernelDensityImpl.h:921-983)
-    void flush() {
-      if (pairs.empty()) return;
-      std::vector<double> s;
-      dev->unnormalized_eval_pairs(pairs, s);
-      size_t off = 0;
-      for (size_t k = 0; k < pairs.size(); ++k) {
-        for (size_t i = pairs[k].q_i; i <= pairs[k].q_j; ++i, ++off) {
-          auto &a = qtree->points_[i].attributes();
-          FloatT up = a.upper() + (FloatT)s[off] - dmass[k];
-          const FloatT lo = a.lower() + (FloatT)s[off];
-          if (lo > up) up = lo;
-          a.set_lower(lo);
-          a.set_upper(up);
-        }
-      }
-      pairs.clear();
-      dmass.clear();
-      pending = 0;
-    }
-  };
-
+  // The whole traversal runs on the GPU (bk_eval_bounded): queries in the
+  // query tree's order are taken in blocks, references in tiles of the data
+  // tree's order, and each (block, tile) pair is pruned or summed there.
+  // The contract is the reference's: per query, lower <= f <= upper with
+  // upper - lower <= max(abs_err, rel_err * lower), value() the midpoint.
   template <typename K>
   void tree_eval(KdtreeType &qtree, const K &kernel, FloatType rel_err, FloatType abs_err) const {
     if (qtree.empty()) return;
     require_data();
-    for (auto &q : qtree.points_) {
-      q.attributes().set_lower(0);
-      q.attributes().set_upper(data_tree_.root_->attr_.mass());
-    }
-    qtree.refresh_node_attributes(qtree.root_);
     const FloatType norm = (FloatType)kernel.normalization();
     Device<K> dev(points(), qtree.points());
     dev.kernel() = kernel;
-    LeafBatch<K> batch{&dev, &qtree, {}, {}, 0};
-    dual_tree(data_tree_.root_, qtree.root_, kernel, FloatType(1), FloatType(0), rel_err, abs_err / norm, qtree,
-              batch);
-    batch.flush();
-    for (auto &q : qtree.points_) {
-      q.attributes().set_lower(q.attributes().lower() * norm);
-      q.attributes().set_upper(q.attributes().upper() * norm);
-      report_error(std::cerr, q.point(), q.attributes().upper(), q.attributes().lower(), rel_err, abs_err);
+    std::vector<double> lo, up;
+    dev.unnormalized_eval_bounded(0, qtree.size() - 1, (double)rel_err, (double)(abs_err / norm), lo, up);
+    for (size_t i = 0; i < qtree.points_.size(); ++i) {
+      auto &a = qtree.points_[i].attributes();
+      a.set_lower((FloatType)lo[i] * norm);
+      a.set_upper((FloatType)up[i] * norm);
+      report_error(std::cerr, qtree.points_[i].point(), a.upper(), a.lower(), rel_err, abs_err);
     }
-  }
-
-  template <typename K>
-  void dual_tree(const TreeNodeType *Dn, TreeNodeType *Qn, const K &kernel, FloatType du, FloatType dl,
-                 FloatType rel_err, FloatType abs_err, KdtreeType &qtree, LeafBatch<K> &batch) const {
-    FloatType du_new, dl_new;
-    estimate_contributions(Dn, Qn->bbox_, kernel, du_new, dl_new);
-
-    if (can_approximate(Dn, du_new, dl_new, du, dl, Qn->attr_.upper(), Qn->attr_.lower(), rel_err, abs_err)) {
-      tighten_node(Dn, Qn, du_new, dl_new, du, dl);
-      for (auto i = Qn->start_idx_; i <= Qn->end_idx_; ++i) {
-        auto &a = qtree.points_[i].attributes();
-        FloatType up = a.upper(), lo = a.lower();
-        tighten_bounds(Dn, du_new, dl_new, FloatType(1), FloatType(0), up, lo);
-        a.set_upper(up);
-        a.set_lower(lo);
-      }
-      return;
-    }
-
-    if (Qn->is_leaf() && Dn->is_leaf()) {
-      batch.add(Dn, Qn);
-      return;
-    }
-
-    if (Qn->is_leaf()) {
-      tighten_node(Dn, Qn, du_new, dl_new, du, dl);
-      const TreeNodeType *near = Dn->left_, *far = Dn->right_;
-      order_by_distance(&near, &far, Qn->bbox_);
-      dual_tree(near, Qn, kernel, du_new, dl_new, rel_err, abs_err, qtree, batch);
-      dual_tree(far, Qn, kernel, du_new, dl_new, rel_err, abs_err, qtree, batch);
-      return;
-    }
-
-    tighten_node(Dn, Qn->left_, du_new, dl_new, du, dl);
-    tighten_node(Dn, Qn->right_, du_new, dl_new, du, dl);
-    if (Dn->is_leaf()) {
-      dual_tree(Dn, Qn->left_, kernel, du_new, dl_new, rel_err, abs_err, qtree, batch);
-      dual_tree(Dn, Qn->right_, kernel, du_new, dl_new, rel_err, abs_err, qtree, batch);
-    } else {
-      for (TreeNodeType *Qc : {Qn->left_, Qn->right_}) {
-        const TreeNodeType *near = Dn->left_, *far = Dn->right_;
-        order_by_distance(&near, &far, Qc->bbox_);
-        dual_tree(near, Qc, kernel, du_new, dl_new, rel_err, abs_err, qtree, batch);
-        dual_tree(far, Qc, kernel, du_new, dl_new, rel_err, abs_err, qtree, batch);
-      }
-    }
-    Qn->attr_.set_lower(std::min(Qn->left_->attr_.lower(), Qn->right_->attr_.lower()));
-    Qn->attr_.set_upper(std::max(Qn->left_->attr_.upper(), Qn->right_->attr_.upper()));
-  }
-
-  // bounds of the kernel over the node: its closest and farthest distance
-  // to obj on every axis, at the node's largest / smallest abw
-  template <typename ObjT, typename K>
-  void estimate_contributions(const TreeNodeType *Dn, const ObjT &obj, const K &kernel, FloatType &du,
-                              FloatType &dl) const {
-    GeomPointType near, far;
-    const GeomPointType origin;
-    for (int i = 0; i < D; ++i) {
-      near[i] = Dn->bbox_.min_dist(i, obj);
-      far[i] = Dn->bbox_.max_dist(i, obj);
-    }
-    du = (FloatType)kernel.unnormalized_eval(near, origin, (KernelFloatType)Dn->attr_.upper_abw());
-    dl = (FloatType)kernel.unnormalized_eval(far, origin, (KernelFloatType)Dn->attr_.lower_abw());
-  }
-
-  // add the node's new contribution bounds, removing the previous ones
-  void tighten_bounds(const TreeNodeType *Dn, FloatType du_new, FloatType dl_new, FloatType du, FloatType dl,
-                      FloatType &upper, FloatType &lower) const {
-    lower += Dn->attr_.mass() * (dl_new - dl);
-    upper += Dn->attr_.mass() * (du_new - du);
-    if (lower > upper) upper = lower;
-  }
-
-  void tighten_node(const TreeNodeType *Dn, TreeNodeType *Qn, FloatType du_new, FloatType dl_new, FloatType du,
-                    FloatType dl) const {
-    FloatType up = Qn->attr_.upper(), lo = Qn->attr_.lower();
-    tighten_bounds(Dn, du_new, dl_new, du, dl, up, lo);
-    Qn->attr_.set_upper(up);
-    Qn->attr_.set_lower(lo);
-  }
-
-  // prune when the absolute error stays within abs_err (exclusion or
-  // approximation) or the relative error within rel_err (Deng & Moore,
-  // Gray & Moore; KernelDensityImpl.h:1043-1075)
-  bool can_approximate(const TreeNodeType *Dn, FloatType du_new, FloatType dl_new, FloatType du, FloatType dl,
-                       FloatType upper, FloatType lower, FloatType rel_err, FloatType abs_err) const {
-    const FloatType abs_tol = 2 * abs_err / data_tree_.size();
-    if (std::abs(du_new) <= abs_tol) return true;
-    if (std::abs(du_new - dl_new) <= abs_tol) return true;
-    tighten_bounds(Dn, du_new, dl_new, du, dl, upper, lower);
-    return std::abs(upper - lower) <= abs_err || std::abs(upper - lower) <= std::abs(lower) * rel_err;
-  }
-
-  template <typename ObjT>
-  static void order_by_distance(const TreeNodeType **near, const TreeNodeType **far, const ObjT &obj) {
-    if ((*near)->bbox_.min_dist(obj) > (*far)->bbox_.min_dist(obj)) std::swap(*near, *far);
   }
 
   void report_error(std::ostream &os, const GeomPointType &p, FloatType upper, FloatType lower, FloatType rel_err,

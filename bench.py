@@ -71,8 +71,10 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
     ap.add_argument("--sur-steps", type=int, default=16_000, help="training steps of the surrogate")
-    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate,gauss16,adaptive",
-                    help="comma list: gauss, epanprod, cvgrid, surrogate, gauss16, adaptive")
+    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate,gauss16,adaptive,dualtree",
+                    help="comma list: gauss, epanprod, cvgrid, surrogate, gauss16, adaptive, dualtree")
+    ap.add_argument("--no-spatial-order", action="store_true",
+                    help="skip the timing of the same evaluations in device-sorted query order")
     ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
                     help="Gaussian exponent form (auto: tile-centred on compact tiles)")
     return ap.parse_args()
@@ -310,6 +312,24 @@ def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int, t
         torch.cuda.synchronize()
         served = out.cpu().numpy()
         rawv = raw.cpu().numpy()
+    # the exact sum of every served query with the same device query sort and
+    # tile culling the surrogate's check and fallback use: what the network
+    # gains over the exact method without it
+    exact = torch.empty(SUR_SERVE, dtype=torch.float64, device="cuda")
+    with bk.DirectKde(pts, qs, mass) as ex:
+        ex.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
+        stream = torch.cuda.Stream()
+        ex.set_stream(stream.cuda_stream)
+        ex.eval_device(desc, 0, len(pts) - 1, 0, SUR_SERVE - 1, exact.data_ptr(), True)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        ex.eval_device(desc, 0, len(pts) - 1, 0, SUR_SERVE - 1, exact.data_ptr(), True)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        exact_sorted_ms = ev0.elapsed_time(ev1)
+        ex.set_stream(0)
+    del exact
     ms = float(np.mean([st["ms_total"] for st in stats]))
     st = stats[-1]
     # independent audit: every 1000th query, exact direct sum
@@ -326,6 +346,12 @@ def bench_surrogate(bk, pts, mass, direct_qps: float, steps: int, warmup: int, t
                     "exact sample, exact fallback of 64x64 cells with sampled rel. error > 1e-3",
         "unit": "queries/s", "value": qps, "ms_per_step": ms, "steps": steps,
         "vs_direct": qps / direct_qps, "direct_queries_per_s": direct_qps,
+        "exact_sorted_queries_per_s": SUR_SERVE / (exact_sorted_ms * 1e-3), "exact_sorted_ms": exact_sorted_ms,
+        "vs_exact_sorted": qps / (SUR_SERVE / (exact_sorted_ms * 1e-3)),
+        "note": "the check and the fallback evaluate their queries in device-sorted Z-curve order, where each "
+                "workgroup skips the reference tiles beyond 2^-100 of the kernel; vs_direct compares with the "
+                "plain direct sum of random-order queries (the headline), vs_exact_sorted with the exact sum of "
+                "all served queries in the same sorted order",
         "raw_inference_queries_per_s": raw_qps, "raw_vs_direct": raw_qps / direct_qps,
         "label_s": t_label, "train_ms": train_ms, "train_batch": SUR_BATCH,
         "final_loss": loss, "log_rmse": float(np.sqrt(loss) * sigma), "train_steps_run": train_steps,
@@ -415,6 +441,75 @@ def bench_gauss16(bk, steps: int, warmup: int, cpu: bool, seconds: float):
     return res
 
 
+DT_SIZES = (1 << 19, 1 << 22)   # test/test_kde11.cc sizes 2 << 18 and 2 << 21
+DT_H, DT_REL, DT_ABS = 0.1, 1e-6, 1e-10
+DT_CPU_N = 1 << 18
+
+
+def bench_dual_tree(bk, steps: int, warmup: int, cpu: bool):
+    """The reference's dual-tree benchmark (test/test_kde11.cc): N standard-
+    normal 2-D points evaluated at themselves, EpanechnikovKernel<2,float>
+    h = 0.1, rel 1e-6, abs 1e-10; here one bounded device evaluation over the
+    points in kd order (bk_eval_bounded_device)."""
+    import torch
+
+    from bbrcit_kde_amd import datasets as ds
+
+    desc = bk.KernelDesc(bk.BK_EPANECHNIKOV, 2, float(np.float32(DT_H)))
+    res = {"workload": "test_kde11: N standard-normal 2-D points evaluated at themselves, "
+                       "EpanechnikovKernel<2,float> h=0.1, rel 1e-6, abs 1e-10, points in kd order, bounds on "
+                       "the device", "unit": "points/s", "sizes": {}}
+    for n in DT_SIZES:
+        pts = ds.standard_normal(n, 2, 1011)
+        pts = pts[ds.kd_order(pts, 64)]
+        mass = np.full(n, 1.0 / n)
+        lo = torch.empty(n, dtype=torch.float64, device="cuda")
+        up = torch.empty(n, dtype=torch.float64, device="cuda")
+        with bk.DirectKde(pts, None, mass) as kde:
+            stream = torch.cuda.Stream()
+            kde.set_stream(stream.cuda_stream)
+            for _ in range(warmup):
+                kde.eval_bounded_device(desc, 0, n - 1, DT_REL, DT_ABS, lo.data_ptr(), up.data_ptr())
+            torch.cuda.synchronize()
+            ms = []
+            for _ in range(steps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                kde.eval_bounded_device(desc, 0, n - 1, DT_REL, DT_ABS, lo.data_ptr(), up.data_ptr())
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms.append(e0.elapsed_time(e1))
+            kde.set_stream(0)
+            _, _, st = kde.eval_bounded(desc, 0, min(n, 1 << 16) - 1, DT_REL, DT_ABS)
+        l, u = lo.cpu().numpy(), up.cpu().numpy()
+        gap_ok = float(np.mean((u - l) <= np.maximum(DT_ABS, DT_REL * l) * (1 + 1e-5)))
+        avg = float(np.mean(ms))
+        res["sizes"][str(n)] = {"ms": avg, "points_per_s": n / (avg * 1e-3), "frac_within_tolerance": gap_ok,
+                                "pruning_first_65536_queries": st}
+        del lo, up
+    big = res["sizes"][str(DT_SIZES[-1])]
+    res["value"] = big["points_per_s"]
+    res["ms_per_step"] = big["ms"]
+    res["reference_published"] = {"n": DT_SIZES[-1], "gpu_ms": 37719.9, "cpu_ms": 489527.0,
+                                  "source": "test/cached_outputs/test_kde11_{gpu,cpu}.csv:15 (unknown sm_35 GPU "
+                                            "and CPU; includes the host tree build and transfers)"}
+    if cpu:
+        from oracle.oracle import REF_LIB, Reference
+
+        if REF_LIB.exists():
+            pts = ds.standard_normal(DT_CPU_N, 2, 1011)
+            t0 = time.perf_counter()
+            Reference().tree_eval(bk.BK_EPANECHNIKOV, 1, DT_H, DT_H, pts, np.ones(DT_CPU_N), pts, DT_REL, DT_ABS,
+                                  leaf_nmax=32, q_leaf_nmax=32)
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": DT_CPU_N / dt, "unit": "points/s", "cores": 1, "kind": "reference",
+                                   "sample": f"N={DT_CPU_N}: KernelDensity<2,EpanechnikovKernel<2,float>,double>"
+                                             "::eval(queries, 1e-6, 1e-10, 32) (its dual tree, including the query "
+                                             "tree build), 1 thread",
+                                   "seconds": dt}
+    return res
+
+
 def bench_adaptive(bk, pts, mass, q):
     from bbrcit_kde_amd import datasets as ds
 
@@ -446,7 +541,9 @@ def bench_adaptive(bk, pts, mass, q):
                 "s_adaptive_eval": t4 - t3, "g": g, "lcv": lcv,
                 "lambda_range": [float(lam.min()), float(lam.max())],
                 "timing": "host wall clock around each synchronous C-ABI call (includes the FP64 reductions and "
-                          "the host mass update of the adaptive pass)"})
+                          "the host mass update of the adaptive pass)",
+                "note": "the pilot and leave-one-out self sums run over kd-ordered points, so their workgroups "
+                        "skip the reference tiles beyond 2^-100 of the kernel; pairs counts every ordered pair"})
     return res
 
 
@@ -469,7 +566,7 @@ def main() -> int:
         if args.gauss_form == "direct":
             kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         for k in kinds:
-            if k in ("cvgrid", "surrogate", "gauss16", "adaptive"):
+            if k in ("cvgrid", "surrogate", "gauss16", "adaptive", "dualtree"):
                 continue
             if k == "gauss":
                 desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS
@@ -484,6 +581,19 @@ def main() -> int:
             results[k] = {"vals": vals, "ms": ms, "avg_ms": avg_ms, "pair_evals_per_s": pairs / (avg_ms * 1e-3),
                           "tflops": pairs * flop / (avg_ms * 1e-3) / 1e12, "wall_s": wall}
             log(f"{k}: {avg_ms:.3f} ms/step, {pairs / (avg_ms * 1e-3):.4e} pair-evals/s")
+            if args.no_spatial_order:
+                continue
+            # the same evaluation with the queries sorted on the device first
+            kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
+            svals, sms, _ = time_kernel(kde, desc, min(args.steps, 3), 1)
+            kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_GIVEN)
+            results[k]["spatial_order"] = {
+                "ms_per_step": float(np.mean(sms)), "speedup": avg_ms / float(np.mean(sms)),
+                "max_abs_diff_vs_given": float(np.max(np.abs(svals - vals))),
+                "note": "BK_QUERY_ORDER_SPATIAL: queries sorted along a Z curve on the device (sort, gather, "
+                        "scatter inside the timed region); workgroups skip reference tiles beyond 2^-100 of "
+                        "the kernel, so this is not a pair-evals/s figure"}
+            log(f"{k} spatial order: {np.mean(sms):.3f} ms/step")
 
     cv_res = None
     if "cvgrid" in kinds:
@@ -508,6 +618,12 @@ def main() -> int:
         log("adaptive pass + LOO LCV + adaptive evaluation (1M)")
         ad_res = bench_adaptive(bk, pts, mass, q)
         log(f"adaptive: {ad_res['value']:.4e} pair-evals/s")
+
+    dt_res = None
+    if "dualtree" in kinds:
+        log("dual tree (test_kde11 workload, bounded device evaluation)")
+        dt_res = bench_dual_tree(bk, min(args.steps, 3), args.warmup, not args.no_cpu_baseline)
+        log(f"dualtree: {dt_res['ms_per_step']:.2f} ms at N={DT_SIZES[-1]}, {dt_res['value']:.4e} points/s")
 
     head = results.get("gauss") or next(iter(results.values()))
     line = {
@@ -536,10 +652,12 @@ def main() -> int:
     if "epanprod" in results:
         e = results["epanprod"]
         line["epanechnikov_product"] = {"pair_evals_per_s": e["pair_evals_per_s"], "ms_per_step": e["avg_ms"],
+                                        "spatial_order": e.get("spatial_order"),
                                         "arithmetic": "f64 on f32 points",
                                         "tflops": e["tflops"], "frac": e["tflops"] / PEAK_FP32_TFLOPS,
                                         "flop_per_pair": FLOP_PER_PAIR_EPANPROD, "hx": H_EPAN[0],
                                         "hy": H_EPAN[1]}
+    line["spatial_order"] = head.get("spatial_order")
     if not args.no_cpu_baseline:
         log("CPU baseline leg")
         cb = cpu_baseline(pts, mass, q, head["vals"], args.cpu_seconds)
@@ -558,6 +676,8 @@ def main() -> int:
         line["gauss16"] = g16_res
     if ad_res is not None:
         line["adaptive"] = ad_res
+    if dt_res is not None:
+        line["dual_tree"] = dt_res
     print(json.dumps(line), flush=True)
     return 0
 

@@ -78,6 +78,14 @@ extern "C" {
 #define BK_OPT_HIGH_DIM_FORM   2  /* distances for 8 <= D <= 16 (FP32 contexts):      */
 #define BK_HIGH_DIM_AUTO       0  /*   Gram form, dot products on FP64 MFMA (default) */
 #define BK_HIGH_DIM_DIRECT     1  /*   differences q - r on the FP64 VALU             */
+#define BK_OPT_QUERY_ORDER     3  /* order in which bk_eval_range(_device) works     */
+                                  /* through the queries (results always come back  */
+                                  /* in the caller's order):                        */
+#define BK_QUERY_ORDER_GIVEN   0  /*   as stored (default)                          */
+#define BK_QUERY_ORDER_SPATIAL 1  /*   sorted along a Z curve on the device first   */
+                                  /*   (FP32 contexts, D <= 2), so each workgroup   */
+                                  /*   holds nearby queries and skips the reference */
+                                  /*   tiles beyond the kernel's reach              */
 
 /* ---- POD types -------------------------------------------------------- */
 typedef struct bk_ctx bk_ctx;
@@ -173,6 +181,42 @@ int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc,
 int bk_eval_leaf_pairs(bk_ctx *ctx, const bk_kernel_desc *desc,
                        const bk_range_pair *pairs, size_t n_pairs,
                        double *out_host, int normalize);
+
+/* Counts of (query block, reference tile) pairs of one bounded
+ * evaluation: tiles passed over because the kernel is zero across the
+ * pair of boxes, passed over with their bounds, and summed exactly. */
+typedef struct bk_bound_stats {
+  unsigned long long excluded;
+  unsigned long long approximated;
+  unsigned long long evaluated;
+} bk_bound_stats;
+
+/* Dual-tree evaluation with error bounds over every reference, for the
+ * queries q_i..q_j: lower_host / upper_host[k - q_i] bracket the density
+ * of query k (times C(h) if normalize != 0), with
+ * upper - lower <= max(abs_err, rel_err * lower).  Queries are taken in
+ * blocks of consecutive indices (put them in tree or Z-curve order, or set
+ * BK_OPT_QUERY_ORDER) and references in aligned tiles of 256 in their
+ * stored (tree) order.  A tile whose kernel bounds over the two bounding
+ * boxes differ by at most max(rel_err * L, abs_err) / (total mass), L being
+ * a lower bound of the block's sums, enters both bounds through its mass;
+ * every other tile is summed exactly.  FP32 and FP64 contexts, D <= 2
+ * (larger D: exact sums, lower == upper).  Replaces
+ * KernelDensity::eval(query_tree, kernel, rel_err, abs_err)
+ * (include/KernelDensityImpl.h:681-737) with its traversal and pruning rules
+ * (747-983, 1043-1110): the same bounds contract, the pruning decided per
+ * (query block, reference tile) on the GPU.  stats may be NULL. */
+int bk_eval_bounded(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_t q_j,
+                    double rel_err, double abs_err, double *lower_host,
+                    double *upper_host, int normalize, bk_bound_stats *stats);
+
+/* As bk_eval_bounded, with the bounds in device memory; returns after the
+ * work is queued except for reading the pruning counts when stats is not
+ * NULL. */
+int bk_eval_bounded_device(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i,
+                           size_t q_j, double rel_err, double abs_err,
+                           double *lower_dev, double *upper_dev, int normalize,
+                           bk_bound_stats *stats);
 
 /* ---- self evaluations and cross-validation ---------------------------- */
 
@@ -306,7 +350,8 @@ typedef struct bk_mlp bk_mlp;
 typedef struct bk_serve_stats {
   size_t n;              /* queries served                                  */
   size_t n_samples;      /* queries checked against the exact sum           */
-  size_t n_fallback;     /* queries re-evaluated exactly (failing cells)    */
+  size_t n_fallback;     /* queries re-evaluated exactly (the unsampled     
+                            queries of failing cells)                       */
   size_t cells;          /* cells of the check grid                         */
   size_t cells_failed;   /* cells whose sampled error exceeded rel_tol or
                             that held no sample                             */

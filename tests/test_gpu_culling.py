@@ -1,0 +1,106 @@
+"""GPU parity of workgroup-level tile culling and of the spatial query order.
+
+A workgroup skips every reference tile whose bounding box lies beyond the
+kernel's reach from the bounding box of its queries.  For the Epanechnikov
+kinds the skipped terms are exactly zero, so results must be bit-identical
+with and without the spatial order.  For the Gaussian kinds the skipped
+terms of the factored tiles are zero too; those of the difference and Gram
+forms are below 2^-100 of the largest mass, so both orders must meet the
+oracle at the parity bar (relative 1e-5 above the floor of 1e-10 times an
+upper bound of the sum, absolute below it).  The cases use kd-ordered
+references and small bandwidths, where most tiles are out of reach of a
+workgroup of sorted queries.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from bbrcit_kde_amd import datasets as ds
+from tests.test_gpu_direct import assert_parity, f32
+
+pytestmark = pytest.mark.gpu
+
+EPAN, GAUSS, EPANPROD, GAUSSPROD, EPANCONV1, EPANPRODCONV, GAUSSCONV1, GAUSSPRODCONV = range(1, 9)
+CASES = [(GAUSS, (0.03, 0.03)), (GAUSSPROD, (0.025, 0.04)), (GAUSSPRODCONV, (0.025, 0.04)),
+         (EPAN, (0.06, 0.06)), (EPANPROD, (0.05, 0.08)), (EPANPRODCONV, (0.05, 0.08))]
+EPAN_KINDS = (EPAN, EPANPROD, EPANPRODCONV)
+
+
+def kd_mixture(n, seed):
+    pts, w = ds.gaussian_mixture(n, seed=seed)
+    perm = ds.kd_order(pts, 64)
+    pts, w = pts[perm], w[perm]
+    return pts, w / w.sum()
+
+
+def evaluate(bk, kde, desc, nq, order):
+    kde.set_option(bk.BK_OPT_QUERY_ORDER, order)
+    return kde.eval(desc, 0, kde.reference_size() - 1, 0, nq - 1)
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+@pytest.mark.parametrize("kind,h", CASES)
+def test_spatial_order_matches_oracle(bk, oracle, kind, h, adaptive):
+    n_ref, n_q = 24_000, 9_000
+    pts, mass = kd_mixture(n_ref, 500 + kind)
+    q = ds.uniform_box(n_q, -4.0, 4.0, 600 + kind)
+    abw = np.random.default_rng(7).uniform(0.7, 1.5, n_ref) if adaptive else None
+    desc = bk.KernelDesc(kind, 2, *h)
+    with bk.DirectKde(pts, q, mass, abw) as kde:
+        given = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_GIVEN)
+        spatial = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_SPATIAL)
+        again = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_SPATIAL)
+    want = oracle.direct(kind, h[0], h[1], f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
+    bound = oracle.normalization(kind, 2, *h) * mass.sum()
+    assert_parity(spatial, want, bound, what=f"spatial kind {kind} adaptive {adaptive}")
+    assert_parity(given, want, bound, what=f"given kind {kind} adaptive {adaptive}")
+    np.testing.assert_array_equal(spatial, again)  # deterministic order
+    if kind in EPAN_KINDS:
+        np.testing.assert_array_equal(spatial, given)
+    assert (want > 0).mean() > 0.05  # the case has queries inside the support
+
+
+def test_spatial_order_sub_range_and_tiny_sets(bk, oracle):
+    pts, mass = kd_mixture(5_000, 77)
+    q = ds.uniform_box(3_000, -3.0, 3.0, 78)
+    h = 0.04
+    desc = bk.KernelDesc(GAUSS, 2, h)
+    P, Q, M = f32(pts), f32(q), f32(mass)
+    with bk.DirectKde(pts, q, mass) as kde:
+        kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
+        for (r_i, r_j, q_i, q_j) in [(100, 4321, 250, 2999), (0, 4999, 0, 0), (17, 17, 5, 1400), (0, 4999, 10, 300)]:
+            got = kde.unnormalized_eval(desc, r_i, r_j, q_i, q_j)
+            want = oracle.direct(GAUSS, h, h, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
+            assert_parity(got, want, M[r_i:r_j + 1].sum(), what=f"range {(r_i, r_j, q_i, q_j)}")
+
+
+def test_spatial_order_identical_queries(bk, oracle):
+    # all queries at one point: the sort's box is degenerate
+    pts, mass = kd_mixture(4_000, 91)
+    q = np.tile([[0.3, -0.2]], (2_000, 1))
+    with bk.DirectKde(pts, q, mass) as kde:
+        kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
+        got = kde.eval(bk.KernelDesc(EPANPROD, 2, 0.2, 0.3), 0, 3999, 0, 1999)
+    want = oracle.direct(EPANPROD, 0.2, 0.3, f32(pts), f32(q), mass=f32(mass), normalize=True)
+    assert_parity(got, want, oracle.normalization(EPANPROD, 2, 0.2, 0.3))
+    assert np.all(got == got[0])
+
+
+@pytest.mark.parametrize("kind,h", [(GAUSS, (0.02, 0.02)), (EPANPROD, (0.04, 0.05))])
+def test_self_eval_culls_in_tree_order(bk, oracle, kind, h):
+    # references in kd order are their own sorted queries: the leave-one-out
+    # self evaluation culls without any option
+    pts, mass = kd_mixture(30_000, 300 + kind)
+    desc = bk.KernelDesc(kind, 2, *h)
+    with bk.DirectKde(pts, None, mass) as kde:
+        got = kde.self_eval(desc, exclude_self=True)
+    want = oracle.direct(kind, h[0], h[1], f32(pts), f32(pts), mass=f32(mass), exclude_self=True, normalize=True)
+    assert_parity(got, want, oracle.normalization(kind, 2, *h) * mass.sum(), what=f"loo kind {kind}")
+
+
+def test_query_order_option_is_checked(bk):
+    pts, mass = kd_mixture(1_000, 5)
+    with bk.DirectKde(pts, pts, mass) as kde:
+        with pytest.raises(ValueError):
+            kde.set_option(bk.BK_OPT_QUERY_ORDER, 7)

@@ -123,3 +123,55 @@ def test_serve_replaces_failing_cells_with_exact_sums(bk, oracle):
     assert saved >= 0
     print(f"serve: cells failed {st['cells_failed']}/{st['cells']}, fallback {st['n_fallback']}, "
           f"exact evaluations saved {saved} of {len(qs)}, max sampled err {st['max_sample_err']:.3e}")
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 (Steele, Lea, Flood 2014) on uint64 arrays, wrapping."""
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def test_serve_checks_exactly_the_deterministic_sample(bk, oracle):
+    # The sampled check takes query i when splitmix64(seed ^ i) % period == 0
+    # and sorts the sample along a Z curve before the exact sum.  With far
+    # more queries than samples (indices much wider than the sample count)
+    # and a tolerance no cell can fail, the served values must equal the
+    # surrogate's everywhere except at exactly those queries, which carry the
+    # exact sums.
+    torch = pytest.importorskip("torch")
+    pts, w = ds.gaussian_mixture(5000, seed=81)
+    mass = w / w.sum()
+    h = 0.15
+    desc = bk.KernelDesc(bk.BK_GAUSSIAN, 2, h)
+    floor = 1e-10 / (2 * np.pi * h * h)
+    n, seed, period = 1 << 20, 9, 100
+    qs = ds.uniform_box(n, -4, 4, 82)
+    with bk.DirectKde(pts, None, mass) as kde, bk.Surrogate(seed=5) as s:
+        qt = ds.uniform_box(8192, -4, 4, 83)
+        with bk.DirectKde(pts, qt, mass) as lab:
+            ft = lab.eval(desc, 0, len(pts) - 1, 0, len(qt) - 1)
+        s.set_training_data(qt, ft, -4.0, 4.0, 1e-3 * floor)
+        s.train(50, 1024, 3e-3, 1e-5, seed=6)
+        qx = torch.tensor(qs[:, 0], dtype=torch.float32, device="cuda")
+        qy = torch.tensor(qs[:, 1], dtype=torch.float32, device="cuda")
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        surr = torch.empty(n, dtype=torch.float64, device="cuda")
+        s.infer_device(n, qx.data_ptr(), qy.data_ptr(), surr.data_ptr())
+        st = s.serve_device(kde, desc, n, qx.data_ptr(), qy.data_ptr(), out.data_ptr(), sample_fraction=1.0 / period,
+                            rel_tol=1e30, floor=floor, cells=1, seed=seed)
+        torch.cuda.synchronize()
+        served, raw = out.cpu().numpy(), surr.cpu().numpy()
+    idx = np.arange(n, dtype=np.uint64)
+    sample = np.flatnonzero(_splitmix64(np.uint64(seed) ^ idx) % np.uint64(period) == 0)
+    assert st["n_samples"] == len(sample) and st["n_fallback"] == 0 and st["cells_failed"] == 0
+    assert sample.max() > 4 * len(sample)  # indices far wider than the sample count
+    others = np.ones(n, bool)
+    others[sample] = False
+    np.testing.assert_array_equal(served[others], raw[others])
+    want = oracle.direct(bk.BK_GAUSSIAN, h, h, ds.round_f32(pts), ds.round_f32(qs[sample]), mass=ds.round_f32(mass),
+                         normalize=True)
+    err = np.abs(served[sample] - want) / np.maximum(want, floor)
+    assert err.max() <= 1e-5
