@@ -72,7 +72,7 @@ struct bk_ctx {
   DevBuf ord_list, ord_keys_a, ord_keys_b, ord_tmp, ord_x, ord_y, ord_out;  // spatial query order
   // bounded evaluation: tile summaries of the reference set (per version),
   // per-block thresholds, pruning counters, bounds planes
-  DevBuf tsum, bnd_thr, bnd_counts, bnd_gap, bnd_hi;
+  DevBuf tsum, bnd_mask, bnd_lo, bnd_gap, bnd_counts, bnd_exact, bnd_hi;
   unsigned long long tsum_version = 0;
 
   // staging
@@ -628,12 +628,16 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   }
   ensure_tile_summary(ctx);
   const long long ntiles = (nr + bk::kTile - 1) / bk::kTile;
+  const long long nwords = (ntiles + 63) / 64;
   const double scale = path_scale(ctx, path, p, normalize);
+  const double factor = normalize ? p.norm : 1.0;  // mass units -> output units
   const double abs_u = normalize ? abs_err / p.norm : abs_err;
   const double m_total = ctx->refs.m_sum > 0.0 ? ctx->refs.m_sum : 1.0;
   const int qpb = path == Path::F64 ? bk::kBlock : queries_per_item(path);
   const long long nblocks = (nq + qpb - 1) / qpb;
-  ctx->bnd_thr.reserve((size_t)nblocks * sizeof(double));
+  ctx->bnd_mask.reserve((size_t)(nblocks * nwords) * sizeof(unsigned long long));
+  ctx->bnd_lo.reserve((size_t)nblocks * sizeof(double));
+  ctx->bnd_gap.reserve((size_t)nblocks * sizeof(double));
   ctx->bnd_counts.reserve(4 * sizeof(unsigned long long));
   hip_check(hipMemsetAsync(ctx->bnd_counts.ptr, 0, 4 * sizeof(unsigned long long), s), "clear counters");
   const PointSet &qs = query_set(ctx);
@@ -664,36 +668,26 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
     qx32 = qs.x.as<float>();
     qy32 = qs.y.as<float>();
   }
-  double *thr = ctx->bnd_thr.as<double>();
+  unsigned long long *mask = ctx->bnd_mask.as<unsigned long long>();
   unsigned long long *counts = ctx->bnd_counts.as<unsigned long long>();
+  double *lo_b = ctx->bnd_lo.as<double>(), *gap_b = ctx->bnd_gap.as<double>();
   hip_check(bk::launch_bound_pass(p, qx32 ? qx32 + q_off : nullptr, qy32 ? qy32 + q_off : nullptr,
                                   qx64 ? qx64 + q_off : nullptr, qy64 ? qy64 + q_off : nullptr, nq, qpb,
-                                  ctx->tsum.as<double4>(), ntiles, rel_err, abs_u, m_total, thr, counts, s),
+                                  ctx->tsum.as<double4>(), ntiles, rel_err, abs_u, m_total, nwords, mask, lo_b, gap_b,
+                                  counts, s),
             "bound pass launch");
-  // output planes: lower and gap (in Z-curve order when sorted)
-  double *lo_out = lower;
-  if (spatial) {
-    ctx->ord_out.reserve((size_t)nq * sizeof(double));
-    lo_out = ctx->ord_out.as<double>();
-  }
-  ctx->bnd_gap.reserve((size_t)nq * sizeof(double));
-  double *gap_out = ctx->bnd_gap.as<double>();
+  // the summed tiles' exact part
+  ctx->bnd_exact.reserve((size_t)nq * sizeof(double));
+  double *exact = ctx->bnd_exact.as<double>();
   const long long slots =
       path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
   const int splits = choose_splits(nq, nr, qpb, tile_of(path), slots);
-  double *dst = lo_out, *gdst = gap_out;
+  double *dst = exact;
   if (splits > 1) {
-    ctx->partial.reserve((size_t)2 * splits * nq * sizeof(double));
+    ctx->partial.reserve((size_t)splits * nq * sizeof(double));
     dst = ctx->partial.as<double>();
-    gdst = dst + (size_t)splits * nq;
   }
-  bk::BoundArgs ba{};
-  ba.tsum = ctx->tsum.as<double4>();
-  ba.thr = thr;
-  ba.gap = gdst;
-  ba.ms = path == Path::GAUSS32 ? 1.0 / mass_scale(ctx->refs) : 1.0;
-  ba.kx = p.family == bk::FAM_EPANCONV ? p.ix : p.gx;
-  ba.ky = p.family == bk::FAM_EPANCONV ? p.iy : p.gy;
+  bk::BoundArgs ba{mask, nwords};
   hipError_t e;
   if (path == Path::GAUSS32)
     e = bk::launch_gauss(p, qx32, qy32, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
@@ -706,18 +700,17 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
     e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr, q_off,
                               nq, splits, dst, scale, 0, s, &ba);
   hip_check(e, "bounded kernel launch");
-  if (splits > 1) {
-    hip_check(bk::launch_reduce_splits(dst, splits, nq, lo_out, s), "reduce launch");
-    hip_check(bk::launch_reduce_splits(gdst, splits, nq, gap_out, s), "reduce launch");
-  }
+  if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, exact, s), "reduce launch");
   if (spatial) {
+    // bounds in Z-curve order, then back to the caller's order
+    ctx->ord_out.reserve((size_t)nq * sizeof(double));
     ctx->bnd_hi.reserve((size_t)nq * sizeof(double));
-    double *hi_ord = ctx->bnd_hi.as<double>();
-    hip_check(bk::launch_add_planes(lo_out, gap_out, nq, hi_ord, s), "bounds launch");
-    hip_check(bk::launch_scatter_f64(list, nq, lo_out, lower, s), "scatter launch");
+    double *lo_ord = ctx->ord_out.as<double>(), *hi_ord = ctx->bnd_hi.as<double>();
+    hip_check(bk::launch_bounds_combine(exact, lo_b, gap_b, nq, qpb, factor, lo_ord, hi_ord, s), "bounds launch");
+    hip_check(bk::launch_scatter_f64(list, nq, lo_ord, lower, s), "scatter launch");
     hip_check(bk::launch_scatter_f64(list, nq, hi_ord, upper, s), "scatter launch");
   } else {
-    hip_check(bk::launch_add_planes(lo_out, gap_out, nq, upper, s), "bounds launch");
+    hip_check(bk::launch_bounds_combine(exact, lo_b, gap_b, nq, qpb, factor, lower, upper, s), "bounds launch");
   }
   if (stats) {
     unsigned long long c[4] = {0, 0, 0, 0};
@@ -890,7 +883,8 @@ int bk_destroy(bk_ctx *ctx) {
       for (DevBuf *b : {&ctx->f0, &ctx->f1, &ctx->tiles, &ctx->partial, &ctx->out, &ctx->items, &ctx->work_a, &ctx->work_b,
                         &ctx->weights, &ctx->scratch, &ctx->scalar, &ctx->ord_list, &ctx->ord_keys_a,
                         &ctx->ord_keys_b, &ctx->ord_tmp, &ctx->ord_x, &ctx->ord_y, &ctx->ord_out, &ctx->tsum,
-                        &ctx->bnd_thr, &ctx->bnd_counts, &ctx->bnd_gap, &ctx->bnd_hi})
+                        &ctx->bnd_mask, &ctx->bnd_lo, &ctx->bnd_gap, &ctx->bnd_counts, &ctx->bnd_exact,
+                        &ctx->bnd_hi})
         b->release();
       if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }

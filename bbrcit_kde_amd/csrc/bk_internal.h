@@ -77,32 +77,32 @@ enum GaussMode : int {
 // (x0, x1, y0, y1) and (mass sum, smallest abw, largest abw, 0).  For a block
 // of queries and a tile, du / dl bound the kernel U between any query of the
 // block and any reference of the tile (U at the boxes' nearest per-axis
-// distances with the largest abw, at the farthest with the smallest).  The
-// block's tiles with du - dl <= thr are not evaluated: they add mass * dl to
-// the lower bound and mass * (du - dl) to the gap between the bounds.  With
+// distances with the largest abw, at the farthest with the smallest).  A
+// block's tiles with du - dl <= thr are not summed: they add mass * dl to the
+// lower bound and mass * (du - dl) to the gap between the bounds.  With
 // thr = max(rel * L, abs) / (total mass), L = sum over tiles of mass * dl (a
 // lower bound of every query's sum), the gap stays <= max(rel * lower, abs).
 constexpr int kTileSummary = 2;
 struct BoundArgs {
-  const double4 *tsum;  // tile summaries
-  const double *thr;    // per query block
-  double *gap;          // second output plane, laid out like dst
-  double ms;            // accumulator units per unit of mass
-  double kx, ky;        // the family's per-axis constants (gx, gy or ix, iy)
+  const unsigned long long *mask;  // per query block, nwords words: bit t set = tile t is summed
+  long long nwords;
 };
 hipError_t launch_tile_summary_f32(const float *x, const float *y, const float *m, const float *abw, size_t n,
                                    double4 *tsum, hipStream_t s);
 hipError_t launch_tile_summary_f64(const double *x, const double *y, const double *m, const double *abw, size_t n,
                                    double4 *tsum, hipStream_t s);
-// One workgroup per block of qpb queries: thr[b] as above, and counts[0..2]
-// += tiles passed over with du == 0, approximated with du > 0, evaluated.
-// queries are FP32 (qx32, qy32) or FP64 (qx64, qy64; the other pair null).
+// First pass, one workgroup per block of qpb queries: the block's tile mask
+// (nwords = ceil(ntiles / 64) words per block), lo_b[b] / gap_b[b] (mass units)
+// over the tiles not summed, and counts[0..2] += tiles passed over with
+// du == 0, passed over with du > 0, summed.  Queries are FP32 (qx32, qy32) or
+// FP64 (qx64, qy64; the other pair null).
 hipError_t launch_bound_pass(const EvalParams &p, const float *qx32, const float *qy32, const double *qx64,
                              const double *qy64, long long nq, int qpb, const double4 *tsum, long long ntiles,
-                             double rel, double abs_u, double m_total, double *thr, unsigned long long *counts,
-                             hipStream_t s);
-// hi[k] = lo[k] + gap[k] for k < n.
-hipError_t launch_add_planes(const double *lo, const double *gap, long long n, double *hi, hipStream_t s);
+                             double rel, double abs_u, double m_total, long long nwords, unsigned long long *mask,
+                             double *lo_b, double *gap_b, unsigned long long *counts, hipStream_t s);
+// lower[k] = exact[k] + factor * lo_b[k / qpb], upper[k] = lower[k] + factor * gap_b[k / qpb].
+hipError_t launch_bounds_combine(const double *exact, const double *lo_b, const double *gap_b, long long n, int qpb,
+                                 double factor, double *lower, double *upper, hipStream_t s);
 
 // FP32 Gaussian path, D <= 2 ---------------------------------------------
 // Per reference: fd = direct-form features, fg = centred-form features; per
@@ -118,7 +118,8 @@ hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y
 // items == nullptr: range mode, grid (query blocks) x (reference splits),
 // dst holds `splits` consecutive arrays of nq partial sums.
 // items != nullptr: one workgroup per work item.
-// ba != nullptr (range mode only, r_i aligned to kTile): bounded evaluation.
+// ba != nullptr (range mode only, r_i aligned to kTile): only the tiles
+// listed in each block's mask are summed.
 hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
                         const float4 *fd, const float4 *fg, const float4 *tiles,
                         const WorkItem *items, int n_items, long long r_i,

@@ -156,6 +156,21 @@ __device__ __forceinline__ void tile_bounds(const double4 *__restrict__ tsum, lo
   dl = kernel_u_bound<FAM>(fx, fy, w.y, kx, ky);
 }
 
+// Bounded evaluation: first tile at or after `tl` (a multiple of T) whose bit
+// is set in the block's tile mask (bit t of word t / 64: tile t is summed), or
+// a value >= r_end.
+template <int T>
+__device__ __forceinline__ long long next_listed_tile(const unsigned long long *__restrict__ mask, long long tl,
+                                                      long long r_end) {
+  while (tl < r_end) {
+    const long long ti = tl / T;
+    const unsigned long long w = mask[ti >> 6] >> (ti & 63);
+    if (w & 1ULL) break;
+    tl = (w == 0ULL) ? ((ti >> 6) + 1) * 64 * (long long)T : tl + (long long)__builtin_ctzll(w) * T;
+  }
+  return tl;
+}
+
 // ---- FP32 Gaussian ----------------------------------------------------------
 // References are grouped in aligned tiles of kGaussTile points; prep_gauss
 // gives every tile a centre c = (cx, cy), a form and (for the factored
@@ -270,23 +285,21 @@ __device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cn
     gauss_tile_loop<FORM, ISO, false>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
 }
 
-// BOUNDED: tiles with du - dl <= thr_b are passed over and enter the bounds
-// (lower in dst, gap in gap_dst) through their mass; the others are summed.
+// BOUNDED: only the tiles listed in bmask (the block's tile mask) are summed;
+// the bounds of the others are added by the caller.
 template <bool ISO, bool BOUNDED>
 __device__ __forceinline__ void gauss_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_begin, long long r_end,
     long long q_begin, int q_count, double *__restrict__ dst, double out_scale,
-    int self_excl, float rho, double gx, double gy, int mode, const BoundArgs &ba, double thr_b,
-    double *__restrict__ gap_dst) {
+    int self_excl, float rho, double gx, double gy, int mode, const unsigned long long *__restrict__ bmask) {
   constexpr int T = kGaussTile;
-  static_assert(!BOUNDED || kGaussTile == kTile, "tile summaries are per kTile references");
+  static_assert(!BOUNDED || kGaussTile == kTile, "tile masks are per kTile references");
   __shared__ float4 sf[2][T];
   const int t = threadIdx.x;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
-    if constexpr (BOUNDED) write_zeros(gap_dst, q_count);
     return;
   }
   __shared__ float sbox[4][kBlock / 64];
@@ -305,17 +318,11 @@ __device__ __forceinline__ void gauss_item(
 
   float bx0, bx1, by0, by1;
   block_query_box<kGaussQpt>(px, py, sbox, bx0, bx1, by0, by1);
-  double lo_add = 0.0, gap_add = 0.0;  // bounded: passed-over tiles
-  // first tile at or after `tl` that is not culled (or passed over), or r_end
+  // first tile at or after `tl` that is not culled (or listed), or >= r_end
   auto live_from = [&](long long tl) {
+    if constexpr (BOUNDED) return next_listed_tile<T>(bmask, tl, r_end);
     for (; tl < r_end; tl += T) {
-      if constexpr (BOUNDED) {
-        double mt, du, dl;
-        tile_bounds<FAM_GAUSS>(ba.tsum, tl / T, bx0, bx1, by0, by1, gx, gy, mt, du, dl);
-        if (!(du - dl <= thr_b)) break;
-        lo_add += ba.ms * mt * dl;
-        gap_add += ba.ms * mt * (du - dl);
-      } else {
+      {
         const float4 *m = tiles + (tl / T) * kTileMeta;
         const float4 c = m[0], b = m[kTileMeta - 1];
         const double dx =
@@ -430,14 +437,7 @@ __device__ __forceinline__ void gauss_item(
 #pragma unroll
   for (int k = 0; k < kGaussQpt; ++k) {
     const int qi = t + k * kBlock;
-    if (qi < q_count) {
-      if constexpr (BOUNDED) {
-        dst[qi] = (acc[k] + lo_add) * out_scale;
-        gap_dst[qi] = gap_add * out_scale;
-      } else {
-        dst[qi] = acc[k] * out_scale;
-      }
-    }
+    if (qi < q_count) dst[qi] = acc[k] * out_scale;
   }
 }
 
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
   const int q_count = (int)(left < kGaussQueriesPerBlock ? left : kGaussQueriesPerBlock);
   const long long off = (long long)blockIdx.y * nq + qb;
   gauss_item<ISO, BOUNDED>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
-                           gx, gy, mode, ba, BOUNDED ? ba.thr[blockIdx.x] : 0.0, BOUNDED ? ba.gap + off : nullptr);
+                           gx, gy, mode, BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
 }
 
 template <bool ISO>
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void gauss_list_kernel(
     double gx, double gy, int mode) {
   const WorkItem it = items[blockIdx.x];
   gauss_item<ISO, false>(qx, qy, fd, fg, tiles, it.r_begin, it.r_end, it.q_begin, it.q_count,
-                         dst + it.out_index, out_scale, self_excl, rho, gx, gy, mode, BoundArgs{}, 0.0, nullptr);
+                         dst + it.out_index, out_scale, self_excl, rho, gx, gy, mode, nullptr);
 }
 
 // Block-wide min / max over N threads (one value per thread).
@@ -641,8 +641,7 @@ __device__ __forceinline__ void epan_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_begin, long long r_end, long long q_begin, int q_count,
-    double *__restrict__ dst, double out_scale, int self_excl, const BoundArgs &ba, double thr_b,
-    double *__restrict__ gap_dst) {
+    double *__restrict__ dst, double out_scale, int self_excl, const unsigned long long *__restrict__ bmask) {
   constexpr int kNf = EpanNf<FAM>::kNf;
   __shared__ double4 s0[2][kTile];
   __shared__ double4 s1[2][kNf == 2 ? kTile : 1];
@@ -650,7 +649,6 @@ __device__ __forceinline__ void epan_item(
   const int t = threadIdx.x;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
-    if constexpr (BOUNDED) write_zeros(gap_dst, q_count);
     return;
   }
   double px[kQpt], py[kQpt], acc[kQpt];
@@ -664,19 +662,10 @@ __device__ __forceinline__ void epan_item(
   }
   float bx0, bx1, by0, by1;
   block_query_box<kQpt>(px, py, sbox, bx0, bx1, by0, by1);
-  double lo_add = 0.0, gap_add = 0.0;  // bounded: passed-over tiles
   auto live_from = [&](long long tl) {
-    for (; tl < r_end; tl += kTile) {
-      if constexpr (BOUNDED) {
-        double mt, du, dl;
-        tile_bounds<FAM>(ba.tsum, tl / kTile, bx0, bx1, by0, by1, ba.kx, ba.ky, mt, du, dl);
-        if (!(du - dl <= thr_b)) break;
-        lo_add += ba.ms * mt * dl;
-        gap_add += ba.ms * mt * (du - dl);
-      } else {
-        if (!epan_tile_far<FAM>(tb, tl / kTile, bx0, bx1, by0, by1)) break;
-      }
-    }
+    if constexpr (BOUNDED) return next_listed_tile<kTile>(bmask, tl, r_end);
+    for (; tl < r_end; tl += kTile)
+      if (!epan_tile_far<FAM>(tb, tl / kTile, bx0, bx1, by0, by1)) break;
     return tl;
   };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
@@ -733,14 +722,7 @@ __device__ __forceinline__ void epan_item(
 #pragma unroll
   for (int k = 0; k < kQpt; ++k) {
     const int qi = t + k * kBlock;
-    if (qi < q_count) {
-      if constexpr (BOUNDED) {
-        dst[qi] = (acc[k] + lo_add) * out_scale;
-        gap_dst[qi] = gap_add * out_scale;
-      } else {
-        dst[qi] = acc[k] * out_scale;
-      }
-    }
+    if (qi < q_count) dst[qi] = acc[k] * out_scale;
   }
 }
 
@@ -756,8 +738,8 @@ __global__ __launch_bounds__(kBlock) void epan_range_kernel(
   const long long left = nq - qb;
   const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
   const long long off = (long long)blockIdx.y * nq + qb;
-  epan_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, ba,
-                          BOUNDED ? ba.thr[blockIdx.x] : 0.0, BOUNDED ? ba.gap + off : nullptr);
+  epan_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
+                          BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
 }
 
 template <int FAM>
@@ -768,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void epan_list_kernel(
     double out_scale, int self_excl) {
   const WorkItem it = items[blockIdx.x];
   epan_item<FAM, false>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
-                        out_scale, self_excl, BoundArgs{}, 0.0, nullptr);
+                        out_scale, self_excl, nullptr);
 }
 
 __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
@@ -844,36 +826,13 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   }
 }
 
-// Bounding box of the workgroup's FP64 queries (one per thread).
-__device__ __forceinline__ void block_query_box_d(double px, double py, double (&sbox)[4][kBlock / 64], double &bx0,
-                                                  double &bx1, double &by0, double &by1) {
-  const int t = threadIdx.x;
-  double v[4] = {px, -px, py, -py};  // min of each (max as min of the negation)
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    for (int o = 32; o > 0; o >>= 1) v[i] = fmin(v[i], __shfl_xor(v[i], o, 64));
-    if ((t & 63) == 0) sbox[i][t >> 6] = v[i];
-  }
-  __syncthreads();
-  double r[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    r[i] = sbox[i][0];
-    for (int w = 1; w < kBlock / 64; ++w) r[i] = fmin(r[i], sbox[i][w]);
-  }
-  bx0 = r[0];
-  bx1 = -r[1];
-  by0 = r[2];
-  by1 = -r[3];
-}
-
 template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void direct_f64_item(
     const double *__restrict__ qx, const double *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1,
     long long r_begin, long long r_end, long long q_begin, int q_count,
-    double *__restrict__ dst, double out_scale, int self_excl, double rho, const BoundArgs &ba, double thr_b,
-    double *__restrict__ gap_dst) {
+    double *__restrict__ dst, double out_scale, int self_excl, double rho,
+    const unsigned long long *__restrict__ bmask) {
   constexpr int kNf = (FAM == FAM_EPANPROD || FAM == FAM_EPANCONV) ? 2 : 1;
   static_assert(kBlock64 == kBlock && kTile64 == kTile, "one query per thread, tile summaries per kTile");
   __shared__ double4 s0[kTile64];
@@ -883,20 +842,8 @@ __device__ __forceinline__ void direct_f64_item(
   const long long q = q_begin + (valid ? t : 0);
   const double px = qx[q], py = qy[q];
   double sum = 0.0;
-  double lo_add = 0.0, gap_add = 0.0;
-  __shared__ double sbox[4][kBlock / 64];
-  double bx0 = 0.0, bx1 = 0.0, by0 = 0.0, by1 = 0.0;
-  if constexpr (BOUNDED) block_query_box_d(px, py, sbox, bx0, bx1, by0, by1);
   auto live_from = [&](long long tl) {
-    if constexpr (BOUNDED) {
-      for (; tl < r_end; tl += kTile64) {
-        double mt, du, dl;
-        tile_bounds<FAM>(ba.tsum, tl / kTile64, bx0, bx1, by0, by1, ba.kx, ba.ky, mt, du, dl);
-        if (!(du - dl <= thr_b)) break;
-        lo_add += ba.ms * mt * dl;
-        gap_add += ba.ms * mt * (du - dl);
-      }
-    }
+    if constexpr (BOUNDED) return next_listed_tile<kTile64>(bmask, tl, r_end);
     return tl;
   };
   for (long long tile = live_from(r_begin); tile < r_end; tile = live_from(tile + kTile64)) {
@@ -915,14 +862,7 @@ __device__ __forceinline__ void direct_f64_item(
     }
     __syncthreads();
   }
-  if (valid) {
-    if constexpr (BOUNDED) {
-      dst[t] = (sum + lo_add) * out_scale;
-      gap_dst[t] = gap_add * out_scale;
-    } else {
-      dst[t] = sum * out_scale;
-    }
-  }
+  if (valid) dst[t] = sum * out_scale;
 }
 
 template <int FAM, bool BOUNDED>
@@ -940,7 +880,7 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
   const int q_count = (int)(left < kBlock64 ? left : kBlock64);
   const long long off = (long long)blockIdx.y * nq + qb;
   direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
-                                ba, BOUNDED ? ba.thr[blockIdx.x] : 0.0, BOUNDED ? ba.gap + off : nullptr);
+                                BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
 }
 
 template <int FAM>
@@ -951,7 +891,7 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_list_kernel(
     double out_scale, int self_excl, double rho) {
   const WorkItem it = items[blockIdx.x];
   direct_f64_item<FAM, false>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
-                              out_scale, self_excl, rho, BoundArgs{}, 0.0, nullptr);
+                              out_scale, self_excl, rho, nullptr);
 }
 
 __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
@@ -1379,12 +1319,17 @@ __global__ __launch_bounds__(kTile) void tile_summary_kernel(const T *__restrict
   }
 }
 
-// One workgroup of kTile threads per block of qpb queries.
+// One workgroup of kTile threads per block of qpb queries: the block's box,
+// L = sum_t M_t dl, thr = max(rel L, abs_u) / m_total; then per tile the
+// decision (summed: bit set in the block's mask; otherwise its mass times dl
+// and times du - dl enter lo_b and gap_b), and the pruning counts.
 template <int FAM, typename T>
 __global__ __launch_bounds__(kTile) void bound_pass_kernel(const T *__restrict__ qx, const T *__restrict__ qy,
                                                            long long nq, int qpb, const double4 *__restrict__ tsum,
                                                            long long ntiles, double kx, double ky, double rel,
-                                                           double abs_u, double m_total, double *__restrict__ thr,
+                                                           double abs_u, double m_total, long long nwords,
+                                                           unsigned long long *__restrict__ mask,
+                                                           double *__restrict__ lo_b, double *__restrict__ gap_b,
                                                            unsigned long long *__restrict__ counts) {
   __shared__ double sh[kTile];
   const long long q0 = (long long)blockIdx.x * qpb;
@@ -1408,34 +1353,55 @@ __global__ __launch_bounds__(kTile) void bound_pass_kernel(const T *__restrict__
   }
   L = block_reduce_d<true>(L, sh);
   const double th = fmax(rel * L, abs_u) / m_total;
-  double c_ex = 0.0, c_ap = 0.0, c_ev = 0.0;
-  for (long long ti = threadIdx.x; ti < ntiles; ti += kTile) {
-    double mt, du, dl;
-    tile_bounds<FAM>(tsum, ti, bx0, bx1, by0, by1, kx, ky, mt, du, dl);
-    if (du - dl <= th) {
-      if (du == 0.0)
-        c_ex += 1.0;
-      else
-        c_ap += 1.0;
-    } else {
-      c_ev += 1.0;
+  double lo = 0.0, gp = 0.0, c_ex = 0.0, c_ap = 0.0, c_ev = 0.0;
+  unsigned long long *bm = mask + (long long)blockIdx.x * nwords;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (long long base = 0; base < ntiles; base += kTile) {
+    const long long ti = base + threadIdx.x;
+    bool summed = false;
+    if (ti < ntiles) {
+      double mt, du, dl;
+      tile_bounds<FAM>(tsum, ti, bx0, bx1, by0, by1, kx, ky, mt, du, dl);
+      if (du - dl <= th) {
+        lo += mt * dl;
+        gp += mt * (du - dl);
+        if (du == 0.0)
+          c_ex += 1.0;
+        else
+          c_ap += 1.0;
+      } else {
+        summed = true;
+        c_ev += 1.0;
+      }
     }
+    const unsigned long long bits = __ballot(summed);
+    const long long word = (base >> 6) + wave;
+    if (lane == 0 && word < nwords) bm[word] = bits;
   }
+  lo = block_reduce_d<true>(lo, sh);
+  gp = block_reduce_d<true>(gp, sh);
   c_ex = block_reduce_d<true>(c_ex, sh);
   c_ap = block_reduce_d<true>(c_ap, sh);
   c_ev = block_reduce_d<true>(c_ev, sh);
   if (threadIdx.x == 0) {
-    thr[blockIdx.x] = th;
+    lo_b[blockIdx.x] = lo;
+    gap_b[blockIdx.x] = gp;
     atomicAdd(counts + 0, (unsigned long long)c_ex);
     atomicAdd(counts + 1, (unsigned long long)c_ap);
     atomicAdd(counts + 2, (unsigned long long)c_ev);
   }
 }
 
-__global__ void add_planes_kernel(const double *__restrict__ lo, const double *__restrict__ gap, long long n,
-                                  double *__restrict__ hi) {
+// lower[k] = exact[k] + factor lo_b[k / qpb], upper[k] = lower[k] + factor gap_b[k / qpb].
+__global__ void bounds_combine_kernel(const double *__restrict__ exact, const double *__restrict__ lo_b,
+                                      const double *__restrict__ gap_b, long long n, int qpb, double factor,
+                                      double *__restrict__ lower, double *__restrict__ upper) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) hi[k] = lo[k] + gap[k];
+  if (k >= n) return;
+  const long long b = k / qpb;
+  const double lo = exact[k] + factor * lo_b[b];
+  lower[k] = lo;
+  upper[k] = lo + factor * gap_b[b];
 }
 
 }  // namespace
@@ -1459,8 +1425,8 @@ hipError_t launch_tile_summary_f64(const double *x, const double *y, const doubl
 
 hipError_t launch_bound_pass(const EvalParams &p, const float *qx32, const float *qy32, const double *qx64,
                              const double *qy64, long long nq, int qpb, const double4 *tsum, long long ntiles,
-                             double rel, double abs_u, double m_total, double *thr, unsigned long long *counts,
-                             hipStream_t s) {
+                             double rel, double abs_u, double m_total, long long nwords, unsigned long long *mask,
+                             double *lo_b, double *gap_b, unsigned long long *counts, hipStream_t s) {
   if (nq <= 0) return hipSuccess;
   const unsigned grid = (unsigned)((nq + qpb - 1) / qpb);
   const bool conv = p.family == FAM_EPANCONV;
@@ -1468,10 +1434,10 @@ hipError_t launch_bound_pass(const EvalParams &p, const float *qx32, const float
 #define BK_BP(F)                                                                                                  \
   if (qx64)                                                                                                       \
     bound_pass_kernel<F, double><<<grid, kTile, 0, s>>>(qx64, qy64, nq, qpb, tsum, ntiles, kx, ky, rel, abs_u,    \
-                                                        m_total, thr, counts);                                    \
+                                                        m_total, nwords, mask, lo_b, gap_b, counts);              \
   else                                                                                                            \
     bound_pass_kernel<F, float><<<grid, kTile, 0, s>>>(qx32, qy32, nq, qpb, tsum, ntiles, kx, ky, rel, abs_u,     \
-                                                       m_total, thr, counts)
+                                                       m_total, nwords, mask, lo_b, gap_b, counts)
   switch (p.family) {
     case FAM_GAUSS: BK_BP(FAM_GAUSS); break;
     case FAM_EPAN: BK_BP(FAM_EPAN); break;
@@ -1482,9 +1448,10 @@ hipError_t launch_bound_pass(const EvalParams &p, const float *qx32, const float
   return hipGetLastError();
 }
 
-hipError_t launch_add_planes(const double *lo, const double *gap, long long n, double *hi, hipStream_t s) {
+hipError_t launch_bounds_combine(const double *exact, const double *lo_b, const double *gap_b, long long n, int qpb,
+                                 double factor, double *lower, double *upper, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  add_planes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(lo, gap, n, hi);
+  bounds_combine_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(exact, lo_b, gap_b, n, qpb, factor, lower, upper);
   return hipGetLastError();
 }
 
