@@ -89,11 +89,29 @@ def build_cpp_driver(verbose: bool = False) -> Path:
     return CPP_DRIVER_LIB
 
 
+GRID_SEARCH_SRC = PKG_DIR / "grid_search" / "grid_search.cc"
+GRID_SEARCH_BIN = PKG_DIR / "grid_search" / "grid_search"
+
+
+def build_grid_search(verbose: bool = False) -> Path:
+    """Compile the grid_search tool (host C++ over the C ABI) with g++."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = GRID_SEARCH_BIN.with_suffix(".tmp")
+    cmd = [gxx, "-std=c++14", "-O2", "-Wall", "-I", str(PKG_DIR / "include"), "-I", str(REPO_DIR / "include"),
+           str(GRID_SEARCH_SRC), "-L", str(LIB_DIR), "-lbbrcit_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(tmp, GRID_SEARCH_BIN)
+    return GRID_SEARCH_BIN
+
+
 def main() -> int:
     path = build_library(verbose=True)
     print(f"built {path}")
     if CPP_DRIVER_SRC.exists():
         print(f"built {build_cpp_driver(verbose=True)}")
+    print(f"built {build_grid_search(verbose=True)}")
     return 0
 
 

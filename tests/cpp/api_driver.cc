@@ -11,6 +11,7 @@
 #include <HipDirectKde.h>
 #include <KernelDensity.h>
 #include <Kernels/ConvKernelAssociator.h>
+#include <MarginalDensityAssociator.h>
 
 #include <exception>
 #include <iostream>
@@ -204,6 +205,36 @@ int api_direct_eval(int kind, int dim, int kf, double hx, double hy, const doubl
       for (size_t i = 0; i < nq; ++i) out[i] = q[i].attributes().value();
       return 0;
     });
+  });
+}
+
+// MarginalDensityAssociator: the dim_idx marginal of a 2-D product-kernel
+// density (kind 3 or 4), adapted first with alpha when alpha > 0, evaluated
+// at the 1-D queries q1; the 2-D density itself at the 2-D queries q2.
+int api_marginal_eval(int kind, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
+                      int leaf_nmax, double alpha, int dim_idx, const double *q1, size_t n1, double *out1,
+                      const double *q2, size_t n2, double *out2) {
+  return guarded([&] {
+    auto run = [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      using KDE = KernelDensity<2, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      if (alpha > 0) kde.adapt_density(alpha);
+      using MDA = MarginalDensityAssociator<KDE>;
+      auto marginal = dim_idx == 0 ? MDA::marginal_density_x(kde) : MDA::marginal_density_y(kde);
+      using M = decltype(marginal);
+      auto q = make_points<M>(q1, nullptr, n1);
+      marginal.direct_eval(q);
+      for (size_t i = 0; i < n1; ++i) out1[i] = q[i].attributes().value();
+      auto r = make_points<KDE>(q2, nullptr, n2);
+      kde.direct_eval(r);
+      for (size_t i = 0; i < n2; ++i) out2[i] = r[i].attributes().value();
+      return 0;
+    };
+    if (kind == 3) return kf ? run(Tag<2, EpanechnikovProductKernel2d<float>>()) : run(Tag<2, EpanechnikovProductKernel2d<double>>());
+    if (kind == 4) return kf ? run(Tag<2, GaussianProductKernel2d<float>>()) : run(Tag<2, GaussianProductKernel2d<double>>());
+    return -1;
   });
 }
 

@@ -40,6 +40,11 @@ def inputs(seed=2024, n=600, nq=200):
     return pts, w, q
 
 
+def grid_search_case():
+    """Bandwidth grid ends of the grid_search fixture."""
+    return {"bw_x0": 0.2, "bw_x1": 0.6, "bw_y0": 0.25, "bw_y1": 0.7}
+
+
 def main() -> int:
     ref = Reference()
     pts, w, q = inputs()
@@ -78,6 +83,28 @@ def main() -> int:
     ni["numint_gaussprod"] = ref.lsq_numint_cv(4, 0, 0.3, 0.45, pts, w, -4.0, 4.0, 100, -4.0, 4.0, 100,
                                                rel=1e-12, abs_err=1e-15)
     np.savez_compressed(OUT / "numint_ref.npz", **ni)
+    # the steps of grid_search/grid_search.cc with the reference's own
+    # functions: lsq_conv scores over an automatic 4 x 4 bandwidth grid
+    # (EpanechnikovProductKernel2d<float>, double tree, rel 1e-6, abs 1e-8,
+    # leaves of 32), the density at the best pair over a 21 x 21 plotting
+    # grid (dual tree), and the adaptive density (alpha 0.5) there: the
+    # reference's adapted masses and abw, summed exactly by the oracle
+    from oracle.oracle import Oracle
+
+    gs = grid_search_case()
+    grid_bw = [(gs["bw_x0"] + i * (gs["bw_x1"] - gs["bw_x0"]) / 3, gs["bw_y0"] + j * (gs["bw_y1"] - gs["bw_y0"]) / 3)
+               for j in range(4) for i in range(4)]
+    scores = np.array([ref.lsq_conv_cv(3, 1, hx, hy, pts, w, rel=1e-6, abs_err=1e-8, leaf_nmax=32)
+                       for hx, hy in grid_bw])
+    bx, by = grid_bw[int(np.argmin(scores))]
+    g = np.linspace(-2.5, 2.5, 21)
+    qgrid = np.array([(x, y) for y in g for x in g])
+    qo, lo, up = ref.tree_eval(3, 1, bx, by, pts, w, qgrid, rel=1e-6, abs_err=1e-8, leaf_nmax=32, q_leaf_nmax=32)
+    order = np.lexsort((qo[:, 0], qo[:, 1]))
+    c, _, a, m = ref.adapt(3, 1, bx, by, pts, w, 0.5, rel=1e-6, abs_err=1e-8, leaf_nmax=32)
+    adaptive = Oracle().direct(3, bx, by, c, qgrid, mass=m, abw=a, normalize=True)
+    np.savez_compressed(OUT / "grid_search_ref.npz", pts=pts, w=w, bw=np.array(grid_bw), lscv=scores,
+                        best=np.array([bx, by]), qgrid=qgrid, kde=0.5 * (lo + up)[order], akde=adaptive)
     print("fixtures written to", OUT)
     return 0
 

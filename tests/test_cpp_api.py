@@ -52,6 +52,7 @@ def api():
     lib.api_hip_direct_range.argtypes = [_I, _I, _I, _D, _D, _DP, _SZ, _DP, _SZ, _SZ, _SZ, _SZ, _SZ, _I, _DP]
     lib.api_cv_grid.argtypes = [_I, _DP, _DP, _SZ, _I, _DP, _DP, _SZ, _I, _D, _DP, _DP, _DP, _DP]
     lib.api_grid_layouts.argtypes = [_DP, _SZ, _DP, _SZ, _D, _D, _I, _D, _D, _I, _DP, _DP]
+    lib.api_marginal_eval.argtypes = [_I, _I, _D, _D, _DP, _DP, _SZ, _I, _D, _I, _DP, _SZ, _DP, _DP, _SZ, _DP]
     return lib
 
 
@@ -269,3 +270,34 @@ def test_grid_search_matches_per_bandwidth_calls(api, kind, adapt):
     if not adapt:
         assert lcv_g[0] == pytest.approx(float(f[f"lcv_{case}"]), rel=1e-5, abs=1e-6)
         assert lscv_g[0] == pytest.approx(float(f[f"lscv_{case}"]), rel=1e-5, abs=1e-7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,hx,hy,kf", [(GAUSSPROD, 0.3, 0.45, 0), (EPANPROD, 0.4, 0.55, 0),
+                                           (GAUSSPROD, 0.3, 0.45, 1)])
+@pytest.mark.parametrize("alpha", [0.0, 0.5])
+@pytest.mark.parametrize("dim_idx", [0, 1])
+def test_marginal_density_is_the_integral_of_the_2d_density(api, oracle, kind, hx, hy, kf, alpha, dim_idx):
+    # MarginalDensityAssociator (include/MarginalDensityAssociator.h:13-128):
+    # the marginal along one axis equals the 2-D density integrated over the
+    # other axis (fine trapezoid rule on the GPU's own 2-D values), and without
+    # adaptation it is the 1-D density of that coordinate (oracle)
+    f = np.load(GOLDEN / "tree_ref.npz")
+    pts, w = f["pts"], f["w"]
+    t = np.linspace(-2.5, 2.5, 21)
+    other = np.linspace(-25.0, 25.0, 25001)  # adapted bandwidth factors reach about 5
+    q2 = np.array([(a, b) if dim_idx == 0 else (b, a) for a in t for b in other])
+    m1, d2 = np.empty(len(t)), np.empty(len(q2))
+    assert api.api_marginal_eval(kind, kf, hx, hy, _p(pts), _p(w), len(pts), 16, alpha, dim_idx, _p(t), len(t),
+                                 _p(m1), _p(q2), len(q2), _p(d2)) == 0, _err(api)
+    integral = np.trapezoid(d2.reshape(len(t), len(other)), other, axis=1)
+    tol = 2e-5 if kf else 1e-9
+    rtol = (1e-4 if kind == EPANPROD else 1e-7) + tol  # trapezoid error at the Epanechnikov support kinks
+    np.testing.assert_allclose(m1, integral, rtol=rtol, atol=1e-9)
+    if alpha == 0.0:
+        h = hx if dim_idx == 0 else hy
+        kind1 = GAUSS if kind == GAUSSPROD else EPAN
+        x1 = pts[:, dim_idx:dim_idx + 1]
+        want = oracle.direct(kind1, h, h, ds.round_f32(x1) if kf else x1, t[:, None], mass=w / w.sum(),
+                             normalize=True)
+        np.testing.assert_allclose(m1, want, rtol=tol, atol=1e-12)
