@@ -300,4 +300,8 @@ def test_marginal_density_is_the_integral_of_the_2d_density(api, oracle, kind, h
         x1 = pts[:, dim_idx:dim_idx + 1]
         want = oracle.direct(kind1, h, h, ds.round_f32(x1) if kf else x1, t[:, None], mass=w / w.sum(),
                              normalize=True)
+        if kind1 == GAUSS and not kf:
+            # GaussianTraits<1,double>::normalization takes h as float
+            # (include/Kernels/KernelTraits.h:143), kept as the reference has it
+            want = want * (h / float(np.float32(h)))
         np.testing.assert_allclose(m1, want, rtol=tol, atol=1e-12)
