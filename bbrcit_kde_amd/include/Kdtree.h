@@ -12,14 +12,23 @@
 //      the points' bounding box, grown from the reference's initial values:
 //      lowest edge numeric_limits::max(), highest edge numeric_limits::min())
 //      and the merge of its points' attributes.
+// Large sets (kParallelMin points and more) are built on several host
+// threads without changing the result: the sort runs as sorted chunks merged
+// pairwise (the same order, since ExactLexicoLess is a strict order on
+// distinct points; only the order among exact duplicates, which are merged
+// next, may differ), and sibling subtrees, whose ranges are disjoint, are
+// partitioned concurrently with the same std::nth_element calls.
 #ifndef BBRCIT_AMD_KDTREE_H_
 #define BBRCIT_AMD_KDTREE_H_
 
 #include <algorithm>
+#include <future>
 #include <iostream>
+#include <iterator>
 #include <limits>
 #include <memory>
 #include <stack>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -144,14 +153,65 @@ class Kdtree {
   }
 
  private:
+  static constexpr size_t kParallelMin = size_t(1) << 18;
+
+  static int build_threads() {
+    const unsigned t = std::thread::hardware_concurrency();
+    return t == 0 ? 1 : (int)std::min(t, 16u);
+  }
+
   void build() {
     merge_duplicates();
-    if (!points_.empty()) root_ = make_node(0, points_.size() - 1, 0, bounding_box());
+    if (points_.empty()) return;
+    int spawn_depth = 0;
+    if (points_.size() >= kParallelMin)
+      while ((1 << spawn_depth) < build_threads()) ++spawn_depth;
+    root_ = make_node(0, points_.size() - 1, 0, bounding_box(), spawn_depth);
+  }
+
+  // chunks sorted on their own threads, then merged pairwise
+  void sort_points() {
+    const int t = build_threads();
+    if (points_.size() < kParallelMin || t < 2) {
+      std::sort(points_.begin(), points_.end(), ExactLexicoLess<DataPointType>);
+      return;
+    }
+    const size_t n = points_.size();
+    std::vector<size_t> bounds;
+    for (int c = 0; c <= t; ++c) bounds.push_back(n * (size_t)c / (size_t)t);
+    {
+      std::vector<std::future<void>> jobs;
+      for (int c = 0; c < t; ++c)
+        jobs.push_back(std::async(std::launch::async, [this, &bounds, c] {
+          std::sort(points_.begin() + bounds[c], points_.begin() + bounds[c + 1], ExactLexicoLess<DataPointType>);
+        }));
+      for (auto &j : jobs) j.get();
+    }
+    std::vector<DataPointType> buf(n);
+    std::vector<DataPointType> *src = &points_, *dst = &buf;
+    while (bounds.size() > 2) {
+      std::vector<size_t> next;
+      std::vector<std::future<void>> jobs;
+      for (size_t c = 0; c + 1 < bounds.size(); c += 2) {
+        const size_t a = bounds[c], b = bounds[c + 1], e = (c + 2 < bounds.size()) ? bounds[c + 2] : b;
+        next.push_back(a);
+        jobs.push_back(std::async(std::launch::async, [src, dst, a, b, e] {
+          std::merge(std::make_move_iterator(src->begin() + a), std::make_move_iterator(src->begin() + b),
+                     std::make_move_iterator(src->begin() + b), std::make_move_iterator(src->begin() + e),
+                     dst->begin() + a, ExactLexicoLess<DataPointType>);
+        }));
+      }
+      next.push_back(bounds.back());
+      for (auto &j : jobs) j.get();
+      bounds.swap(next);
+      std::swap(src, dst);
+    }
+    if (src != &points_) points_.swap(buf);
   }
 
   void merge_duplicates() {
     if (points_.empty()) return;
-    std::sort(points_.begin(), points_.end(), ExactLexicoLess<DataPointType>);
+    sort_points();
     size_t last = 0;
     for (size_t k = 1; k < points_.size(); ++k) {
       if (ExactEqual(points_[last], points_[k])) {
@@ -184,7 +244,8 @@ class Kdtree {
     return a;
   }
 
-  Node *make_node(IndexType i, IndexType j, int d, const RectangleType &cell) {
+  // spawn_depth > 0: the left subtree is built on a thread of its own
+  Node *make_node(IndexType i, IndexType j, int d, const RectangleType &cell, int spawn_depth = 0) {
     Node *n = new Node();
     n->start_idx_ = i;
     n->end_idx_ = j;
@@ -197,8 +258,17 @@ class Kdtree {
     std::nth_element(points_.begin() + i, points_.begin() + m, points_.begin() + j + 1,
                      [d](const DataPointType &a, const DataPointType &b) { return a[d] < b[d]; });
     n->split_ = points_[m][d];
-    n->left_ = make_node(i, m, (d + 1) % D, cell.lower_halfspace(d, n->split_));
-    n->right_ = make_node(m + 1, j, (d + 1) % D, cell.upper_halfspace(d, n->split_));
+    const RectangleType lo_cell = cell.lower_halfspace(d, n->split_), hi_cell = cell.upper_halfspace(d, n->split_);
+    if (spawn_depth > 0) {
+      auto left = std::async(std::launch::async, [this, i, m, d, &lo_cell, spawn_depth] {
+        return make_node(i, m, (d + 1) % D, lo_cell, spawn_depth - 1);
+      });
+      n->right_ = make_node(m + 1, j, (d + 1) % D, hi_cell, spawn_depth - 1);
+      n->left_ = left.get();
+    } else {
+      n->left_ = make_node(i, m, (d + 1) % D, lo_cell);
+      n->right_ = make_node(m + 1, j, (d + 1) % D, hi_cell);
+    }
     n->attr_ = merge(n->left_->attr_, n->right_->attr_);
     return n;
   }

@@ -490,6 +490,34 @@ def bench_dual_tree(bk, steps: int, warmup: int, cpu: bool):
                                 "pruning": st, "pairs_summed": summed, "pairs_all": float(n) * n,
                                 "summed_pair_evals_per_s": summed / (avg * 1e-3)}
         del lo, up
+    # end to end through the C++ API, as test_kde11.cc:87-116 calls it:
+    # KernelDensity<2, EpanechnikovKernel<2,float>, double>(data, 4096) and
+    # eval(queries, 1e-6, 1e-10, 4096): host kd-tree builds, uploads, the
+    # device evaluation, results back in the query tree's order
+    import ctypes
+
+    drv = ROOT / "tests" / "cpp" / "libapi_driver.so"
+    if drv.exists():
+        lib = ctypes.CDLL(str(drv))
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.api_tree_eval.restype = ctypes.c_long
+        lib.api_tree_eval.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp,
+                                      dp, ctypes.c_size_t, ctypes.c_int, dp, ctypes.c_size_t, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, dp, dp, dp]
+        n = DT_SIZES[-1]
+        pts = np.ascontiguousarray(ds.standard_normal(n, 2, 1011))
+        w = np.ones(n)
+        qo, lo_h, up_h = np.empty((n, 2)), np.empty(n), np.empty(n)
+        ptr = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+        t0 = time.perf_counter()
+        k = lib.api_tree_eval(bk.BK_EPANECHNIKOV, 2, 1, DT_H, DT_H, ptr(pts), ptr(w), n, 4096, ptr(pts), n, DT_REL,
+                              DT_ABS, 4096, ptr(qo), ptr(lo_h), ptr(up_h))
+        dt = time.perf_counter() - t0
+        if k == n:
+            res["end_to_end_cpp_api"] = {"n": n, "ms": dt * 1e3, "points_per_s": n / dt,
+                                         "vs_reference_published_gpu": 37719.9 / (dt * 1e3),
+                                         "vs_reference_published_cpu": 489527.0 / (dt * 1e3),
+                                         "host_threads": os.cpu_count()}
     big = res["sizes"][str(DT_SIZES[-1])]
     res["value"] = big["points_per_s"]
     res["ms_per_step"] = big["ms"]

@@ -77,6 +77,24 @@ def test_kdtree_order_matches_reference(api, reference):
         np.testing.assert_allclose(m[:k], m_ref, rtol=1e-15)
 
 
+def test_kdtree_parallel_build_matches_reference(api, reference):
+    # 300,000 points take the multi-threaded build (sorted chunks merged,
+    # sibling subtrees partitioned concurrently): same order as the
+    # reference's single-threaded Kdtree
+    rng = np.random.default_rng(4)
+    n = 300_000
+    pts = rng.normal(size=(n, 2))
+    w = rng.uniform(0.5, 1.5, n)
+    for leaf in (32, 4096):
+        c_ref, w_ref, m_ref, _ = reference.tree_points(EPAN, 1, 0.1, 0.1, pts, w, leaf_nmax=leaf)
+        c, ww, m = np.empty((n, 2)), np.empty(n), np.empty(n)
+        k = api.api_tree_points(EPAN, 2, 1, 0.1, 0.1, _p(pts), _p(w), n, leaf, _p(c), _p(ww), _p(m))
+        assert k == len(c_ref) == n, _err(api)
+        np.testing.assert_array_equal(c, c_ref)
+        np.testing.assert_array_equal(ww, w_ref)
+        np.testing.assert_array_equal(m, m_ref)
+
+
 @pytest.mark.parametrize("kind,dim", [(EPAN, 1), (EPAN, 2), (GAUSS, 1), (GAUSS, 2), (EPANPROD, 2), (GAUSSPROD, 2),
                                       (EPANCONV1, 1), (EPANPRODCONV, 2), (GAUSSCONV1, 1), (GAUSSPRODCONV, 2)])
 @pytest.mark.parametrize("kf", [0, 1])

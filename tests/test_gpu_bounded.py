@@ -140,3 +140,27 @@ def test_bounded_argument_errors(bk):
             kde.eval_bounded(desc, 0, 100, 1e-3, 0.0)
         with pytest.raises(ValueError):
             kde.eval_bounded(desc, 50, 10, 1e-3, 0.0)
+
+
+def test_bounded_full_size_test_kde11(bk, oracle):
+    # The reference's dual-tree benchmark at N = 1,048,576 (test_kde11.cc:
+    # standard-normal points evaluated at themselves, EpanechnikovKernel<2,
+    # float> h = 0.1, rel 1e-6, abs 1e-10): on 3 x 512 sampled points the
+    # bounds bracket the oracle's exact sum and meet the tolerance; the
+    # pruning passes over most (block, tile) pairs as exactly zero.
+    n = 1 << 20
+    pts = ds.standard_normal(n, 2, 1011)
+    pts = pts[ds.kd_order(pts, 64)]
+    mass = np.full(n, 1.0 / n)
+    h = float(np.float32(0.1))
+    desc = bk.KernelDesc(EPAN, 2, h)
+    with bk.DirectKde(pts, None, mass) as kde:
+        lo, up, st = kde.eval_bounded(desc, 0, n - 1, 1e-6, 1e-10)
+    assert st["excluded"] > 0.9 * (st["excluded"] + st["approximated"] + st["evaluated"])
+    for i0 in (0, n // 2, n - 512):
+        sl = slice(i0, i0 + 512)
+        exact = oracle.direct(EPAN, h, h, f32(pts), f32(pts[sl]), mass=f32(mass), normalize=True)
+        check_bounds(lo[sl], up[sl], exact, oracle.normalization(EPAN, 2, h, h), 1e-6, 1e-10, False,
+                     f"test_kde11 sample at {i0}")
+    gap = up - lo
+    assert np.all(gap <= np.maximum(1e-10, 1e-6 * lo) * (1 + 1e-5))

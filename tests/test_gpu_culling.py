@@ -104,3 +104,25 @@ def test_query_order_option_is_checked(bk):
     with bk.DirectKde(pts, pts, mass) as kde:
         with pytest.raises(ValueError):
             kde.set_option(bk.BK_OPT_QUERY_ORDER, 7)
+
+
+def test_spatial_order_full_size_is_bit_identical_for_epanechnikov(bk):
+    # The headline workload (1M kd-ordered mixture references x 1M uniform
+    # queries), EpanechnikovProductKernel2d (0.08, 0.12): every culled term is
+    # exactly zero, so the sorted evaluation equals the given-order one bit
+    # for bit; the Gaussian kernel (h = 0.05) agrees to the terms below 2^-100
+    # of the largest mass that culling drops.
+    pts, w = ds.gaussian_mixture(1_000_000, seed=ds.SEED_MIXTURE_1M)
+    perm = ds.kd_order(pts, 64)
+    pts, w = pts[perm], w[perm]
+    mass = w / w.sum()
+    q = ds.uniform_box(1_000_000, -4.0, 4.0, ds.SEED_MIXTURE_1M + 1)
+    with bk.DirectKde(pts, q, mass) as kde:
+        for desc, exact in ((bk.KernelDesc(EPANPROD, 2, 0.08, 0.12), True), (bk.KernelDesc(GAUSS, 2, 0.05), False)):
+            given = evaluate(bk, kde, desc, len(q), bk.BK_QUERY_ORDER_GIVEN)
+            spatial = evaluate(bk, kde, desc, len(q), bk.BK_QUERY_ORDER_SPATIAL)
+            if exact:
+                np.testing.assert_array_equal(spatial, given)
+            else:
+                bound = 1.0 / (2 * np.pi * 0.05 ** 2)
+                assert np.all(np.abs(spatial - given) <= 1e-12 * np.maximum(np.abs(given), 1e-10 * bound))
