@@ -634,10 +634,13 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   const double abs_u = normalize ? abs_err / p.norm : abs_err;
   const double m_total = ctx->refs.m_sum > 0.0 ? ctx->refs.m_sum : 1.0;
   const int qpb = path == Path::F64 ? bk::kBlock : queries_per_item(path);
-  const long long nblocks = (nq + qpb - 1) / qpb;
-  ctx->bnd_mask.reserve((size_t)(nblocks * nwords) * sizeof(unsigned long long));
-  ctx->bnd_lo.reserve((size_t)nblocks * sizeof(double));
-  ctx->bnd_gap.reserve((size_t)nblocks * sizeof(double));
+  // queries are processed in chunks whose tile masks stay within 2^29 words (4 GiB)
+  const long long chunk_blocks = std::max<long long>(1, (1LL << 29) / nwords);
+  const long long chunk_q = chunk_blocks * qpb;
+  const long long mask_blocks = std::min<long long>(chunk_blocks, (nq + qpb - 1) / qpb);
+  ctx->bnd_mask.reserve((size_t)(mask_blocks * nwords) * sizeof(unsigned long long));
+  ctx->bnd_lo.reserve((size_t)mask_blocks * sizeof(double));
+  ctx->bnd_gap.reserve((size_t)mask_blocks * sizeof(double));
   ctx->bnd_counts.reserve(4 * sizeof(unsigned long long));
   hip_check(hipMemsetAsync(ctx->bnd_counts.ptr, 0, 4 * sizeof(unsigned long long), s), "clear counters");
   const PointSet &qs = query_set(ctx);
@@ -647,7 +650,7 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   const float *qx32 = nullptr, *qy32 = nullptr;
   const double *qx64 = nullptr, *qy64 = nullptr;
   long long *list = nullptr;
-  long long q_off = (long long)q_i;
+  long long q_base = (long long)q_i;
   if (path == Path::F64) {
     qx64 = qs.xd.as<double>();
     qy64 = qs.yd.as<double>();
@@ -663,7 +666,7 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
               "gather launch");
     qx32 = ctx->ord_x.as<float>();
     qy32 = ctx->ord_y.as<float>();
-    q_off = 0;
+    q_base = 0;
   } else {
     qx32 = qs.x.as<float>();
     qy32 = qs.y.as<float>();
@@ -671,46 +674,53 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   unsigned long long *mask = ctx->bnd_mask.as<unsigned long long>();
   unsigned long long *counts = ctx->bnd_counts.as<unsigned long long>();
   double *lo_b = ctx->bnd_lo.as<double>(), *gap_b = ctx->bnd_gap.as<double>();
-  hip_check(bk::launch_bound_pass(p, qx32 ? qx32 + q_off : nullptr, qy32 ? qy32 + q_off : nullptr,
-                                  qx64 ? qx64 + q_off : nullptr, qy64 ? qy64 + q_off : nullptr, nq, qpb,
-                                  ctx->tsum.as<double4>(), ntiles, rel_err, abs_u, m_total, nwords, mask, lo_b, gap_b,
-                                  counts, s),
-            "bound pass launch");
-  // the summed tiles' exact part
   ctx->bnd_exact.reserve((size_t)nq * sizeof(double));
   double *exact = ctx->bnd_exact.as<double>();
-  const long long slots =
-      path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
-  const int splits = choose_splits(nq, nr, qpb, tile_of(path), slots);
-  double *dst = exact;
-  if (splits > 1) {
-    ctx->partial.reserve((size_t)splits * nq * sizeof(double));
-    dst = ctx->partial.as<double>();
-  }
-  bk::BoundArgs ba{mask, nwords};
-  hipError_t e;
-  if (path == Path::GAUSS32)
-    e = bk::launch_gauss(p, qx32, qy32, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
-                         0, 0, nr, q_off, nq, splits, dst, scale, 0,
-                         ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, &ba);
-  else if (path == Path::EPAN64)
-    e = bk::launch_epan(p, qx32, qy32, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
-                        nullptr, 0, 0, nr, q_off, nq, splits, dst, scale, 0, s, &ba);
-  else
-    e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr, q_off,
-                              nq, splits, dst, scale, 0, s, &ba);
-  hip_check(e, "bounded kernel launch");
-  if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, exact, s), "reduce launch");
+  double *lo_all = lower, *hi_all = upper;
   if (spatial) {
-    // bounds in Z-curve order, then back to the caller's order
+    // bounds in Z-curve order first, back to the caller's order at the end
     ctx->ord_out.reserve((size_t)nq * sizeof(double));
     ctx->bnd_hi.reserve((size_t)nq * sizeof(double));
-    double *lo_ord = ctx->ord_out.as<double>(), *hi_ord = ctx->bnd_hi.as<double>();
-    hip_check(bk::launch_bounds_combine(exact, lo_b, gap_b, nq, qpb, factor, lo_ord, hi_ord, s), "bounds launch");
-    hip_check(bk::launch_scatter_f64(list, nq, lo_ord, lower, s), "scatter launch");
-    hip_check(bk::launch_scatter_f64(list, nq, hi_ord, upper, s), "scatter launch");
-  } else {
-    hip_check(bk::launch_bounds_combine(exact, lo_b, gap_b, nq, qpb, factor, lower, upper, s), "bounds launch");
+    lo_all = ctx->ord_out.as<double>();
+    hi_all = ctx->bnd_hi.as<double>();
+  }
+  const long long slots =
+      path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
+  for (long long c0 = 0; c0 < nq; c0 += chunk_q) {
+    const long long cn = std::min(chunk_q, nq - c0);
+    const long long q_off = q_base + c0;
+    hip_check(bk::launch_bound_pass(p, qx32 ? qx32 + q_off : nullptr, qy32 ? qy32 + q_off : nullptr,
+                                    qx64 ? qx64 + q_off : nullptr, qy64 ? qy64 + q_off : nullptr, cn, qpb,
+                                    ctx->tsum.as<double4>(), ntiles, rel_err, abs_u, m_total, nwords, mask, lo_b,
+                                    gap_b, counts, s),
+              "bound pass launch");
+    // the summed tiles' exact part
+    const int splits = choose_splits(cn, nr, qpb, tile_of(path), slots);
+    double *dst = exact + c0;
+    if (splits > 1) {
+      ctx->partial.reserve((size_t)splits * cn * sizeof(double));
+      dst = ctx->partial.as<double>();
+    }
+    bk::BoundArgs ba{mask, nwords};
+    hipError_t e;
+    if (path == Path::GAUSS32)
+      e = bk::launch_gauss(p, qx32, qy32, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(),
+                           nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0,
+                           ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, &ba);
+    else if (path == Path::EPAN64)
+      e = bk::launch_epan(p, qx32, qy32, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
+                          nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, s, &ba);
+    else
+      e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr,
+                                q_off, cn, splits, dst, scale, 0, s, &ba);
+    hip_check(e, "bounded kernel launch");
+    if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, cn, exact + c0, s), "reduce launch");
+    hip_check(bk::launch_bounds_combine(exact + c0, lo_b, gap_b, cn, qpb, factor, lo_all + c0, hi_all + c0, s),
+              "bounds launch");
+  }
+  if (spatial) {
+    hip_check(bk::launch_scatter_f64(list, nq, lo_all, lower, s), "scatter launch");
+    hip_check(bk::launch_scatter_f64(list, nq, hi_all, upper, s), "scatter launch");
   }
   if (stats) {
     unsigned long long c[4] = {0, 0, 0, 0};
