@@ -228,12 +228,54 @@ __global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
   }
 }
 
+// Wave-wide minimum of a double (every lane gets it).
+__device__ __forceinline__ double wave_min_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Minima over the workgroup of the 5 values v (one set per thread) through
+// box[5][waves]: written before a barrier, read after it.
+__device__ __forceinline__ void box_write(double (&box)[5][kCvBlock / 64], const double (&v)[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const double m = wave_min_d(v[i]);
+    if ((threadIdx.x & 63) == 0) box[i][threadIdx.x >> 6] = m;
+  }
+}
+__device__ __forceinline__ void box_read(const double (&box)[5][kCvBlock / 64], double (&v)[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    v[i] = box[i][0];
+#pragma unroll
+    for (int w = 1; w < kCvBlock / 64; ++w) v[i] = fmin(v[i], box[i][w]);
+  }
+}
+
+// The Epanechnikov pair kernel skips a stage whose references are all out of
+// the support of every bandwidth of the item's sub-grid for every query of
+// the item: the per-axis gap between the query box and the stage's box,
+// times the stage's smallest 1/abw and the sub-grid's smallest 1/h, is at
+// least 2 (the convolution kernel's support, wider than the kernel's) on one
+// axis.  The test repeats the pair term's own operations on smaller inputs,
+// so every skipped term is exactly zero as computed and no result changes.
 template <int A, int B, bool ADAPT>
 __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
   __shared__ double4 tile[2][kCvTile];
+  __shared__ double qbox[5][kCvBlock / 64];
+  __shared__ double tbox[2][5][kCvBlock / 64];
   __shared__ int s_item;
   const int t = threadIdx.x;
   const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double inf = __builtin_inf();
+  // (x0, -x1, y0, -y1, smallest 1/abw) of one point; a masked slot is neutral
+  auto point_box = [&](const double4 &r, bool valid, double (&v)[5]) {
+    v[0] = valid ? r.x : inf;
+    v[1] = valid ? -r.x : inf;
+    v[2] = valid ? r.y : inf;
+    v[3] = valid ? -r.y : inf;
+    v[4] = valid ? (ADAPT ? r.w : 1.0) : inf;
+  };
   for (;;) {
     __syncthreads();
     if (t == 0) s_item = atomicAdd(g.queue, 1);
@@ -249,25 +291,57 @@ __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
     for (int a = 0; a < A; ++a) kx[a] = g.kx64[w.sub_x + a];
 #pragma unroll
     for (int b = 0; b < B; ++b) ky[b] = g.ky64[w.sub_y + b];
+    // padded sub-grid entries repeat the last bandwidth, so the minimum over
+    // all A (B) entries is the sub-grid's smallest 1/h
+    double kx_min = kx[0], ky_min = ky[0];
+#pragma unroll
+    for (int a = 1; a < A; ++a) kx_min = fmin(kx_min, kx[a]);
+#pragma unroll
+    for (int b = 1; b < B; ++b) ky_min = fmin(ky_min, ky[b]);
     double accK[A][B], accT[A][B];
 #pragma unroll
     for (int a = 0; a < A; ++a)
 #pragma unroll
       for (int b = 0; b < B; ++b) accK[a][b] = accT[a][b] = 0.0;
 
-    tile[0][t] = w.r_begin + t < w.r_end ? g.pts64[w.r_begin + t] : zero;
+    {
+      double v[5];
+      point_box(q, true, v);  // inactive lanes hold the item's first query
+      box_write(qbox, v);
+    }
+    const double4 first = w.r_begin + t < w.r_end ? g.pts64[w.r_begin + t] : zero;
+    tile[0][t] = first;
+    {
+      double v[5];
+      point_box(first, w.r_begin + t < w.r_end, v);
+      box_write(tbox[0], v);
+    }
     __syncthreads();
+    double qb[5];
+    box_read(qbox, qb);
     int buf = 0;
     const int q_end = w.q_begin + w.q_count;
     for (int r0 = w.r_begin; r0 < w.r_end; r0 += kCvTile) {
       const int rn = r0 + kCvTile;
       double4 nxt = zero;
       if (rn + t < w.r_end) nxt = g.pts64[rn + t];
-      if (r0 < q_end && w.q_begin < rn)
-        epan_stage<A, B, ADAPT, true>(tile[buf], q.x, q.y, qi - r0, kx, ky, accK, accT);
-      else
-        epan_stage<A, B, ADAPT, false>(tile[buf], q.x, q.y, 0, kx, ky, accK, accT);
-      if (rn < w.r_end) tile[buf ^ 1][t] = nxt;
+      double tb[5];
+      box_read(tbox[buf], tb);
+      const double gx = fmax(0.0, fmax(tb[0] + qb[1], qb[0] + tb[1]));  // x0_t - x1_q, x0_q - x1_t
+      const double gy = fmax(0.0, fmax(tb[2] + qb[3], qb[2] + tb[3]));
+      const double ux = (ADAPT ? gx * tb[4] : gx) * kx_min, uy = (ADAPT ? gy * tb[4] : gy) * ky_min;
+      if (!(ux >= 2.0 || uy >= 2.0)) {
+        if (r0 < q_end && w.q_begin < rn)
+          epan_stage<A, B, ADAPT, true>(tile[buf], q.x, q.y, qi - r0, kx, ky, accK, accT);
+        else
+          epan_stage<A, B, ADAPT, false>(tile[buf], q.x, q.y, 0, kx, ky, accK, accT);
+      }
+      if (rn < w.r_end) {
+        tile[buf ^ 1][t] = nxt;
+        double v[5];
+        point_box(nxt, rn + t < w.r_end, v);
+        box_write(tbox[buf ^ 1], v);
+      }
       __syncthreads();
       buf ^= 1;
     }
@@ -502,7 +576,8 @@ struct bk_cv_grid {
   long long n_points = 0;
   bool adapt = false;
   DevBuf pts, pts64, pd, w, set_of_point, set_begin;
-  std::vector<double> m_host;    // FP32-rounded masses (as double)
+  std::vector<double> m_host;    // FP32-rounded masses (as double), device order
+  std::vector<int> perm;         // device position -> the caller's point index
 
   // per grid shape
   int n_hx = 0, n_hy = 0;
@@ -527,6 +602,35 @@ struct bk_cv_grid {
 };
 
 namespace {
+
+// perm[begin..end) = begin + the points of [begin, end) in Z-curve order:
+// Morton code of (x, y) on a 1024 x 1024 grid over the set's box, ties by
+// index.
+void spatial_order(const double *coords, int begin, int end, std::vector<int> &perm) {
+  double x0 = coords[2 * begin], x1 = x0, y0 = coords[2 * begin + 1], y1 = y0;
+  for (int i = begin; i < end; ++i) {
+    x0 = std::min(x0, coords[2 * i]);
+    x1 = std::max(x1, coords[2 * i]);
+    y0 = std::min(y0, coords[2 * i + 1]);
+    y1 = std::max(y1, coords[2 * i + 1]);
+  }
+  const double sx = x1 > x0 ? 1023.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 1023.0 / (y1 - y0) : 0.0;
+  auto spread = [](uint64_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+  };
+  std::vector<uint64_t> key(end - begin);
+  for (int i = begin; i < end; ++i) {
+    const uint64_t cx = (uint64_t)((coords[2 * i] - x0) * sx), cy = (uint64_t)((coords[2 * i + 1] - y0) * sy);
+    key[i - begin] = ((spread(cx) | (spread(cy) << 1)) << 32) | (uint64_t)(i - begin);
+  }
+  std::sort(key.begin(), key.end());
+  for (int k = 0; k < end - begin; ++k) perm[begin + k] = begin + (int)(key[k] & 0xFFFFFFFFu);
+}
 
 void drop_graph(bk_cv_grid *g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
@@ -944,36 +1048,48 @@ int bk_cv_grid_create(bk_ctx *ctx, int kind, size_t n_sets, const size_t *sizes,
     std::vector<double4> pts64(g->fam == CV_GAUSS ? 0 : total);
     std::vector<double2> pd(total);
     std::vector<int> set_of(total);
+    std::vector<double> w_dev(total);
     g->m_host.resize(total);
+    g->perm.resize(total);
     g->m_scale.assign(n_sets, 1.0);
     g->adapt = false;
     for (size_t d = 0; d < n_sets; ++d) {
       double mmax = 0.0;
       for (int i = g->begin[d]; i < g->begin[d + 1]; ++i) {
+        if (!std::isfinite(coords[2 * i]) || !std::isfinite(coords[2 * i + 1]))
+          fail(BK_ERR_INVALID_ARG, "coordinates must be finite");
+      }
+      // each dataset goes to the device in Z-curve order (Morton code on a
+      // 1024 x 1024 grid over its box, ties by index): consecutive points
+      // are close, so a work item's queries and a stage's references span
+      // small boxes (the Epanechnikov kernel skips stages out of reach)
+      spatial_order(coords, g->begin[d], g->begin[d + 1], g->perm);
+      for (int j = g->begin[d]; j < g->begin[d + 1]; ++j) {
+        const int i = g->perm[j];
         const double a = abw ? abw[i] : 1.0;
         if (!(a > 0.0) || !std::isfinite(a)) fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
         if (!(weights[i] >= 0.0) || !std::isfinite(weights[i]))
           fail(BK_ERR_INVALID_ARG, "weights must be non-negative and finite");
-        if (!std::isfinite(coords[2 * i]) || !std::isfinite(coords[2 * i + 1]))
-          fail(BK_ERR_INVALID_ARG, "coordinates must be finite");
         const float a32 = (float)a;
         if (a32 != 1.0f) g->adapt = true;
         const float m32 = (float)(weights[i] / (a * a));
-        g->m_host[i] = (double)m32;
+        g->m_host[j] = (double)m32;
+        w_dev[j] = weights[i];
         mmax = std::max(mmax, (double)m32);
-        set_of[i] = (int)d;
+        set_of[j] = (int)d;
       }
       const double ms = mmax > 0.0 ? mmax : 1.0;
       g->m_scale[d] = ms;
-      for (int i = g->begin[d]; i < g->begin[d + 1]; ++i) {
+      for (int j = g->begin[d]; j < g->begin[d + 1]; ++j) {
+        const int i = g->perm[j];
         const double a = (double)(float)(abw ? abw[i] : 1.0);
-        const double rel = g->m_host[i] / ms;
+        const double rel = g->m_host[j] / ms;
         const float x = (float)coords[2 * i], y = (float)coords[2 * i + 1];
         if (g->fam == CV_GAUSS)
-          pts[i] = make_float4(x, y, rel > 0.0 ? (float)std::log2(rel) : kMaskedLog2, (float)(1.0 / (a * a)));
+          pts[j] = make_float4(x, y, rel > 0.0 ? (float)std::log2(rel) : kMaskedLog2, (float)(1.0 / (a * a)));
         else
-          pts64[i] = make_double4(x, y, rel, 1.0 / a);
-        pd[i] = make_double2(rel, 1.0 / (a * a));
+          pts64[j] = make_double4(x, y, rel, 1.0 / a);
+        pd[j] = make_double2(rel, 1.0 / (a * a));
       }
     }
     hipStream_t s = nullptr;
@@ -994,7 +1110,7 @@ int bk_cv_grid_create(bk_ctx *ctx, int kind, size_t n_sets, const size_t *sizes,
       hip_check(hipMemcpyAsync(g->pts64.ptr, pts64.data(), total * sizeof(double4), hipMemcpyHostToDevice, s),
                 "upload");
     hip_check(hipMemcpyAsync(g->pd.ptr, pd.data(), total * sizeof(double2), hipMemcpyHostToDevice, s), "upload");
-    hip_check(hipMemcpyAsync(g->w.ptr, weights, total * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+    hip_check(hipMemcpyAsync(g->w.ptr, w_dev.data(), total * sizeof(double), hipMemcpyHostToDevice, s), "upload");
     hip_check(hipMemcpyAsync(g->set_of_point.ptr, set_of.data(), total * sizeof(int), hipMemcpyHostToDevice, s),
               "upload");
     hip_check(hipMemcpyAsync(g->set_begin.ptr, g->begin.data(), (n_sets + 1) * sizeof(int), hipMemcpyHostToDevice,
@@ -1066,11 +1182,12 @@ int bk_cv_grid_loo(bk_cv_grid *g, size_t set, size_t ix, size_t iy, double *out)
     DeviceGuard dg(g->device);
     const int b = g->begin[set], n = g->begin[set + 1] - b;
     const double *src = g->P.as<double>() + (long long)(ix * g->n_hy + iy) * g->n_points + b;
-    hip_check(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost), "copy densities");
+    std::vector<double> tmp(n);
+    hip_check(hipMemcpy(tmp.data(), src, n * sizeof(double), hipMemcpyDeviceToHost), "copy densities");
     const double hxv = g->hx_last[ix], hyv = g->hy_last[iy];
     const double C = g->fam == CV_GAUSS ? 1.0 / (2.0 * kPi * hxv * hyv) : 0.5625 / (hxv * hyv);
     const double f = C * g->m_scale[set];
-    for (int i = 0; i < n; ++i) out[i] *= f;
+    for (int j = 0; j < n; ++j) out[g->perm[b + j] - b] = tmp[j] * f;  // back to the caller's order
   });
 }
 

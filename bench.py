@@ -227,6 +227,28 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
         "best_lscv": [list(map(int, np.unravel_index(np.argmin(lscv[d]), lscv[d].shape)))
                       for d in range(len(CV_SIZES))],
     })
+    # the same plan with EpanechnikovProductKernel2d, the grid_search tool's
+    # kernel (FP64 arithmetic; stages out of every bandwidth's support are
+    # skipped, their terms being exactly zero)
+    with bk.CvGrid(bk.BK_EPANECHNIKOV_PRODUCT_2D, [p for p, _ in sets], [w for _, w in sets]) as cv:
+        cv.set_option(bk.BK_CVG_GRAPH, 0)
+        elcv, elscv = cv.run(CV_GRID, CV_GRID)
+        e_main_ms = cv.last_timing()[0]
+        cv.set_option(bk.BK_CVG_GRAPH, 1)
+        cv.run(CV_GRID, CV_GRID)
+        e_ms = []
+        for _ in range(max(1, min(steps, 2))):
+            cv.run(CV_GRID, CV_GRID)
+            e_ms.append(cv.last_timing()[1])
+        e_variant = cv.get_option(bk.BK_CVG_VARIANT)
+    e_avg = float(np.mean(e_ms))
+    res["epanechnikov_product"] = {
+        "ms_per_step": e_avg, "value": units / (e_avg * 1e-3), "unit": "pair-bw-evals/s", "main_kernel_ms": e_main_ms,
+        "tiling_variant": int(e_variant),
+        "note": "all (ordered pair, bandwidth pair) units counted; the pair kernel skips the 256-reference stages "
+                "out of the support of every bandwidth of its sub-grid (exactly zero terms)",
+        "best_lscv": [list(map(int, np.unravel_index(np.argmin(elscv[d]), elscv[d].shape)))
+                      for d in range(len(CV_SIZES))]}
     # independent check: the single-bandwidth C-ABI path on the 50k dataset
     p0, w0 = sets[0]
     checks = []
