@@ -141,6 +141,23 @@ class HipDirectKde {
     hip_detail::check(bk_eval_leaf_pairs(ctx_, &d, pairs.data(), pairs.size(), out.data(), 0));
   }
 
+  // Unnormalised sums in double precision; with spatial, the library works
+  // through the queries in Z-curve order on the device (BK_OPT_QUERY_ORDER),
+  // the results still in query order.
+  void unnormalized_eval_range(size_t r_i, size_t r_j, size_t q_i, size_t q_j, std::vector<double> &out,
+                               bool spatial) {
+    require_ctx();
+    if (r_j >= reference_size() || q_j >= query_size())
+      throw std::out_of_range("HipDirectKde<>: unnormalized_eval_range(): j_r and j_q should be less than the number "
+                              "of reference and query points. ");
+    if (r_i > r_j || q_i > q_j)
+      throw std::invalid_argument("HipDirectKde<>: unnormalized_eval_range(): must have i_r <= j_r and i_q <= j_q. ");
+    hip_detail::check(bk_set_option(ctx_, BK_OPT_QUERY_ORDER, spatial ? BK_QUERY_ORDER_SPATIAL : BK_QUERY_ORDER_GIVEN));
+    out.resize(q_j - q_i + 1);
+    const bk_kernel_desc d = kernel_.desc();
+    hip_detail::check(bk_eval_range(ctx_, &d, r_i, r_j, q_i, q_j, out.data(), 0));
+  }
+
   // Bounds of the unnormalised sums over every reference for queries
   // q_i..q_j: upper - lower <= max(abs_err, rel_err * lower)
   // (bk_eval_bounded).

@@ -14,7 +14,8 @@
 //   * the dual-tree evaluation meets the reference's bounds and tolerance
 //     contract (681-1110).
 // Device work goes through the C ABI (bbrcit_hip.h):
-//   * every direct sum is one GPU evaluation;
+//   * every direct sum is one GPU evaluation (queries sorted along a Z curve
+//     on the device, results in the caller's order);
 //   * the dual-tree evaluation is one bounded GPU evaluation
 //     (bk_eval_bounded): the pruning is decided per (query block, reference
 //     tile) on the device against the same bounds contract, instead of a
@@ -301,14 +302,11 @@ class KernelDensity {
     return *self_dev_.dev;
   }
 
+  // every reference against every query; the device takes the queries in
+  // Z-curve order (its workgroups then skip the reference tiles out of reach)
   template <typename K>
   void eval_all(Device<K> &dev, size_t nq, std::vector<double> &s) const {
-    std::vector<bk_range_pair> one(1);
-    one[0].r_i = 0;
-    one[0].r_j = size() - 1;
-    one[0].q_i = 0;
-    one[0].q_j = nq - 1;
-    dev.unnormalized_eval_pairs(one, s);
+    dev.unnormalized_eval_range(0, size() - 1, 0, nq - 1, s, true);
   }
 
   // ---- dual tree (KernelDensityImpl.h:681-1110) ------------------------------------
