@@ -34,6 +34,12 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-source additions.  bk_kernels.hip: MFMA results go straight to VGPRs
+# (the Gram kernels read every result with the VALU; the AGPR form would add
+# two v_accvgpr_read per pair term).
+SOURCE_FLAGS = {"bk_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def hipcc() -> str:
     exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not Path(exe).exists():
@@ -57,7 +63,8 @@ def build_library(verbose: bool = False) -> Path:
     objs = []
     for src in HIP_SOURCES:
         obj = build_dir / (Path(src).stem + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(REPO_DIR / "include"), "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src, []), "-I", str(REPO_DIR / "include"), "-c",
+               str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)

@@ -1006,13 +1006,17 @@ __global__ void prep_general_kernel(EvalParams p, const float *__restrict__ abw,
 }
 
 // ---- Gram form for 8 <= D <= 16: FP64 MFMA dot products ----------------------
-// |q - r|^2 = |q|^2 + |r|^2 - 2 q.r, with the q.r of 16 references x 16
-// queries per v_mfma_f64_16x16x4f64 chain (K = padded D in steps of 4).
-// FP64 products of FP32 coordinates are exact and the accumulation is FP64,
-// so the Gram form loses nothing to cancellation (an FP32 Gram matrix leaves
-// ~1e-5 relative error in the terms at d = 16); the matrix pipe runs the
-// dot products beside the VALU work of the terms.  Layout of the FP64
-// 16x16x4 MFMA (gfx950): lane l gives A[m = l%16][k = l/16] and
+// The exponent of a Gaussian term, log2 m - g |q - r|^2, is the bilinear form
+//   C_r + (2 g r) . q - g |q|^2,   C_r = log2 m - g |r|^2,
+// and the Epanechnikov factor 1 - g |q - r|^2 is the same form with
+// C_r = 1 - g |r|^2.  A chain of v_mfma_f64_16x16x4f64 (K = padded D in steps
+// of 4) computes C_r + (2 g r) . q for 16 references x 16 queries, with C_r as
+// the accumulator's initial value, so the VALU is left with one FP64 FMA
+// (- g |q|^2), the conversion to FP32 and the v_exp_f32 of a Gaussian term (or
+// a max and an FMA for Epanechnikov).  Everything up to that point is FP64, so
+// the cancellation between the three parts costs ~1e-16 of their size (an FP32
+// Gram matrix leaves ~1e-5 relative error in the terms at d = 16).  Layout of
+// the FP64 16x16x4 MFMA (gfx950): lane l gives A[m = l%16][k = l/16] and
 // B[k = l/16][n = l%16] and receives D[l/16 + 4i][l%16], i = 0..3 (the FP32
 // and BF16 16x16 forms return rows 4(l/16) + i instead).  References sit on
 // M, queries on N, so a lane's four results belong to its own query.
@@ -1027,30 +1031,33 @@ __device__ __forceinline__ double exp2_wide(double e) {
 
 constexpr int kGramGroups = 4;  // groups of 16 queries per wavefront (64 queries)
 
-// Terms of one 16 x 16 block of (reference, query) pairs from the dot
-// products d (lane layout above).  Gaussian: w = (|r|^2, g / a^2, log2 m):
-// m 2^(-g d2) = exp2(log2 m - g d2), the exponent formed in FP64 and rounded
-// to FP32 once (|error| <= 2^-24 |e|: 2.6e-6 relative at e = -128, below
-// which a term is < 2^-126 m and far under any density floor).
-// Epanechnikov: w = (|r|^2, g / a^2, m): m max(1 - g d2, 0) in FP64.
+// Terms of one 16 x 16 block of (reference, query) pairs from d = C_r +
+// (2 g r) . q (lane layout above); g4 holds g of the lane's four reference
+// rows, m4 their masses (Epanechnikov only).  Gaussian: exp2 of the exponent
+// formed in FP64 and rounded to FP32 once (|error| <= 2^-24 |e|: 2.6e-6
+// relative at e = -128, below which a term is < 2^-126 m and far under any
+// density floor), summed in FP32 within a tile of 128 references.
+// Epanechnikov: m max(e, 0) in FP64.
 template <int FAM, bool DIAG>
-__device__ __forceinline__ void gram_terms(const v4d &d, const double4 (&ws)[4], int jb, double qn, long long tile,
-                                           long long q, double &acc) {
+__device__ __forceinline__ void gram_terms(const v4d &d, const v4d &g4, const v4d &m4, int jb, double qn,
+                                           long long tile, long long q, float &acc32, double &acc) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int j = jb + 4 * i;
-    const double4 w = ws[i];
-    const double d2 = __builtin_fma(-2.0, d[i], qn + w.x);
-    double v;
-    if constexpr (FAM == FAM_GAUSS)
-      v = (double)__builtin_amdgcn_exp2f((float)__builtin_fma(-w.y, d2, w.z));
-    else
-      v = w.z * __builtin_fmax(__builtin_fma(-w.y, d2, 1.0), 0.0);
-    if (DIAG) v = (tile + j == q) ? 0.0 : v;
-    acc += v;
+    const double e = __builtin_fma(-g4[i], qn, d[i]);
+    if constexpr (FAM == FAM_GAUSS) {
+      float v = __builtin_amdgcn_exp2f((float)e);
+      if (DIAG) v = (tile + j == q) ? 0.0f : v;
+      acc32 += v;
+    } else {
+      double v = __builtin_fmax(e, 0.0);
+      if (DIAG) v = (tile + j == q) ? 0.0 : v;
+      acc = __builtin_fma(m4[i], v, acc);
+    }
   }
 }
 
+// rw: per reference (C_r, g, m, |r|^2) from prep_gram_kernel.
 template <int DP, int FAM>
 __device__ __forceinline__ void gram_item(const float *__restrict__ qc, const float *__restrict__ rc,
                                           const double4 *__restrict__ rw, int D, long long r_begin,
@@ -1061,6 +1068,7 @@ __device__ __forceinline__ void gram_item(const float *__restrict__ qc, const fl
   __shared__ double4 sw[kTileG];
   const int t = threadIdx.x, wave = t / 64, lane = t % 64, g16 = lane / 16, n16 = lane % 16;
   double bq[kGramGroups][KS], qn[kGramGroups], acc[kGramGroups];
+  float acc32[kGramGroups];
   long long qidx[kGramGroups];
   bool qok[kGramGroups];
 #pragma unroll
@@ -1081,36 +1089,58 @@ __device__ __forceinline__ void gram_item(const float *__restrict__ qc, const fl
       bq[c][s] = k < D ? (double)qc[q * D + k] : 0.0;
     }
     acc[c] = 0.0;
+    acc32[c] = 0.0f;
   }
-  const double4 pad = FAM == FAM_GAUSS ? make_double4(0.0, 0.0, -__builtin_inf(), 0.0)
+  // padding rows: A = 0 and g = 0; C = -inf gives a zero Gaussian term, and
+  // m = 0 a zero Epanechnikov term
+  const double4 pad = FAM == FAM_GAUSS ? make_double4(-__builtin_inf(), 0.0, 0.0, 0.0)
                                        : make_double4(0.0, 0.0, 0.0, 0.0);
   const long long q_end = q_begin + q_count;
   for (long long tile = r_begin; tile < r_end; tile += kTileG) {
     const int cnt = (int)((r_end - tile) < kTileG ? (r_end - tile) : kTileG);
     for (int e = t; e < kTileG * DP; e += kBlockG) {
       const int j = e / DP, k = e % DP;
-      sr[j][k] = (j < cnt && k < D) ? (double)rc[(tile + j) * D + k] : 0.0;
+      // A = 2 g r, rounded once in FP64
+      sr[j][k] = (j < cnt && k < D) ? 2.0 * rw[tile + j].y * (double)rc[(tile + j) * D + k] : 0.0;
     }
     for (int e = t; e < kTileG; e += kBlockG) sw[e] = e < cnt ? rw[tile + e] : pad;
     __syncthreads();
     const bool diag = self_excl && tile < q_end && q_begin < tile + kTileG;
+#pragma unroll 1
     for (int sub = 0; sub < kTileG; sub += 16) {
       double a[KS];
 #pragma unroll
       for (int s = 0; s < KS; ++s) a[s] = sr[sub + n16][4 * s + g16];
       // the four references of this lane's result rows, shared by the query groups
-      double4 ws[4];
+      v4d c4, g4, m4;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) ws[i] = sw[sub + g16 + 4 * i];
+      for (int i = 0; i < 4; ++i) {
+        const double4 w = sw[sub + g16 + 4 * i];
+        c4[i] = w.x;
+        g4[i] = w.y;
+        m4[i] = w.z;
+      }
+      v4d d[kGramGroups];
+#pragma unroll
+      for (int c = 0; c < kGramGroups; ++c) d[c] = c4;
+      // independent chains side by side: consecutive MFMAs never wait on each other
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int c = 0; c < kGramGroups; ++c) d[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bq[c][s], d[c], 0, 0, 0);
 #pragma unroll
       for (int c = 0; c < kGramGroups; ++c) {
-        v4d d = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s = 0; s < KS; ++s) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bq[c][s], d, 0, 0, 0);
         if (diag)
-          gram_terms<FAM, true>(d, ws, sub + g16, qn[c], tile, qidx[c], acc[c]);
+          gram_terms<FAM, true>(d[c], g4, m4, sub + g16, qn[c], tile, qidx[c], acc32[c], acc[c]);
         else
-          gram_terms<FAM, false>(d, ws, sub + g16, qn[c], tile, qidx[c], acc[c]);
+          gram_terms<FAM, false>(d[c], g4, m4, sub + g16, qn[c], tile, qidx[c], acc32[c], acc[c]);
+      }
+    }
+    if constexpr (FAM == FAM_GAUSS) {
+#pragma unroll
+      for (int c = 0; c < kGramGroups; ++c) {
+        acc[c] += (double)acc32[c];
+        acc32[c] = 0.0f;
       }
     }
     __syncthreads();
@@ -1148,8 +1178,10 @@ __global__ __launch_bounds__(kBlockG) void gram_list_kernel(
                      self_excl);
 }
 
-// Per reference: (|r|^2 in FP64 of the FP32 coordinates, g / a^2, w, 0)
-// with w = log2 m (Gaussian; -inf for m = 0) or m (Epanechnikov).
+// Per reference: (C_r, g, m, |r|^2) with g = gx / a^2 (in base-2 exponent
+// units for the Gaussian), |r|^2 in FP64 of the FP32 coordinates and
+// C_r = log2 m - g |r|^2 (Gaussian; -inf for m = 0) or 1 - g |r|^2
+// (Epanechnikov).
 __global__ void prep_gram_kernel(EvalParams p, const float *__restrict__ rc, const float *__restrict__ abw,
                                  const float *__restrict__ m, long long n, int D, double4 *__restrict__ rw) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1161,8 +1193,10 @@ __global__ void prep_gram_kernel(EvalParams p, const float *__restrict__ rc, con
   }
   const double a = (double)abw[i];
   const double mi = (double)m[i];
-  const double wv = p.family == FAM_GAUSS ? (mi > 0.0 ? log2(mi) : -__builtin_inf()) : mi;
-  rw[i] = make_double4(nrm, p.gx / (a * a), wv, 0.0);
+  const double g = p.gx / (a * a);
+  const double c = p.family == FAM_GAUSS ? (mi > 0.0 ? __builtin_fma(-g, nrm, log2(mi)) : -__builtin_inf())
+                                         : __builtin_fma(-g, nrm, 1.0);
+  rw[i] = make_double4(c, g, mi, nrm);
 }
 
 // ---- FP64 rescue of underflowed Gaussian sums ---------------------------------

@@ -196,6 +196,29 @@ def test_gram_leave_one_out_and_forms_agree(bk, oracle, dim):
     assert_parity(loo_direct, want, bound, what=f"direct LOO D={dim}")
 
 
+@pytest.mark.parametrize("kind", [GAUSS, EPAN])
+def test_gram_far_from_origin(bk, oracle, kind):
+    # The Gram form adds |r|^2, r.q and |q|^2 parts of the exponent; points
+    # 40 units from the origin make each part ~1e4 times their sum, which the
+    # FP64 parts must still resolve to the parity bar (adaptive bandwidths,
+    # uneven masses, D = 12 and 16).
+    for dim in (12, 16):
+        rng = np.random.default_rng(300 + 10 * kind + dim)
+        pts = 40.0 + rng.standard_normal((2500, dim))
+        q = 40.0 + 1.2 * rng.standard_normal((900, dim))
+        mass = rng.uniform(0.5, 1.5, len(pts))
+        mass /= mass.sum()
+        abw = rng.uniform(0.8, 1.25, len(pts))
+        h = 0.8 if kind == GAUSS else 3.0
+        desc = bk.KernelDesc(kind, dim, h)
+        with bk.DirectKde(pts, q, mass, abw) as kde:
+            got = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        want = oracle.direct(kind, h, h, f32(pts), f32(q), mass=f32(mass), abw=f32(abw), normalize=True)
+        bound = oracle.normalization(kind, dim, h, h) * mass.sum()
+        assert (want > 1e-10 * bound).mean() > 0.5  # most sums lie above the floor
+        assert_parity(got, want, bound, what=f"offset D={dim} kind {kind}")
+
+
 def test_mixture_1m_refs_sampled_queries(bk, oracle):
     # Full-size reference set of the headline workload (1M mixture points,
     # kd order), checked on sampled query ranges against the oracle.
