@@ -136,6 +136,7 @@ EvalParams params_of(const bk_kernel_desc *d) {
       need_dim(1, 16);
       p.family = bk::FAM_EPAN;
       p.gx = p.gy = 1.0 / (hx * hx);
+      p.ix = p.iy = 1.0 / hx;
       p.iso = 1;
       const double D = d->dim;
       p.norm = 0.5 * (D + 2.0) / (std::pow(kPi, 0.5 * D) / std::tgamma(1.0 + 0.5 * D)) /
@@ -170,6 +171,8 @@ EvalParams params_of(const bk_kernel_desc *d) {
       p.family = bk::FAM_EPANPROD;
       p.gx = 1.0 / (hx * hx);
       p.gy = 1.0 / (hy * hy);
+      p.ix = 1.0 / hx;
+      p.iy = 1.0 / hy;
       p.iso = 0;
       p.norm = 0.5625 / (hx * hy);
       break;

@@ -23,7 +23,7 @@ struct EvalParams {
   int dim;          // point dimension
   int iso;          // 1 if the quadratic form weights both axes equally
   double gx, gy;    // FAM_GAUSS: c*log2(e)/h_d^2; FAM_EPAN/EPANPROD: 1/h_d^2
-  double ix, iy;    // FAM_EPANCONV: 1/h_d
+  double ix, iy;    // FAM_EPAN, FAM_EPANPROD, FAM_EPANCONV: 1/h_d
   double c;         // FAM_GAUSS: exponent constant (0.5 or 0.25)
   double mass_fold; // constant folded into every mass (FAM_EPANCONV: 2^-5 per axis)
   double norm;      // normalisation C(h), in double

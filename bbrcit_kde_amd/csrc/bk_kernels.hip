@@ -581,9 +581,15 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
 }
 
 // ---- FP64 Epanechnikov kinds on FP32 points ----------------------------------
-//   FAM_EPAN:     f0 = (x, y, w, m),     w = 1/(h a)^2
-//   FAM_EPANPROD: f0 = (x, y, wx, wy), f1.x = m
-//   FAM_EPANCONV: f0 = (x, y, 1/(hx a), 1/(hy a)), f1.x = m * 2^-10
+// Per reference the scale s = 1 / (h a) of each axis and the scaled offset
+// o = -s r, so that the scaled difference u = s (q - r) is one FMA,
+// fma(s, q, o):
+//   FAM_EPAN:     f0 = (s, -s x, -s y, m)
+//   FAM_EPANPROD: f0 = (sx, -sx x, sy, -sy y), f1.x = m
+//   FAM_EPANCONV: f0 = (sx, -sx x, sy, -sy y), f1.x = m * 2^-10
+// u carries at most 2^-53 (|s r| + |u|) of rounding, ~1e-16 of the
+// coordinates in bandwidth units; the support edge 1 - |u|^2 is then formed
+// by FMAs in FP64.
 template <int FAM>
 struct EpanNf {
   static constexpr int kNf = (FAM == FAM_EPAN) ? 1 : 2;
@@ -591,19 +597,20 @@ struct EpanNf {
 
 template <int FAM>
 __device__ __forceinline__ double epan_term(double qx, double qy, double4 a, double4 b) {
-  const double dx = qx - a.x;
-  const double dy = qy - a.y;
   if constexpr (FAM == FAM_EPAN) {
-    const double d2 = __builtin_fma(dy, dy, dx * dx);
-    const double t = __builtin_fmax(__builtin_fma(-a.z, d2, 1.0), 0.0);
-    return a.w * t;
+    const double ux = __builtin_fma(a.x, qx, a.y);
+    const double uy = __builtin_fma(a.x, qy, a.z);
+    const double t = __builtin_fma(-ux, ux, __builtin_fma(-uy, uy, 1.0));
+    return a.w * __builtin_fmax(t, 0.0);
   } else if constexpr (FAM == FAM_EPANPROD) {
-    const double tx = __builtin_fmax(__builtin_fma(-a.z * dx, dx, 1.0), 0.0);
-    const double ty = __builtin_fmax(__builtin_fma(-a.w * dy, dy, 1.0), 0.0);
+    const double ux = __builtin_fma(a.x, qx, a.y);
+    const double uy = __builtin_fma(a.z, qy, a.w);
+    const double tx = __builtin_fmax(__builtin_fma(-ux, ux, 1.0), 0.0);
+    const double ty = __builtin_fmax(__builtin_fma(-uy, uy, 1.0), 0.0);
     return b.x * (tx * ty);
   } else {
-    const double ux = __builtin_fabs(dx) * a.z;
-    const double uy = __builtin_fabs(dy) * a.w;
+    const double ux = __builtin_fabs(__builtin_fma(a.x, qx, a.y));
+    const double uy = __builtin_fabs(__builtin_fma(a.z, qy, a.w));
     const double cx = __builtin_fmax(2.0 - ux, 0.0);
     const double cy = __builtin_fmax(2.0 - uy, 0.0);
     const double kx = cx * cx * cx * __builtin_fma(ux, ux + 6.0, 4.0);
@@ -613,26 +620,38 @@ __device__ __forceinline__ double epan_term(double qx, double qy, double4 a, dou
 }
 
 // Per aligned tile of kTile references, two double4: the bounding box
-// (x0, x1, y0, y1) and the smallest per-reference scales (min f0.z, min
-// f0.w) of the tile (EPANPROD and EPANCONV use both, EPAN the first).
-// A tile is culled for a workgroup when every term is exactly zero for the
-// whole box of its queries.  The test evaluates the term's own support
-// condition at the per-axis gaps between the boxes with the tile's smallest
-// scales; every FP64 operation on the way is monotone, so a culled tile's
-// terms are zero as computed, and culling changes no result.
+// (x0, x1, y0, y1) of the FP32 coordinates, and per axis the smallest scale
+// s (rounded down to FP32) and the largest |o| (rounded up to FP32).
+// A tile is culled for a workgroup when every term is exactly zero, as
+// computed, for the whole box of its queries.
+//
+// u_lower bounds |u| = |fma(s, q, o)| of every (query, reference) pair of
+// the two boxes on one axis from below, up to its rounding: with G = s_min
+// times the gap between the boxes (G <= s |q - r| exactly), u_lower <=
+// G - 2^-50 (1 + o_max), while the computed |u| >= G (1 - 2^-53) -
+// 2^-52 o_max.  So u_lower >= c (c >= 1) gives |u| > c, and u_lower <= 4
+// gives |u| >= u_lower.
+__device__ __forceinline__ double u_lower(double r0, double r1, float b0, float b1, double s_min, double o_max) {
+  const double gap = fmax(0.0, fmax(r0 - (double)b1, (double)b0 - r1));
+  return fmax(0.0, __builtin_fma(s_min * gap, 1.0 - 0x1p-50, -0x1p-50 * (1.0 + o_max)));
+}
+
 template <int FAM>
 __device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, long long tile_idx, float bx0, float bx1,
                                               float by0, float by1) {
   const double4 b = tb[2 * tile_idx];
   const double4 w = tb[2 * tile_idx + 1];
-  const double gx = fmax(0.0, fmax(b.x - (double)bx1, (double)bx0 - b.y));
-  const double gy = fmax(0.0, fmax(b.z - (double)by1, (double)by0 - b.w));
+  const double lx = u_lower(b.x, b.y, bx0, bx1, w.x, w.z);
+  const double ly = u_lower(b.z, b.w, by0, by1, w.y, w.w);
   if constexpr (FAM == FAM_EPAN) {
-    return __builtin_fma(-w.x, __builtin_fma(gy, gy, gx * gx), 1.0) <= 0.0;
+    // t = fma(-ux, ux, fma(-uy, uy, 1)) <= 0 once ux^2 + uy^2 >= 1 + 2^-52;
+    // ux^2 + uy^2 >= lx^2 + ly^2 when both bounds are <= 4
+    if (lx >= 1.5 || ly >= 1.5) return true;
+    return __builtin_fma(lx, lx, ly * ly) >= 1.0 + 0x1p-48;
   } else if constexpr (FAM == FAM_EPANPROD) {
-    return __builtin_fma(-(w.x * gx), gx, 1.0) <= 0.0 || __builtin_fma(-(w.y * gy), gy, 1.0) <= 0.0;
+    return lx >= 1.0 || ly >= 1.0;  // 1 - u^2 <= 0 on that axis
   } else {
-    return gx * w.x >= 2.0 || gy * w.y >= 2.0;
+    return lx >= 2.0 || ly >= 2.0;  // 2 - u <= 0 on that axis
   }
 }
 
@@ -762,49 +781,62 @@ __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = (double)abw[i];
-  const double ia2 = 1.0 / (a * a);
   const double mass = (double)m[i];
   const double xi = (double)x[i], yi = (double)y[i];
+  const double sx = p.ix / a, sy = p.iy / a;
   switch (p.family) {
     case FAM_EPAN:
-      f0[i] = make_double4(xi, yi, p.gx * ia2, mass);
+      f0[i] = make_double4(sx, -sx * xi, -sx * yi, mass);
       break;
     case FAM_EPANPROD:
-      f0[i] = make_double4(xi, yi, p.gx * ia2, p.gy * ia2);
+      f0[i] = make_double4(sx, -sx * xi, sy, -sy * yi);
       f1[i] = make_double4(mass, 0.0, 0.0, 0.0);
       break;
     default:
-      f0[i] = make_double4(xi, yi, p.ix / a, p.iy / a);
+      f0[i] = make_double4(sx, -sx * xi, sy, -sy * yi);
       f1[i] = make_double4(mass * p.mass_fold, 0.0, 0.0, 0.0);
       break;
   }
 }
 
-// One workgroup of kTile threads per aligned tile: the tile's box and
-// smallest scales (see epan_tile_far).
-__global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const double4 *__restrict__ f0, long long n,
-                                                                 double4 *__restrict__ tb) {
+// One workgroup of kTile threads per aligned tile: the tile's box, smallest
+// scales and largest offsets (see epan_tile_far).
+__global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__restrict__ x,
+                                                                 const float *__restrict__ y,
+                                                                 const double4 *__restrict__ f0, long long n,
+                                                                 int iso, double4 *__restrict__ tb) {
   __shared__ float sh[kTile];
   const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
   const bool valid = i < n;
   const double4 a = valid ? f0[i] : make_double4(0.0, 0.0, 0.0, 0.0);
   const float big = 3.0e38f;
+  const float xi = valid ? x[i] : 0.0f, yi = valid ? y[i] : 0.0f;
   // coordinates are FP32 values, so the FP32 reductions are exact
-  const float x0 = block_reduce_f<kTile, false>(valid ? (float)a.x : big, sh);
-  const float x1 = block_reduce_f<kTile, true>(valid ? (float)a.x : -big, sh);
-  const float y0 = block_reduce_f<kTile, false>(valid ? (float)a.y : big, sh);
-  const float y1 = block_reduce_f<kTile, true>(valid ? (float)a.y : -big, sh);
-  // smallest scales, rounded down to FP32 so the test never overstates them
+  const float x0 = block_reduce_f<kTile, false>(valid ? xi : big, sh);
+  const float x1 = block_reduce_f<kTile, true>(valid ? xi : -big, sh);
+  const float y0 = block_reduce_f<kTile, false>(valid ? yi : big, sh);
+  const float y1 = block_reduce_f<kTile, true>(valid ? yi : -big, sh);
+  // smallest scales rounded down, largest offsets rounded up to FP32, so
+  // the test never overstates the first or understates the second
   auto down = [](double v) {
     float f = (float)v;
     if ((double)f > v) f = __uint_as_float(__float_as_uint(f) - 1u);  // v > 0
     return f;
   };
-  const float wz = block_reduce_f<kTile, false>(valid ? down(a.z) : big, sh);
-  const float ww = block_reduce_f<kTile, false>(valid ? down(a.w) : big, sh);
+  auto up = [](double v) {
+    float f = (float)v;
+    if ((double)f < v) f = __uint_as_float(__float_as_uint(f) + 1u);  // v >= 0
+    return f;
+  };
+  const double sxv = a.x, syv = iso ? a.x : a.z;
+  const double oxv = fabs(a.y), oyv = fabs(iso ? a.z : a.w);
+  const float sx = block_reduce_f<kTile, false>(valid ? down(sxv) : big, sh);
+  const float sy = block_reduce_f<kTile, false>(valid ? down(syv) : big, sh);
+  const float ox = block_reduce_f<kTile, true>(valid ? up(oxv) : 0.0f, sh);
+  const float oy = block_reduce_f<kTile, true>(valid ? up(oyv) : 0.0f, sh);
   if (threadIdx.x == 0) {
     tb[2 * blockIdx.x] = make_double4(x0, x1, y0, y1);
-    tb[2 * blockIdx.x + 1] = make_double4(wz, ww, 0.0, 0.0);
+    tb[2 * blockIdx.x + 1] = make_double4(sx, sy, ox, oy);
   }
 }
 
@@ -812,6 +844,11 @@ __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const double4 *_
 constexpr int kBlock64 = 256;
 constexpr int kTile64 = 256;
 
+// FP64 points: differences of the coordinates themselves.
+//   FAM_GAUSS:    a = (x, y, gx / a_r^2, m)
+//   FAM_EPAN:     a = (x, y, w, m),     w = 1/(h a)^2
+//   FAM_EPANPROD: a = (x, y, wx, wy), b.x = m
+//   FAM_EPANCONV: a = (x, y, 1/(hx a), 1/(hy a)), b.x = m * 2^-10
 template <int FAM>
 __device__ __forceinline__ double pair_term_f64(double qx, double qy,
                                                 double4 a, double4 b,
@@ -819,10 +856,23 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   const double dx = qx - a.x;
   const double dy = qy - a.y;
   if constexpr (FAM == FAM_GAUSS) {
-    // a = (x, y, gx/a_r^2, m): exp2 of the base-2 exponent
+    // exp2 of the base-2 exponent
     return a.w * exp2(-a.z * (dx * dx + rho * dy * dy));
+  } else if constexpr (FAM == FAM_EPAN) {
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    return a.w * __builtin_fmax(__builtin_fma(-a.z, d2, 1.0), 0.0);
+  } else if constexpr (FAM == FAM_EPANPROD) {
+    const double tx = __builtin_fmax(__builtin_fma(-a.z * dx, dx, 1.0), 0.0);
+    const double ty = __builtin_fmax(__builtin_fma(-a.w * dy, dy, 1.0), 0.0);
+    return b.x * (tx * ty);
   } else {
-    return epan_term<FAM>(qx, qy, a, b);
+    const double ux = __builtin_fabs(dx) * a.z;
+    const double uy = __builtin_fabs(dy) * a.w;
+    const double cx = __builtin_fmax(2.0 - ux, 0.0);
+    const double cy = __builtin_fmax(2.0 - uy, 0.0);
+    const double kx = cx * cx * cx * __builtin_fma(ux, ux + 6.0, 4.0);
+    const double ky = cy * cy * cy * __builtin_fma(uy, uy + 6.0, 4.0);
+    return b.x * (kx * ky);
   }
 }
 
@@ -1548,7 +1598,8 @@ hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
   prep_epan_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, x, y, m, abw, (long long)n, f0, f1);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  prep_epan_tiles_kernel<<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(f0, (long long)n, tb);
+  prep_epan_tiles_kernel<<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(x, y, f0, (long long)n,
+                                                                                p.family == FAM_EPAN ? 1 : 0, tb);
   return hipGetLastError();
 }
 
