@@ -99,6 +99,24 @@ def test_self_eval_culls_in_tree_order(bk, oracle, kind, h):
     assert_parity(got, want, oracle.normalization(kind, 2, *h) * mass.sum(), what=f"loo kind {kind}")
 
 
+@pytest.mark.parametrize("kind,h", [(EPAN, (0.06, 0.06)), (EPANPROD, (0.05, 0.08)), (EPANPRODCONV, (0.05, 0.08))])
+def test_epanechnikov_culling_far_from_origin(bk, oracle, kind, h):
+    # Points 5000 units from the origin: the scaled offsets s r reach 1e5, so
+    # the culling test's allowance for their rounding is ~1e-10 in u; culled
+    # sums must still equal the unculled ones bit for bit and meet the oracle.
+    pts, mass = kd_mixture(20_000, 700 + kind)
+    pts = pts + np.array([5000.0, -3000.0])
+    q = ds.uniform_box(6_000, -4.0, 4.0, 800 + kind) + np.array([5000.0, -3000.0])
+    desc = bk.KernelDesc(kind, 2, *h)
+    with bk.DirectKde(pts, q, mass) as kde:
+        given = evaluate(bk, kde, desc, len(q), bk.BK_QUERY_ORDER_GIVEN)
+        spatial = evaluate(bk, kde, desc, len(q), bk.BK_QUERY_ORDER_SPATIAL)
+    np.testing.assert_array_equal(spatial, given)
+    want = oracle.direct(kind, h[0], h[1], f32(pts), f32(q), mass=f32(mass), normalize=True)
+    assert_parity(given, want, oracle.normalization(kind, 2, *h) * mass.sum(), what=f"offset kind {kind}")
+    assert (want > 0).mean() > 0.05
+
+
 def test_query_order_option_is_checked(bk):
     pts, mass = kd_mixture(1_000, 5)
     with bk.DirectKde(pts, pts, mass) as kde:
