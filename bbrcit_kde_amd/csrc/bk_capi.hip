@@ -370,7 +370,7 @@ Path path_of(const bk_ctx *ctx, const EvalParams &p) {
 // Queries handled by one workgroup / work item on a path.
 int queries_per_item(Path path) {
   if (path == Path::GAUSS32) return bk::kGaussQueriesPerBlock;
-  return path == Path::EPAN64 ? bk::kQueriesPerBlock : 256;
+  return (path == Path::EPAN64 || path == Path::F64) ? bk::kQueriesPerBlock : 256;
 }
 
 int tile_of(Path path) {
@@ -636,7 +636,7 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   const double factor = normalize ? p.norm : 1.0;  // mass units -> output units
   const double abs_u = normalize ? abs_err / p.norm : abs_err;
   const double m_total = ctx->refs.m_sum > 0.0 ? ctx->refs.m_sum : 1.0;
-  const int qpb = path == Path::F64 ? bk::kBlock : queries_per_item(path);
+  const int qpb = queries_per_item(path);
   // queries are processed in chunks whose tile masks stay within 2^29 words (4 GiB)
   const long long chunk_blocks = std::max<long long>(1, (1LL << 29) / nwords);
   const long long chunk_q = chunk_blocks * qpb;

@@ -844,6 +844,38 @@ __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__r
 constexpr int kBlock64 = 256;
 constexpr int kTile64 = 256;
 
+// 2^e in FP64 for e <= 0: e = n + f with n = rint(e) and |f| <= 1/2 (for
+// every finite e), 2^f = exp(f ln 2) by its Taylor series to degree 12 in
+// Horner form (truncation below 2e-16 relative for |f ln 2| <= 0.347;
+// coefficients (ln 2)^k / k! rounded once), scaled by ldexp.  The scale is
+// raised to 2^-2000 at least, which underflows to 0 as the true value does;
+// NaN stays NaN (f is NaN).
+constexpr double kLn2 = 0.6931471805599453;
+constexpr double exp2_coef(int k) {
+  double c = 1.0;
+  for (int i = 1; i <= k; ++i) c = c * kLn2 / i;
+  return c;
+}
+
+__device__ __forceinline__ double exp2_f64(double e) {
+  const double n = __builtin_rint(e);
+  const double f = e - n;
+  double p = exp2_coef(12);
+  p = __builtin_fma(p, f, exp2_coef(11));
+  p = __builtin_fma(p, f, exp2_coef(10));
+  p = __builtin_fma(p, f, exp2_coef(9));
+  p = __builtin_fma(p, f, exp2_coef(8));
+  p = __builtin_fma(p, f, exp2_coef(7));
+  p = __builtin_fma(p, f, exp2_coef(6));
+  p = __builtin_fma(p, f, exp2_coef(5));
+  p = __builtin_fma(p, f, exp2_coef(4));
+  p = __builtin_fma(p, f, exp2_coef(3));
+  p = __builtin_fma(p, f, exp2_coef(2));
+  p = __builtin_fma(p, f, exp2_coef(1));
+  p = __builtin_fma(p, f, 1.0);
+  return __builtin_ldexp(p, (int)__builtin_fmax(n, -2000.0));
+}
+
 // FP64 points: differences of the coordinates themselves.
 //   FAM_GAUSS:    a = (x, y, gx / a_r^2, m)
 //   FAM_EPAN:     a = (x, y, w, m),     w = 1/(h a)^2
@@ -857,7 +889,7 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   const double dy = qy - a.y;
   if constexpr (FAM == FAM_GAUSS) {
     // exp2 of the base-2 exponent
-    return a.w * exp2(-a.z * (dx * dx + rho * dy * dy));
+    return a.w * exp2_f64(-a.z * __builtin_fma(rho * dy, dy, dx * dx));
   } else if constexpr (FAM == FAM_EPAN) {
     const double d2 = __builtin_fma(dy, dy, dx * dx);
     return a.w * __builtin_fmax(__builtin_fma(-a.z, d2, 1.0), 0.0);
@@ -876,6 +908,9 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   }
 }
 
+// kQpt queries per thread (kBlock64 * kQpt per workgroup), each reference
+// read once from LDS for all of them; the self pair is excluded only on
+// tiles that hold the workgroup's own queries.
 template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void direct_f64_item(
     const double *__restrict__ qx, const double *__restrict__ qy,
@@ -884,18 +919,24 @@ __device__ __forceinline__ void direct_f64_item(
     double *__restrict__ dst, double out_scale, int self_excl, double rho,
     const unsigned long long *__restrict__ bmask) {
   constexpr int kNf = (FAM == FAM_EPANPROD || FAM == FAM_EPANCONV) ? 2 : 1;
-  static_assert(kBlock64 == kBlock && kTile64 == kTile, "one query per thread, tile summaries per kTile");
+  static_assert(kBlock64 == kBlock && kTile64 == kTile, "workgroup geometry of the tile summaries and masks");
   __shared__ double4 s0[kTile64];
   __shared__ double4 s1[kNf == 2 ? kTile64 : 1];
   const int t = threadIdx.x;
-  const bool valid = t < q_count;
-  const long long q = q_begin + (valid ? t : 0);
-  const double px = qx[q], py = qy[q];
-  double sum = 0.0;
+  double px[kQpt], py[kQpt], acc[kQpt];
+#pragma unroll
+  for (int k = 0; k < kQpt; ++k) {
+    const int qi = t + k * kBlock64;
+    const long long q = q_begin + (qi < q_count ? qi : 0);
+    px[k] = qx[q];
+    py[k] = qy[q];
+    acc[k] = 0.0;
+  }
   auto live_from = [&](long long tl) {
     if constexpr (BOUNDED) return next_listed_tile<kTile64>(bmask, tl, r_end);
     return tl;
   };
+  const long long q_lo = q_begin, q_hi = q_begin + q_count;
   for (long long tile = live_from(r_begin); tile < r_end; tile = live_from(tile + kTile64)) {
     const long long r = tile + t;
     if (r < r_end) {
@@ -904,15 +945,34 @@ __device__ __forceinline__ void direct_f64_item(
     }
     __syncthreads();
     const int cnt = (int)((r_end - tile) < kTile64 ? (r_end - tile) : kTile64);
-    for (int j = 0; j < cnt; ++j) {
-      const double4 a = s0[j];
-      const double4 b = (kNf == 2) ? s1[j] : a;
-      const double v = pair_term_f64<FAM>(px, py, a, b, rho);
-      if (!(self_excl && tile + j == q)) sum += v;
+    const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
+    if (!diag) {
+#pragma unroll 2
+      for (int j = 0; j < cnt; ++j) {
+        const double4 a = s0[j];
+        const double4 b = (kNf == 2) ? s1[j] : a;
+#pragma unroll
+        for (int k = 0; k < kQpt; ++k) acc[k] += pair_term_f64<FAM>(px[k], py[k], a, b, rho);
+      }
+    } else {
+      for (int j = 0; j < cnt; ++j) {
+        const double4 a = s0[j];
+        const double4 b = (kNf == 2) ? s1[j] : a;
+        const long long rj = tile + j;
+#pragma unroll
+        for (int k = 0; k < kQpt; ++k) {
+          const double v = pair_term_f64<FAM>(px[k], py[k], a, b, rho);
+          acc[k] += (rj == q_begin + t + k * kBlock64) ? 0.0 : v;
+        }
+      }
     }
     __syncthreads();
   }
-  if (valid) dst[t] = sum * out_scale;
+#pragma unroll
+  for (int k = 0; k < kQpt; ++k) {
+    const int qi = t + k * kBlock64;
+    if (qi < q_count) dst[qi] = acc[k] * out_scale;
+  }
 }
 
 template <int FAM, bool BOUNDED>
@@ -922,12 +982,13 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
     long long r_i, long long r_end, long long split_len, long long q_i,
     long long nq, double *__restrict__ dst, double out_scale, int self_excl,
     double rho, BoundArgs ba) {
-  const long long qb = (long long)blockIdx.x * kBlock64;
+  constexpr int kQb = kBlock64 * kQpt;
+  const long long qb = (long long)blockIdx.x * kQb;
   const long long rb = r_i + (long long)blockIdx.y * split_len;
   long long re = rb + split_len;
   if (re > r_end) re = r_end;
   const long long left = nq - qb;
-  const int q_count = (int)(left < kBlock64 ? left : kBlock64);
+  const int q_count = (int)(left < kQb ? left : kQb);
   const long long off = (long long)blockIdx.y * nq + qb;
   direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
                                 BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
@@ -1679,7 +1740,7 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
     if (nq <= 0) return hipSuccess;
     long long len = (r_end - r_i + splits - 1) / splits;
     len = (len + kTile64 - 1) / kTile64 * kTile64;
-    const dim3 grid((unsigned)((nq + kBlock64 - 1) / kBlock64), (unsigned)splits);
+    const dim3 grid((unsigned)((nq + kBlock64 * kQpt - 1) / (kBlock64 * kQpt)), (unsigned)splits);
     const BoundArgs b = ba ? *ba : BoundArgs{};
 #define BK_R64(F, B)                                                          \
   direct_f64_range_kernel<F, B><<<grid, kBlock64, 0, s>>>(                    \
