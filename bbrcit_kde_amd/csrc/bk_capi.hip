@@ -287,6 +287,16 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
       ps.m_max = std::max(ps.m_max, m[i]);
       ps.m_sum += m[i];
     }
+    if (dim <= 2) {
+      // square for spatial sorting, in FP32 (the Morton cells only group points)
+      float lo = (float)x[0], hi = lo;
+      for (size_t i = 0; i < n; ++i) {
+        lo = std::min(lo, std::min((float)x[i], (float)y[i]));
+        hi = std::max(hi, std::max((float)x[i], (float)y[i]));
+      }
+      ps.box_lo = std::isfinite(lo) ? lo : -1.0f;
+      ps.box_hi = std::isfinite(hi) && hi > ps.box_lo ? hi : ps.box_lo + 1.0f;
+    }
     for (DevBuf *b : {&ps.xd, &ps.yd, &ps.md, &ps.abwd}) b->reserve(n * sizeof(double));
     hip_check(hipMemcpyAsync(ps.xd.ptr, x.data(), n * 8, hipMemcpyHostToDevice, s), "upload x");
     hip_check(hipMemcpyAsync(ps.yd.ptr, y.data(), n * 8, hipMemcpyHostToDevice, s), "upload y");
@@ -393,8 +403,10 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
     case Path::F64:
       ctx->f0.reserve(n * sizeof(double4));
       ctx->f1.reserve(n * sizeof(double4));
+      ctx->tiles.reserve((n + bk::kTile - 1) / bk::kTile * 2 * sizeof(double4));
       hip_check(bk::launch_prep_f64(p, r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
-                                    r.abwd.as<double>(), n, ctx->f0.as<double4>(), ctx->f1.as<double4>(), s),
+                                    r.abwd.as<double>(), n, ctx->f0.as<double4>(), ctx->f1.as<double4>(),
+                                    ctx->tiles.as<double4>(), s),
                 "prep_f64 launch");
       break;
     case Path::GAUSS32: {
@@ -502,7 +514,7 @@ hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkIt
   switch (path) {
     case Path::F64:
       return bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
-                                   ctx->f1.as<double4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
+                                   ctx->f1.as<double4>(), ctx->tiles.as<double4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
                                    self_excl, s);
     case Path::GAUSS32:
       return bk::launch_gauss(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
@@ -539,23 +551,19 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
       path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
   const int splits = choose_splits(nq, nr, queries_per_item(path), tile_of(path), slots);
   const bool spatial = ctx->query_order == BK_QUERY_ORDER_SPATIAL && !self_excl &&
-                       (path == Path::GAUSS32 || path == Path::EPAN64) && nq > queries_per_item(path) &&
-                       nq < (1LL << 32);
+                       (path == Path::GAUSS32 || path == Path::EPAN64 || path == Path::F64) &&
+                       nq > queries_per_item(path) && nq < (1LL << 32);
   if (spatial) {
     // evaluate the queries in Z-curve order, then scatter back: each workgroup
     // holds nearby queries and skips the reference tiles out of their reach
     const hipStream_t s = ctx->stream;
     const PointSet &qs = query_set(ctx);
-    const float *qx = qs.x.as<float>() + q_i, *qy = qs.y.as<float>() + q_i;
+    const size_t csize = path == Path::F64 ? sizeof(double) : sizeof(float);
     ctx->ord_list.reserve((size_t)nq * sizeof(long long));
-    ctx->ord_x.reserve((size_t)nq * sizeof(float));
-    ctx->ord_y.reserve((size_t)nq * sizeof(float));
+    ctx->ord_x.reserve((size_t)nq * csize);
+    ctx->ord_y.reserve((size_t)nq * csize);
     ctx->ord_out.reserve((size_t)nq * sizeof(double));
     long long *list = ctx->ord_list.as<long long>();
-    bk::spatial_sort(qx, qy, nullptr, nq, nq, qs.box_lo, qs.box_hi, list, ctx->ord_keys_a, ctx->ord_keys_b,
-                     ctx->ord_tmp, s);
-    float *ox = ctx->ord_x.as<float>(), *oy = ctx->ord_y.as<float>();
-    hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
     double *res = ctx->ord_out.as<double>();
     double *dst = res;
     if (splits > 1) {
@@ -563,6 +571,24 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
       dst = ctx->partial.as<double>();
     }
     hipError_t e;
+    if (path == Path::F64) {
+      const double *qx = qs.xd.as<double>() + q_i, *qy = qs.yd.as<double>() + q_i;
+      bk::spatial_sort(qx, qy, nullptr, nq, nq, qs.box_lo, qs.box_hi, list, ctx->ord_keys_a, ctx->ord_keys_b,
+                       ctx->ord_tmp, s);
+      double *ox = ctx->ord_x.as<double>(), *oy = ctx->ord_y.as<double>();
+      hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
+      e = bk::launch_direct_f64(p, ox, oy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
+                                nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s);
+      hip_check(e, "direct kernel launch");
+      if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, res, s), "reduce launch");
+      hip_check(bk::launch_scatter_f64(list, nq, res, out, s), "scatter launch");
+      return;
+    }
+    const float *qx = qs.x.as<float>() + q_i, *qy = qs.y.as<float>() + q_i;
+    bk::spatial_sort(qx, qy, nullptr, nq, nq, qs.box_lo, qs.box_hi, list, ctx->ord_keys_a, ctx->ord_keys_b,
+                     ctx->ord_tmp, s);
+    float *ox = ctx->ord_x.as<float>(), *oy = ctx->ord_y.as<float>();
+    hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
     if (path == Path::GAUSS32)
       e = bk::launch_gauss(p, ox, oy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
                            0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0,
@@ -648,13 +674,25 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   hip_check(hipMemsetAsync(ctx->bnd_counts.ptr, 0, 4 * sizeof(unsigned long long), s), "clear counters");
   const PointSet &qs = query_set(ctx);
   // queries: as stored, or gathered in Z-curve order (FP32 contexts)
-  const bool spatial =
-      ctx->query_order == BK_QUERY_ORDER_SPATIAL && path != Path::F64 && nq > qpb && nq < (1LL << 32);
+  const bool spatial = ctx->query_order == BK_QUERY_ORDER_SPATIAL && nq > qpb && nq < (1LL << 32);
   const float *qx32 = nullptr, *qy32 = nullptr;
   const double *qx64 = nullptr, *qy64 = nullptr;
   long long *list = nullptr;
   long long q_base = (long long)q_i;
-  if (path == Path::F64) {
+  if (path == Path::F64 && spatial) {
+    const double *bx = qs.xd.as<double>() + q_i, *by = qs.yd.as<double>() + q_i;
+    ctx->ord_list.reserve((size_t)nq * sizeof(long long));
+    ctx->ord_x.reserve((size_t)nq * sizeof(double));
+    ctx->ord_y.reserve((size_t)nq * sizeof(double));
+    list = ctx->ord_list.as<long long>();
+    bk::spatial_sort(bx, by, nullptr, nq, nq, qs.box_lo, qs.box_hi, list, ctx->ord_keys_a, ctx->ord_keys_b,
+                     ctx->ord_tmp, s);
+    hip_check(bk::launch_gather_xy(bx, by, list, nq, ctx->ord_x.as<double>(), ctx->ord_y.as<double>(), s),
+              "gather launch");
+    qx64 = ctx->ord_x.as<double>();
+    qy64 = ctx->ord_y.as<double>();
+    q_base = 0;
+  } else if (path == Path::F64) {
     qx64 = qs.xd.as<double>();
     qy64 = qs.yd.as<double>();
   } else if (spatial) {
@@ -714,8 +752,9 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
       e = bk::launch_epan(p, qx32, qy32, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
                           nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, s, &ba);
     else
-      e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(), nullptr, 0, 0, nr,
-                                q_off, cn, splits, dst, scale, 0, s, &ba);
+      e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(),
+                                ctx->tiles.as<double4>(), nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, s,
+                                &ba);
     hip_check(e, "bounded kernel launch");
     if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, cn, exact + c0, s), "reduce launch");
     hip_check(bk::launch_bounds_combine(exact + c0, lo_b, gap_b, cn, qpb, factor, lo_all + c0, hi_all + c0, s),

@@ -145,10 +145,12 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
 hipError_t launch_prep_f64(const EvalParams &p, const double *x,
                            const double *y, const double *m,
                            const double *abw, size_t n, double4 *f0,
-                           double4 *f1, hipStream_t s);
+                           double4 *f1, double4 *tb, hipStream_t s);
+// tb: per aligned tile of kTile references, the box and smallest scales
+// written by launch_prep_f64 (tile culling).
 hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              const double *qy, const double4 *f0,
-                             const double4 *f1, const WorkItem *items,
+                             const double4 *f1, const double4 *tb, const WorkItem *items,
                              int n_items, long long r_i, long long r_end,
                              long long q_i, long long nq, int splits,
                              double *dst, double out_scale, int self_excl,
@@ -221,9 +223,14 @@ struct DevBuf;  // bk_common.h
 void spatial_sort(const float *qx, const float *qy, const long long *list, long long n, long long index_limit,
                   float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
                   hipStream_t s);
+void spatial_sort(const double *qx, const double *qy, const long long *list, long long n, long long index_limit,
+                  float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
+                  hipStream_t s);
 // gx[k] = qx[list[k]], gy[k] = qy[list[k]] for k < n.
 hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
                             float *gy, hipStream_t s);
+hipError_t launch_gather_xy(const double *qx, const double *qy, const long long *list, long long n, double *gx,
+                            double *gy, hipStream_t s);
 // out[list[k]] = vals[k] for k < n.
 hipError_t launch_scatter_f64(const long long *list, long long n, const double *vals, double *out, hipStream_t s);
 

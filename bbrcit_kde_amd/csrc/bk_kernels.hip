@@ -118,6 +118,52 @@ __device__ __forceinline__ void block_query_box(const T *px, const T *py, float 
   }
 }
 
+__device__ __forceinline__ double wave_reduce_d(double v, bool is_max) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(v, o, 64);
+    v = is_max ? __builtin_fmax(v, u) : __builtin_fmin(v, u);
+  }
+  return v;
+}
+
+// The same box in FP64 for FP64 queries (exact minima and maxima).
+template <int NQ>
+__device__ __forceinline__ void block_query_box_d(const double *px, const double *py,
+                                                  double (&sbox)[4][kBlock / 64], double &bx0, double &bx1,
+                                                  double &by0, double &by1) {
+  const int t = threadIdx.x;
+  double x0 = px[0], x1 = x0, y0 = py[0], y1 = y0;
+#pragma unroll
+  for (int k = 1; k < NQ; ++k) {
+    x0 = __builtin_fmin(x0, px[k]);
+    x1 = __builtin_fmax(x1, px[k]);
+    y0 = __builtin_fmin(y0, py[k]);
+    y1 = __builtin_fmax(y1, py[k]);
+  }
+  x0 = wave_reduce_d(x0, false);
+  x1 = wave_reduce_d(x1, true);
+  y0 = wave_reduce_d(y0, false);
+  y1 = wave_reduce_d(y1, true);
+  if ((t & 63) == 0) {
+    sbox[0][t >> 6] = x0;
+    sbox[1][t >> 6] = x1;
+    sbox[2][t >> 6] = y0;
+    sbox[3][t >> 6] = y1;
+  }
+  __syncthreads();
+  bx0 = sbox[0][0];
+  bx1 = sbox[1][0];
+  by0 = sbox[2][0];
+  by1 = sbox[3][0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) {
+    bx0 = __builtin_fmin(bx0, sbox[0][w]);
+    bx1 = __builtin_fmax(bx1, sbox[1][w]);
+    by0 = __builtin_fmin(by0, sbox[2][w]);
+    by1 = __builtin_fmax(by1, sbox[3][w]);
+  }
+}
+
 // ---- bounds of a tile's contribution (bounded evaluation) ---------------------
 // U at per-axis distances dx, dy >= 0 with bandwidth factor a, in the form the
 // evaluation kernels use (kx, ky: gx, gy of FAM_GAUSS / FAM_EPAN /
@@ -908,13 +954,42 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   }
 }
 
+// Per aligned tile of kTile64 FP64 references, two double4: the box (x0, x1,
+// y0, y1) and the smallest scales (min a.z, min a.w of pair_term_f64's
+// layout; a.w only for FAM_EPANPROD and FAM_EPANCONV).  A tile is culled for
+// a workgroup when its terms are exactly zero (Epanechnikov kinds) or below
+// 2^-kFarExp of their masses (Gaussian) for the whole box of its queries.
+// The test runs the term's own FP64 operations on the per-axis gaps between
+// the boxes with the tile's smallest scales; each is monotone, so the
+// Epanechnikov terms of a culled tile are zero as computed.
+template <int FAM>
+__device__ __forceinline__ bool f64_tile_far(const double4 *__restrict__ tb, long long tile_idx, double bx0,
+                                             double bx1, double by0, double by1, double rho) {
+  const double4 b = tb[2 * tile_idx];
+  const double4 w = tb[2 * tile_idx + 1];
+  const double gx = fmax(0.0, fmax(b.x - bx1, bx0 - b.y));
+  const double gy = fmax(0.0, fmax(b.z - by1, by0 - b.w));
+  if constexpr (FAM == FAM_GAUSS) {
+    return w.x * __builtin_fma(rho * gy, gy, gx * gx) >= (double)kFarExp;
+  } else if constexpr (FAM == FAM_EPAN) {
+    return __builtin_fma(-w.x, __builtin_fma(gy, gy, gx * gx), 1.0) <= 0.0;
+  } else if constexpr (FAM == FAM_EPANPROD) {
+    return __builtin_fma(-(w.x * gx), gx, 1.0) <= 0.0 || __builtin_fma(-(w.y * gy), gy, 1.0) <= 0.0;
+  } else {
+    return gx * w.x >= 2.0 || gy * w.y >= 2.0;
+  }
+}
+
 // kQpt queries per thread (kBlock64 * kQpt per workgroup), each reference
 // read once from LDS for all of them; the self pair is excluded only on
-// tiles that hold the workgroup's own queries.
+// tiles that hold the workgroup's own queries.  Reference tiles are aligned
+// to kTile64 (entries before r_begin are loaded as zero masses), and tiles out
+// of reach of the workgroup's query box are skipped (tb) or, when BOUNDED,
+// only the tiles listed in bmask are summed.
 template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void direct_f64_item(
     const double *__restrict__ qx, const double *__restrict__ qy,
-    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_begin, long long r_end, long long q_begin, int q_count,
     double *__restrict__ dst, double out_scale, int self_excl, double rho,
     const unsigned long long *__restrict__ bmask) {
@@ -922,7 +997,12 @@ __device__ __forceinline__ void direct_f64_item(
   static_assert(kBlock64 == kBlock && kTile64 == kTile, "workgroup geometry of the tile summaries and masks");
   __shared__ double4 s0[kTile64];
   __shared__ double4 s1[kNf == 2 ? kTile64 : 1];
+  __shared__ double sbox[4][kBlock / 64];
   const int t = threadIdx.x;
+  if (r_begin >= r_end) {
+    write_zeros(dst, q_count);
+    return;
+  }
   double px[kQpt], py[kQpt], acc[kQpt];
 #pragma unroll
   for (int k = 0; k < kQpt; ++k) {
@@ -932,17 +1012,22 @@ __device__ __forceinline__ void direct_f64_item(
     py[k] = qy[q];
     acc[k] = 0.0;
   }
+  double bx0, bx1, by0, by1;
+  block_query_box_d<kQpt>(px, py, sbox, bx0, bx1, by0, by1);
   auto live_from = [&](long long tl) {
     if constexpr (BOUNDED) return next_listed_tile<kTile64>(bmask, tl, r_end);
+    for (; tl < r_end; tl += kTile64)
+      if (!f64_tile_far<FAM>(tb, tl / kTile64, bx0, bx1, by0, by1, rho)) break;
     return tl;
   };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
-  for (long long tile = live_from(r_begin); tile < r_end; tile = live_from(tile + kTile64)) {
+  const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
+  for (long long tile = live_from((r_begin / kTile64) * kTile64); tile < r_end;
+       tile = live_from(tile + kTile64)) {
     const long long r = tile + t;
-    if (r < r_end) {
-      s0[t] = f0[r];
-      if constexpr (kNf == 2) s1[t] = f1[r];
-    }
+    const bool in = r >= r_begin && r < r_end;
+    s0[t] = in ? f0[r] : zero;
+    if constexpr (kNf == 2) s1[t] = in ? f1[r] : zero;
     __syncthreads();
     const int cnt = (int)((r_end - tile) < kTile64 ? (r_end - tile) : kTile64);
     const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
@@ -978,7 +1063,7 @@ __device__ __forceinline__ void direct_f64_item(
 template <int FAM, bool BOUNDED>
 __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
     const double *__restrict__ qx, const double *__restrict__ qy,
-    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_i, long long r_end, long long split_len, long long q_i,
     long long nq, double *__restrict__ dst, double out_scale, int self_excl,
     double rho, BoundArgs ba) {
@@ -990,19 +1075,20 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
   const long long left = nq - qb;
   const int q_count = (int)(left < kQb ? left : kQb);
   const long long off = (long long)blockIdx.y * nq + qb;
-  direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
+  direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
+                                rho,
                                 BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
 }
 
 template <int FAM>
 __global__ __launch_bounds__(kBlock64) void direct_f64_list_kernel(
     const double *__restrict__ qx, const double *__restrict__ qy,
-    const double4 *__restrict__ f0, const double4 *__restrict__ f1,
+    const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     const WorkItem *__restrict__ items, double *__restrict__ dst,
     double out_scale, int self_excl, double rho) {
   const WorkItem it = items[blockIdx.x];
-  direct_f64_item<FAM, false>(qx, qy, f0, f1, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
-                              out_scale, self_excl, rho, nullptr);
+  direct_f64_item<FAM, false>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count,
+                              dst + it.out_index, out_scale, self_excl, rho, nullptr);
 }
 
 __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
@@ -1030,6 +1116,44 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
       f0[i] = make_double4(x[i], y[i], p.ix / a, p.iy / a);
       f1[i] = make_double4(m[i] * p.mass_fold, 0.0, 0.0, 0.0);
       break;
+  }
+}
+
+// One workgroup of kTile64 threads per aligned tile: the tile's FP64 box and
+// smallest scales (see f64_tile_far).
+__global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *__restrict__ x,
+                                                                  const double *__restrict__ y,
+                                                                  const double4 *__restrict__ f0, long long n,
+                                                                  double4 *__restrict__ tb) {
+  __shared__ double sh[kTile64];
+  const long long i = (long long)blockIdx.x * kTile64 + threadIdx.x;
+  const bool valid = i < n;
+  const double big = 1.0e308;
+  const double4 a = valid ? f0[i] : make_double4(0.0, 0.0, big, big);
+  const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
+  auto reduce = [&](double v, bool is_max) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = kTile64 / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        const double u = sh[threadIdx.x + o], w = sh[threadIdx.x];
+        sh[threadIdx.x] = is_max ? __builtin_fmax(w, u) : __builtin_fmin(w, u);
+      }
+      __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+  };
+  const double x0 = reduce(valid ? xi : big, false);
+  const double x1 = reduce(valid ? xi : -big, true);
+  const double y0 = reduce(valid ? yi : big, false);
+  const double y1 = reduce(valid ? yi : -big, true);
+  const double wz = reduce(a.z, false);
+  const double ww = reduce(a.w, false);
+  if (threadIdx.x == 0) {
+    tb[2 * blockIdx.x] = make_double4(x0, x1, y0, y1);
+    tb[2 * blockIdx.x + 1] = make_double4(wz, ww, 0.0, 0.0);
   }
 }
 
@@ -1708,17 +1832,20 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
 hipError_t launch_prep_f64(const EvalParams &p, const double *x,
                            const double *y, const double *m,
                            const double *abw, size_t n, double4 *f0,
-                           double4 *f1, hipStream_t s) {
+                           double4 *f1, double4 *tb, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int th = 256;
   prep_f64_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(
       p, x, y, m, abw, (long long)n, f0, f1);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  prep_f64_tiles_kernel<<<(unsigned)((n + kTile64 - 1) / kTile64), kTile64, 0, s>>>(x, y, f0, (long long)n, tb);
   return hipGetLastError();
 }
 
 hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              const double *qy, const double4 *f0,
-                             const double4 *f1, const WorkItem *items,
+                             const double4 *f1, const double4 *tb, const WorkItem *items,
                              int n_items, long long r_i, long long r_end,
                              long long q_i, long long nq, int splits,
                              double *dst, double out_scale, int self_excl,
@@ -1728,7 +1855,7 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
     if (n_items <= 0) return hipSuccess;
 #define BK_L64(F)                                                             \
   direct_f64_list_kernel<F><<<n_items, kBlock64, 0, s>>>(                     \
-      qx, qy, f0, f1, items, dst, out_scale, self_excl, rho)
+      qx, qy, f0, f1, tb, items, dst, out_scale, self_excl, rho)
     switch (p.family) {
       case FAM_GAUSS: BK_L64(FAM_GAUSS); break;
       case FAM_EPAN: BK_L64(FAM_EPAN); break;
@@ -1744,7 +1871,7 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
     const BoundArgs b = ba ? *ba : BoundArgs{};
 #define BK_R64(F, B)                                                          \
   direct_f64_range_kernel<F, B><<<grid, kBlock64, 0, s>>>(                    \
-      qx, qy, f0, f1, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho, b)
+      qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho, b)
     if (ba) {
       switch (p.family) {
         case FAM_GAUSS: BK_R64(FAM_GAUSS, true); break;

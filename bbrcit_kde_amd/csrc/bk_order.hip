@@ -27,7 +27,10 @@ __device__ __forceinline__ unsigned int spread_bits(unsigned int v) {
   return v;
 }
 
-__global__ void morton_keys_kernel(const float *__restrict__ qx, const float *__restrict__ qy,
+// Keys of FP32 or FP64 coordinates (the cell of an FP64 point is found from
+// its FP32 rounding: the order only has to group nearby points).
+template <typename T>
+__global__ void morton_keys_kernel(const T *__restrict__ qx, const T *__restrict__ qy,
                                    const long long *__restrict__ list, long long n, float lo, float scale,
                                    int index_bits, unsigned long long *__restrict__ keys) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -35,8 +38,8 @@ __global__ void morton_keys_kernel(const float *__restrict__ qx, const float *__
   const long long i = list ? list[k] : k;
   constexpr float top = (float)((1 << kMortonBits) - 1);
   // fminf/fmaxf return the number when the other operand is NaN
-  const float fx = __builtin_fminf(__builtin_fmaxf((qx[i] - lo) * scale, 0.0f), top);
-  const float fy = __builtin_fminf(__builtin_fmaxf((qy[i] - lo) * scale, 0.0f), top);
+  const float fx = __builtin_fminf(__builtin_fmaxf(((float)qx[i] - lo) * scale, 0.0f), top);
+  const float fy = __builtin_fminf(__builtin_fmaxf(((float)qy[i] - lo) * scale, 0.0f), top);
   const unsigned int code = spread_bits((unsigned int)fx) | (spread_bits((unsigned int)fy) << 1);
   keys[k] = ((unsigned long long)code << index_bits) | (unsigned long long)i;
 }
@@ -47,9 +50,10 @@ __global__ void keys_to_list_kernel(const unsigned long long *__restrict__ keys,
   if (k < n) list[k] = (long long)(keys[k] & mask);
 }
 
-__global__ void gather_xy_kernel(const float *__restrict__ qx, const float *__restrict__ qy,
-                                 const long long *__restrict__ list, long long n, float *__restrict__ gx,
-                                 float *__restrict__ gy) {
+template <typename T>
+__global__ void gather_xy_kernel(const T *__restrict__ qx, const T *__restrict__ qy,
+                                 const long long *__restrict__ list, long long n, T *__restrict__ gx,
+                                 T *__restrict__ gy) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const long long i = list[k];
@@ -68,7 +72,14 @@ __global__ void scatter_f64_kernel(const long long *__restrict__ list, long long
 hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
                             float *gy, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  gather_xy_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(qx, qy, list, n, gx, gy);
+  gather_xy_kernel<float><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(qx, qy, list, n, gx, gy);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_xy(const double *qx, const double *qy, const long long *list, long long n, double *gx,
+                            double *gy, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  gather_xy_kernel<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(qx, qy, list, n, gx, gy);
   return hipGetLastError();
 }
 
@@ -78,9 +89,11 @@ hipError_t launch_scatter_f64(const long long *list, long long n, const double *
   return hipGetLastError();
 }
 
-void spatial_sort(const float *qx, const float *qy, const long long *list, long long n, long long index_limit,
-                  float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
-                  hipStream_t s) {
+namespace {
+template <typename T>
+void spatial_sort_impl(const T *qx, const T *qy, const long long *list, long long n, long long index_limit,
+                       float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
+                       hipStream_t s) {
   if (n <= 0) return;
   if (!list) index_limit = n;
   if (index_limit < n || index_limit > (1LL << 32))
@@ -99,7 +112,8 @@ void spatial_sort(const float *qx, const float *qy, const long long *list, long 
   tmp.reserve(tmp_bytes + 256);
   const unsigned grid = (unsigned)((n + 255) / 256);
   const float scale = (float)((double)(1 << kMortonBits) / ((double)hi - (double)lo));
-  morton_keys_kernel<<<grid, 256, 0, s>>>(qx, qy, list, n, lo, scale, index_bits, keys_a.as<unsigned long long>());
+  morton_keys_kernel<T>
+      <<<grid, 256, 0, s>>>(qx, qy, list, n, lo, scale, index_bits, keys_a.as<unsigned long long>());
   hip_check(hipGetLastError(), "morton key launch");
   hip_check(hipcub::DeviceRadixSort::SortKeys(tmp.ptr, tmp_bytes, keys_a.as<unsigned long long>(),
                                               keys_b.as<unsigned long long>(), (int)n, 0, end_bit, s),
@@ -107,6 +121,19 @@ void spatial_sort(const float *qx, const float *qy, const long long *list, long 
   keys_to_list_kernel<<<grid, 256, 0, s>>>(keys_b.as<unsigned long long>(), n, (1ULL << index_bits) - 1ULL,
                                            list_out);
   hip_check(hipGetLastError(), "key unpack launch");
+}
+}  // namespace
+
+void spatial_sort(const float *qx, const float *qy, const long long *list, long long n, long long index_limit,
+                  float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
+                  hipStream_t s) {
+  spatial_sort_impl<float>(qx, qy, list, n, index_limit, lo, hi, list_out, keys_a, keys_b, tmp, s);
+}
+
+void spatial_sort(const double *qx, const double *qy, const long long *list, long long n, long long index_limit,
+                  float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
+                  hipStream_t s) {
+  spatial_sort_impl<double>(qx, qy, list, n, index_limit, lo, hi, list_out, keys_a, keys_b, tmp, s);
 }
 
 }  // namespace bk

@@ -83,7 +83,7 @@ extern "C" {
                                   /* in the caller's order):                        */
 #define BK_QUERY_ORDER_GIVEN   0  /*   as stored (default)                          */
 #define BK_QUERY_ORDER_SPATIAL 1  /*   sorted along a Z curve on the device first   */
-                                  /*   (FP32 contexts, D <= 2), so each workgroup   */
+                                  /*   (D <= 2), so each workgroup                  */
                                   /*   holds nearby queries and skips the reference */
                                   /*   tiles beyond the kernel's reach              */
 

@@ -103,18 +103,21 @@ def test_zero_tolerance_is_exact(bk, oracle, kind, h):
         assert st["approximated"] == 0
 
 
-def test_spatial_order_and_sub_range(bk, oracle):
+@pytest.mark.parametrize("f64", [False, True])
+def test_spatial_order_and_sub_range(bk, oracle, f64):
     # random-order queries, sorted on the device; a sub-range of them
     pts, mass = kd_points(20_000, 51, 2)
     q = ds.uniform_box(7_000, -4.0, 4.0, 52)
     h = (0.04, 0.06)
     desc = bk.KernelDesc(GAUSSPROD, 2, *h)
-    with bk.DirectKde(pts, q, mass) as kde:
+    prec = bk.BK_PREC_F64 if f64 else bk.BK_PREC_F32
+    with bk.DirectKde(pts, q, mass, precision=prec) as kde:
         kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
         lo, up, st = kde.eval_bounded(desc, 1000, 6500, 1e-3, 1e-6)
         lo2, up2, _ = kde.eval_bounded(desc, 1000, 6500, 1e-3, 1e-6)
-    exact = oracle.direct(GAUSSPROD, h[0], h[1], f32(pts), f32(q[1000:6501]), mass=f32(mass), normalize=True)
-    check_bounds(lo, up, exact, oracle.normalization(GAUSSPROD, 2, *h), 1e-3, 1e-6, False, "spatial")
+    P, Q, M = (pts, q, mass) if f64 else (f32(pts), f32(q), f32(mass))
+    exact = oracle.direct(GAUSSPROD, h[0], h[1], P, Q[1000:6501], mass=M, normalize=True)
+    check_bounds(lo, up, exact, oracle.normalization(GAUSSPROD, 2, *h), 1e-3, 1e-6, f64, f"spatial f64 {f64}")
     np.testing.assert_array_equal(lo, lo2)
     np.testing.assert_array_equal(up, up2)
     assert st["approximated"] + st["excluded"] > 0

@@ -117,6 +117,37 @@ def test_epanechnikov_culling_far_from_origin(bk, oracle, kind, h):
     assert (want > 0).mean() > 0.05
 
 
+@pytest.mark.parametrize("kind,h", CASES)
+def test_f64_spatial_order_and_culling(bk, oracle, kind, h):
+    # FP64 contexts: the same culling on FP64 boxes (Epanechnikov terms of a
+    # culled tile are exactly zero, Gaussian ones below 2^-100 of their mass),
+    # held to the FP64 bar of 1e-12 in both orders
+    n_ref, n_q = 20_000, 7_000
+    pts, mass = kd_mixture(n_ref, 900 + kind)
+    q = ds.uniform_box(n_q, -4.0, 4.0, 950 + kind)
+    desc = bk.KernelDesc(kind, 2, *h)
+    with bk.DirectKde(pts, q, mass, precision=bk.BK_PREC_F64) as kde:
+        given = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_GIVEN)
+        spatial = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_SPATIAL)
+    want = oracle.direct(kind, h[0], h[1], pts, q, mass=mass, normalize=True)
+    bound = oracle.normalization(kind, 2, *h) * mass.sum()
+    assert_parity(spatial, want, bound, rtol=1e-12, what=f"f64 spatial kind {kind}")
+    assert_parity(given, want, bound, rtol=1e-12, what=f"f64 given kind {kind}")
+    if kind in EPAN_KINDS:
+        np.testing.assert_array_equal(spatial, given)
+    assert (want > 0).mean() > 0.05
+
+
+@pytest.mark.parametrize("kind,h", [(GAUSS, (0.02, 0.02)), (EPANPROD, (0.04, 0.05))])
+def test_f64_self_eval_culls_in_tree_order(bk, oracle, kind, h):
+    pts, mass = kd_mixture(30_000, 310 + kind)
+    desc = bk.KernelDesc(kind, 2, *h)
+    with bk.DirectKde(pts, None, mass, precision=bk.BK_PREC_F64) as kde:
+        got = kde.self_eval(desc, exclude_self=True)
+    want = oracle.direct(kind, h[0], h[1], pts, pts, mass=mass, exclude_self=True, normalize=True)
+    assert_parity(got, want, oracle.normalization(kind, 2, *h) * mass.sum(), rtol=1e-12, what=f"f64 loo {kind}")
+
+
 def test_query_order_option_is_checked(bk):
     pts, mass = kd_mixture(1_000, 5)
     with bk.DirectKde(pts, pts, mass) as kde:
