@@ -115,9 +115,9 @@ __device__ __forceinline__ void gauss_stage(const float4 *__restrict__ tl, float
 
 // ---- Epanechnikov product kernel and its convolution kernel (FP64) --------
 // ux = max(1 - u^2, 0) cancels near the support edge, which FP32 cannot
-// resolve to 1e-5 relative; gfx950 runs FP64 FMA at the (non-packed) FP32
-// rate, so these sums are FP64 throughout (exact differences of the FP32
-// points, FP64 accumulators, no stage split).  The convolution factor
+// resolve to 1e-5 relative, so these sums are FP64 throughout (exact
+// differences of the FP32 points, FP64 accumulators, no stage split; an FP64
+// instruction costs gfx950 about 1.8 FP32 ones).  The convolution factor
 // (2-u)^3 (u^2 + 6u + 4) for u < 2 carries 32 times the reference's k(u);
 // the 1/1024 of both axes is applied in the score reduction.
 template <int A, int B, bool ADAPT, bool DIAG>

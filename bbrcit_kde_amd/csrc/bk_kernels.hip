@@ -22,8 +22,8 @@
 //     and then added into an FP64 accumulator (block compensation);
 //   * Epanechnikov kinds (FP64 arithmetic on the FP32 points): their
 //     compact support makes 1 - d2/(h a)^2 cancel near the support edge,
-//     which FP32 cannot resolve to 1e-5; MI355X runs FP64 VALU at the FP32
-//     (non-packed) rate, so exact differences cost few extra instructions;
+//     which FP32 cannot resolve to 1e-5; each reference carries its scale
+//     s = 1/(h a) and offset -s r per axis, so u = s (q - r) is one FP64 FMA;
 //   * large reference ranges are split over workgroups at tile boundaries,
 //     with a fixed-order FP64 reduction of the partial sums, so results are
 //     deterministic.
