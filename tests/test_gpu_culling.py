@@ -7,9 +7,9 @@ with and without the spatial order.  For the Gaussian kinds the skipped
 terms of the factored tiles are zero too; those of the difference and Gram
 forms are below 2^-100 of the largest mass, so both orders must meet the
 oracle at the parity bar (relative 1e-5 above the floor of 1e-10 times an
-upper bound of the sum, absolute below it).  The cases use kd-ordered
-references and small bandwidths, where most tiles are out of reach of a
-workgroup of sorted queries.
+upper bound of the sum, absolute below it; 1e-12 for FP64 contexts).  The
+cases use kd-ordered references and small bandwidths, where most tiles are
+out of reach of a workgroup of sorted queries.
 """
 from __future__ import annotations
 
@@ -61,18 +61,26 @@ def test_spatial_order_matches_oracle(bk, oracle, kind, h, adaptive):
     assert (want > 0).mean() > 0.05  # the case has queries inside the support
 
 
-def test_spatial_order_sub_range_and_tiny_sets(bk, oracle):
+@pytest.mark.parametrize("f64", [False, True])
+@pytest.mark.parametrize("kind", [GAUSS, EPANPROD])
+def test_spatial_order_sub_range_and_tiny_sets(bk, oracle, kind, f64):
+    # unaligned reference ranges start inside a culling tile
     pts, mass = kd_mixture(5_000, 77)
     q = ds.uniform_box(3_000, -3.0, 3.0, 78)
-    h = 0.04
-    desc = bk.KernelDesc(GAUSS, 2, h)
-    P, Q, M = f32(pts), f32(q), f32(mass)
-    with bk.DirectKde(pts, q, mass) as kde:
-        kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
-        for (r_i, r_j, q_i, q_j) in [(100, 4321, 250, 2999), (0, 4999, 0, 0), (17, 17, 5, 1400), (0, 4999, 10, 300)]:
-            got = kde.unnormalized_eval(desc, r_i, r_j, q_i, q_j)
-            want = oracle.direct(GAUSS, h, h, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
-            assert_parity(got, want, M[r_i:r_j + 1].sum(), what=f"range {(r_i, r_j, q_i, q_j)}")
+    h = (0.04, 0.04) if kind == GAUSS else (0.06, 0.09)
+    desc = bk.KernelDesc(kind, 2, *h)
+    P, Q, M = (pts, q, mass) if f64 else (f32(pts), f32(q), f32(mass))
+    rtol = 1e-12 if f64 else 1e-5
+    prec = bk.BK_PREC_F64 if f64 else bk.BK_PREC_F32
+    with bk.DirectKde(pts, q, mass, precision=prec) as kde:
+        for order in (bk.BK_QUERY_ORDER_SPATIAL, bk.BK_QUERY_ORDER_GIVEN):
+            kde.set_option(bk.BK_OPT_QUERY_ORDER, order)
+            for (r_i, r_j, q_i, q_j) in [(100, 4321, 250, 2999), (0, 4999, 0, 0), (17, 17, 5, 1400),
+                                         (0, 4999, 10, 300), (300, 2900, 0, 2999)]:
+                got = kde.unnormalized_eval(desc, r_i, r_j, q_i, q_j)
+                want = oracle.direct(kind, h[0], h[1], P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
+                assert_parity(got, want, M[r_i:r_j + 1].sum(), rtol=rtol,
+                              what=f"order {order} range {(r_i, r_j, q_i, q_j)}")
 
 
 def test_spatial_order_identical_queries(bk, oracle):
