@@ -74,8 +74,8 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU work per baseline leg")
     ap.add_argument("--sur-steps", type=int, default=16_000, help="training steps of the surrogate")
-    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate,gauss16,adaptive,dualtree",
-                    help="comma list: gauss, epanprod, cvgrid, surrogate, gauss16, adaptive, dualtree")
+    ap.add_argument("--kernels", default="gauss,epanprod,cvgrid,surrogate,gauss16,adaptive,dualtree,fp64",
+                    help="comma list: gauss, epanprod, cvgrid, surrogate, gauss16, adaptive, dualtree, fp64")
     ap.add_argument("--no-spatial-order", action="store_true",
                     help="skip the timing of the same evaluations in device-sorted query order")
     ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
@@ -609,6 +609,54 @@ def bench_adaptive(bk, pts, mass, q):
     return res
 
 
+# FP64 instructions per pair of the FP64-context Gaussian kernel (gfx950
+# ISA of this build: differences, exponent, the degree-12 exp2, ldexp,
+# accumulate) and the measured cost of one FP64 VALU instruction per wave
+F64_GAUSS_INSTR_PER_PAIR = 24
+F64_CYCLES_PER_INSTR = 4.4
+
+
+def bench_fp64(bk, pts, mass, q, steps: int, warmup: int):
+    """The 1M x 1M workload in an FP64 context (the C++ API's default
+    HipDirectKde<D, double>): both kernels in the given query order and in
+    the device Z-curve order, parity on 256 sampled queries at 1e-12."""
+    from oracle.oracle import Oracle, default_threads
+
+    res = {"workload": "the headline 1M x 1M workload in a BK_PREC_F64 context (FP64 points and arithmetic)",
+           "unit": "pair-evals/s"}
+    pairs = float(N_REF) * float(N_QUERY)
+    idx = np.linspace(0, N_QUERY - 1, 256).astype(np.int64)
+    orc = Oracle()
+    with bk.DirectKde(pts, q, mass, precision=bk.BK_PREC_F64) as kde:
+        for name, kind, h in (("gauss", bk.BK_GAUSSIAN, (H_GAUSS, H_GAUSS)),
+                              ("epanprod", bk.BK_EPANECHNIKOV_PRODUCT_2D, H_EPAN)):
+            desc = bk.KernelDesc(kind, 2, *h)
+            vals, ms, _ = time_kernel(kde, desc, steps, warmup)
+            avg = float(np.mean(ms))
+            kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_SPATIAL)
+            svals, sms, _ = time_kernel(kde, desc, steps, 1)
+            kde.set_option(bk.BK_OPT_QUERY_ORDER, bk.BK_QUERY_ORDER_GIVEN)
+            want = orc.direct(kind, h[0], h[1], pts, q[idx], mass=mass, normalize=True, threads=default_threads())
+            floor = 1e-10 * orc.normalization(kind, 2, *h) * float(mass.sum())
+            above = want >= floor
+            rel = np.abs(vals[idx][above] - want[above]) / want[above]
+            res[name] = {"ms_per_step": avg, "pair_evals_per_s": pairs / (avg * 1e-3),
+                         "spatial_order_ms": float(np.mean(sms)),
+                         "max_abs_diff_spatial_vs_given": float(np.max(np.abs(svals - vals))),
+                         "parity": {"queries": int(len(idx)), "tolerance": 1e-12,
+                                    "frac_below_floor": float(1 - above.mean()),
+                                    "max_rel_err": float(rel.max()) if above.any() else 0.0}}
+    g = res["gauss"]["pair_evals_per_s"]
+    bound = 256 * 4 * 2.4e9 * 64 / (F64_GAUSS_INSTR_PER_PAIR * F64_CYCLES_PER_INSTR)
+    res["value"] = g
+    res["roofline"] = {"bound": "valu", "pipe": "FP64 VALU issue",
+                       "achieved": g, "peak": bound, "unit": "pair-evals/s", "frac": g / bound,
+                       "note": f"{F64_GAUSS_INSTR_PER_PAIR} FP64 instructions per Gaussian pair at "
+                               f"{F64_CYCLES_PER_INSTR} cycles per wave instruction (measured), 256 CUs x 4 SIMDs "
+                               "x 2.4 GHz"}
+    return res
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -628,7 +676,7 @@ def main() -> int:
         if args.gauss_form == "direct":
             kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         for k in kinds:
-            if k in ("cvgrid", "surrogate", "gauss16", "adaptive", "dualtree"):
+            if k in ("cvgrid", "surrogate", "gauss16", "adaptive", "dualtree", "fp64"):
                 continue
             if k == "gauss":
                 desc, flop = bk.KernelDesc(bk.BK_GAUSSIAN, 2, H_GAUSS), FLOP_PER_PAIR_GAUSS
@@ -680,6 +728,13 @@ def main() -> int:
         log("adaptive pass + LOO LCV + adaptive evaluation (1M)")
         ad_res = bench_adaptive(bk, pts, mass, q)
         log(f"adaptive: {ad_res['value']:.4e} pair-evals/s")
+
+    f64_res = None
+    if "fp64" in kinds:
+        log("FP64 context (1M x 1M, both kernels, given and spatial order)")
+        f64_res = bench_fp64(bk, pts, mass, q, min(args.steps, 2), args.warmup)
+        log(f"fp64: gauss {f64_res['gauss']['ms_per_step']:.1f} ms, epanprod "
+            f"{f64_res['epanprod']['ms_per_step']:.1f} ms per step")
 
     dt_res = None
     if "dualtree" in kinds:
@@ -736,6 +791,8 @@ def main() -> int:
         line["surrogate"] = sur_res
     if g16_res is not None:
         line["gauss16"] = g16_res
+    if f64_res is not None:
+        line["fp64_context"] = f64_res
     if ad_res is not None:
         line["adaptive"] = ad_res
     if dt_res is not None:
