@@ -27,7 +27,7 @@ __all__ = [
     "BK_EPANECHNIKOV", "BK_GAUSSIAN", "BK_EPANECHNIKOV_PRODUCT_2D", "BK_GAUSSIAN_PRODUCT_2D",
     "BK_EPANECHNIKOV_CONV_1D", "BK_EPANECHNIKOV_PRODUCT_CONV_2D", "BK_GAUSSIAN_CONV_1D",
     "BK_GAUSSIAN_PRODUCT_CONV_2D", "BK_PREC_F32", "BK_PREC_F64", "BkError", "KernelDesc",
-    "DirectKde", "CvGrid", "Surrogate", "load_library", "library_path",
+    "DirectKde", "CvGrid", "Surrogate", "load_library", "library_path", "lexico_order",
 ]
 
 # ---- constants (mirror include/bbrcit_hip.h) ---------------------------------
@@ -154,6 +154,8 @@ def load_library() -> ctypes.CDLL:
         "bk_alias_queries_to_refs": (ctypes.c_int, [c_ctx_p]),
         "bk_size": (ctypes.c_int, [c_ctx_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
         "bk_update_refs": (ctypes.c_int, [c_ctx_p, _DBL_P, _DBL_P]),
+        "bk_lexico_order": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, _DBL_P,
+                                           ctypes.POINTER(ctypes.c_longlong)]),
         "bk_eval_range": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t, ctypes.c_size_t,
                                          ctypes.c_size_t, ctypes.c_size_t, _DBL_P, ctypes.c_int]),
         "bk_eval_range_device": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
@@ -233,6 +235,17 @@ def _as_points(coords) -> np.ndarray:
     if c.ndim != 2:
         raise ValueError("coordinates must be an (n, D) array")
     return c
+
+
+def lexico_order(coords, device: int = 0) -> np.ndarray:
+    """Indices of the points in lexicographic order (ExactLexicoLess, ties in
+    index order), sorted on the device: bk_lexico_order."""
+    c = _as_points(coords)
+    order = np.empty(len(c), dtype=np.int64)
+    if len(c):
+        _check(load_library().bk_lexico_order(int(device), c.shape[1], len(c), _dptr(c),
+                                              order.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+    return order
 
 
 def _opt_vec(v, n: int) -> Optional[np.ndarray]:

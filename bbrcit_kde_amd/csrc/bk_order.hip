@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <string>
+#include <utility>
+
 namespace bk {
 
 namespace {
@@ -65,6 +68,30 @@ __global__ void scatter_f64_kernel(const long long *__restrict__ list, long long
                                    double *__restrict__ out) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[list[k]] = vals[k];
+}
+
+// Order-preserving 64-bit key of a double: -0 and +0 share the key of +0
+// (ExactLexicoLess compares them equal); negative values take their
+// complemented bits, the others their bits with the sign bit set, so
+// unsigned order is numeric order.
+__device__ __forceinline__ unsigned long long lexico_key(double v) {
+  if (v == 0.0) v = 0.0;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+
+// keys[k] = key of coordinate d of point order[k]
+__global__ void lexico_keys_kernel(const double *__restrict__ coords, int dim, int d,
+                                   const long long *__restrict__ order, long long n,
+                                   unsigned long long *__restrict__ keys) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  keys[k] = lexico_key(coords[order[k] * dim + d]);
+}
+
+__global__ void iota_kernel(long long *__restrict__ v, long long n) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) v[k] = k;
 }
 
 }  // namespace
@@ -137,3 +164,58 @@ void spatial_sort(const double *qx, const double *qy, const long long *list, lon
 }
 
 }  // namespace bk
+
+// Lexicographic order of n points by a stable device radix sort, least
+// significant coordinate first (see include/bbrcit_hip.h).
+extern "C" int bk_lexico_order(int device, int dim, size_t n, const double *coords, long long *order_out) {
+  using namespace bk;
+  return guarded([&] {
+    if (dim < 1 || dim > 16) fail(BK_ERR_INVALID_ARG, "dimension must be between 1 and 16");
+    if (n == 0) return;
+    if (!coords || !order_out) fail(BK_ERR_INVALID_ARG, "null coordinate or order array");
+    if (n > 0x7fffffffULL) fail(BK_ERR_INVALID_ARG, "at most 2^31 - 1 points");
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count) fail(BK_ERR_INVALID_ARG, "no HIP device " + std::to_string(device));
+    DeviceGuard guard(device);
+    struct Work {
+      hipStream_t s = nullptr;
+      DevBuf c, ka, kb, va, vb, tmp;
+      ~Work() {
+        for (DevBuf *b : {&c, &ka, &kb, &va, &vb, &tmp}) b->release();
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } w;
+    hip_check(hipStreamCreate(&w.s), "hipStreamCreate");
+    const long long nn = (long long)n;
+    w.c.reserve(n * (size_t)dim * sizeof(double));
+    w.ka.reserve(n * sizeof(unsigned long long));
+    w.kb.reserve(n * sizeof(unsigned long long));
+    w.va.reserve(n * sizeof(long long));
+    w.vb.reserve(n * sizeof(long long));
+    size_t tmp_bytes = 0;
+    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, w.ka.as<unsigned long long>(),
+                                                 w.kb.as<unsigned long long>(), w.va.as<long long>(),
+                                                 w.vb.as<long long>(), (int)n, 0, 64, w.s),
+              "radix sort size query");
+    w.tmp.reserve(tmp_bytes + 256);
+    hip_check(hipMemcpyAsync(w.c.ptr, coords, n * (size_t)dim * sizeof(double), hipMemcpyHostToDevice, w.s),
+              "upload coordinates");
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    iota_kernel<<<grid, 256, 0, w.s>>>(w.va.as<long long>(), nn);
+    hip_check(hipGetLastError(), "iota launch");
+    for (int d = dim - 1; d >= 0; --d) {
+      lexico_keys_kernel<<<grid, 256, 0, w.s>>>(w.c.as<double>(), dim, d, w.va.as<long long>(), nn,
+                                                w.ka.as<unsigned long long>());
+      hip_check(hipGetLastError(), "key launch");
+      hip_check(hipcub::DeviceRadixSort::SortPairs(w.tmp.ptr, tmp_bytes, w.ka.as<unsigned long long>(),
+                                                   w.kb.as<unsigned long long>(), w.va.as<long long>(),
+                                                   w.vb.as<long long>(), (int)n, 0, 64, w.s),
+                "radix sort");
+      std::swap(w.va, w.vb);
+    }
+    hip_check(hipMemcpyAsync(order_out, w.va.ptr, n * sizeof(long long), hipMemcpyDeviceToHost, w.s),
+              "download order");
+    hip_check(hipStreamSynchronize(w.s), "hipStreamSynchronize");
+  });
+}

@@ -12,9 +12,10 @@
 //      the points' bounding box, grown from the reference's initial values:
 //      lowest edge numeric_limits::max(), highest edge numeric_limits::min())
 //      and the merge of its points' attributes.
-// Large sets (kParallelMin points and more) are built on several host
-// threads without changing the result: the sort runs as sorted chunks merged
-// pairwise (the same order, since ExactLexicoLess is a strict order on
+// Large sets (kParallelMin points and more) are built without changing the
+// result: the sort runs on the HIP device when there is one (a stable radix
+// sort, bk_lexico_order) or else as sorted chunks merged pairwise on several
+// host threads (the same order, since ExactLexicoLess is a strict order on
 // distinct points; only the order among exact duplicates, which are merged
 // next, may differ), and sibling subtrees, whose ranges are disjoint, are
 // partitioned concurrently with the same std::nth_element calls.
@@ -32,6 +33,7 @@
 #include <utility>
 #include <vector>
 
+#include <bbrcit_hip.h>
 #include "Attributes/PointWeights.h"
 #include "DecoratedPoint.h"
 #include "FloatUtils.h"
@@ -169,8 +171,49 @@ class Kdtree {
     root_ = make_node(0, points_.size() - 1, 0, bounding_box(), spawn_depth);
   }
 
+  // true when HIP device 0 runs bk_lexico_order (tried once per process)
+  static bool device_sort_available() {
+    static const bool ok = [] {
+      double x = 0.0;
+      long long o = 0;
+      return bk_lexico_order(0, 1, 1, &x, &o) == BK_OK;
+    }();
+    return ok;
+  }
+
+  // Large sets on a machine with a HIP device: the order from a device radix
+  // sort (bk_lexico_order: ExactLexicoLess, duplicates in input order), then
+  // the points moved into it on several threads.  false: nothing done.
+  bool device_sort() {
+    const size_t n = points_.size();
+    if (n < kParallelMin || n > 0x7fffffffULL || !device_sort_available()) return false;
+    const int t = build_threads();
+    auto on_threads = [t, n](auto &&body) {
+      std::vector<std::future<void>> jobs;
+      for (int c = 0; c < t; ++c) {
+        const size_t a = n * (size_t)c / (size_t)t, b = n * (size_t)(c + 1) / (size_t)t;
+        jobs.push_back(std::async(std::launch::async, [&body, a, b] { body(a, b); }));
+      }
+      for (auto &j : jobs) j.get();
+    };
+    std::vector<double> coords(n * D);
+    on_threads([&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i)
+        for (int d = 0; d < D; ++d) coords[i * D + d] = (double)points_[i][d];
+    });
+    std::vector<long long> order(n);
+    if (bk_lexico_order(0, D, n, coords.data(), order.data()) != BK_OK) return false;
+    std::vector<DataPointType> sorted(n);
+    on_threads([&](size_t a, size_t b) {
+      for (size_t k = a; k < b; ++k) sorted[k] = std::move(points_[(size_t)order[k]]);
+    });
+    points_.swap(sorted);
+    return true;
+  }
+
   // chunks sorted on their own threads, then merged pairwise
   void sort_points() {
+    if (device_sort()) return;
     const int t = build_threads();
     if (points_.size() < kParallelMin || t < 2) {
       std::sort(points_.begin(), points_.end(), ExactLexicoLess<DataPointType>);

@@ -152,6 +152,14 @@ int bk_size(const bk_ctx *ctx, int set, size_t *n_out);
  * after an adaptive pass; NULL leaves that array unchanged. */
 int bk_update_refs(bk_ctx *ctx, const double *mass, const double *abw);
 
+/* order_out[k] = index of the k-th of n points (row-major coords, dim per
+ * point, 1 <= dim <= 16, n < 2^31) in lexicographic order: the order of
+ * ExactLexicoLess (include/FloatUtils.h:72-75; -0 equals +0), ties (exact
+ * duplicates) in index order.  A stable radix sort on HIP device `device`,
+ * least significant coordinate first; needs no context.  Replaces the
+ * std::sort that starts Kdtree construction (include/Kdtree.h:369-371). */
+int bk_lexico_order(int device, int dim, size_t n, const double *coords, long long *order_out);
+
 /* ---- direct sums ----------------------------------------------------- */
 
 /* out_host[k - q_i] = S_k (times C(h) if normalize != 0) for k in [q_i, q_j],

@@ -95,6 +95,26 @@ def test_kdtree_parallel_build_matches_reference(api, reference):
         np.testing.assert_array_equal(m, m_ref)
 
 
+@pytest.mark.gpu
+def test_kdtree_device_sorted_build_matches_reference(api, reference):
+    # with a HIP device the build's lexicographic sort runs on it
+    # (bk_lexico_order); the tree's point order and merged duplicates must
+    # still be the reference's
+    rng = np.random.default_rng(8)
+    n = 400_000
+    pts = rng.normal(size=(n, 2))
+    pts[::1000] = pts[3]  # exact duplicates, merged
+    w = rng.uniform(0.5, 1.5, n)
+    for leaf in (32, 4096):
+        c_ref, w_ref, m_ref, _ = reference.tree_points(EPAN, 1, 0.1, 0.1, pts, w, leaf_nmax=leaf)
+        c, ww, m = np.empty((n, 2)), np.empty(n), np.empty(n)
+        k = api.api_tree_points(EPAN, 2, 1, 0.1, 0.1, _p(pts), _p(w), n, leaf, _p(c), _p(ww), _p(m))
+        assert k == len(c_ref), _err(api)
+        np.testing.assert_array_equal(c[:k], c_ref)
+        np.testing.assert_allclose(ww[:k], w_ref, rtol=1e-14)
+        np.testing.assert_allclose(m[:k], m_ref, rtol=1e-14)
+
+
 @pytest.mark.parametrize("kind,dim", [(EPAN, 1), (EPAN, 2), (GAUSS, 1), (GAUSS, 2), (EPANPROD, 2), (GAUSSPROD, 2),
                                       (EPANCONV1, 1), (EPANPRODCONV, 2), (GAUSSCONV1, 1), (GAUSSPRODCONV, 2)])
 @pytest.mark.parametrize("kf", [0, 1])

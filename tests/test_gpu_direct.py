@@ -281,3 +281,26 @@ def test_reference_cpu_case_10k(bk, oracle, reference, precision):
         want = oracle.direct(GAUSS, 0.1, 0.1, f32(pts), f32(q), mass=f32(w), normalize=True)
         assert_parity(got, want, bound, what="FP32 vs oracle")
         assert_parity(got, want_ref, bound, rtol=2e-5, what="FP32 vs reference (FP64 inputs)")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_lexico_order_matches_numpy(bk, dim):
+    # the device order that starts Kdtree construction: lexicographic by
+    # coordinate 0, then 1, ...; exact duplicates (and -0 against +0) keep
+    # their index order, as numpy's stable lexsort does
+    rng = np.random.default_rng(40 + dim)
+    n = 300_001
+    pts = rng.normal(size=(n, dim))
+    pts[::97] = pts[5]                      # exact duplicates
+    pts[1000:1300, 0] = np.round(pts[1000:1300, 0], 1)  # ties in the first coordinate
+    pts[7, 0], pts[8, 0] = -0.0, 0.0
+    pts[20:30] = np.array([[-0.0] * dim])
+    pts[30:40] = np.array([[0.0] * dim])
+    got = bk.lexico_order(pts)
+    keys = pts + 0.0                        # -0 -> +0
+    want = np.lexsort(tuple(keys[:, d] for d in reversed(range(dim))))
+    np.testing.assert_array_equal(got, want)
+    assert len(bk.lexico_order(np.empty((0, dim)))) == 0
+    lib = bk.load_library()
+    assert lib.bk_lexico_order(0, 17, 1, None, None) == bk.BK_ERR_INVALID_ARG
+    assert lib.bk_lexico_order(0, dim, 10, None, None) == bk.BK_ERR_INVALID_ARG
