@@ -15,9 +15,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -215,6 +217,21 @@ PointSet &set_of(bk_ctx *ctx, int set) {
 
 const PointSet &query_set(const bk_ctx *ctx) { return ctx->alias ? ctx->refs : ctx->queries; }
 
+// body(lo, hi) over disjoint slices of [0, n), on up to 16 host threads for
+// large n; body must not throw.
+template <class F>
+void host_parallel(size_t n, F &&body) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t t = n < (size_t(1) << 18) || hw < 2 ? 1 : std::min<size_t>(hw, 16);
+  if (t == 1) {
+    body(size_t(0), n);
+    return;
+  }
+  std::vector<std::thread> workers;
+  for (size_t c = 0; c < t; ++c) workers.emplace_back([&body, n, c, t] { body(n * c / t, n * (c + 1) / t); });
+  for (auto &w : workers) w.join();
+}
+
 // Copy host data into a point set; masses/abw default to 1.
 void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coords,
                 const double *mass, const double *abw) {
@@ -232,9 +249,14 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
   const hipStream_t s = ctx->stream;
   if (ctx->precision == BK_PREC_F32) {
     std::vector<float> m(n), a(n);
+    host_parallel(n, [&](size_t lo, size_t hi) {
+      for (size_t i = lo; i < hi; ++i) {
+        m[i] = mass ? (float)mass[i] : 1.0f;
+        a[i] = abw ? (float)abw[i] : 1.0f;
+      }
+    });
+    // validation and the mass sum in index order (deterministic)
     for (size_t i = 0; i < n; ++i) {
-      m[i] = mass ? (float)mass[i] : 1.0f;
-      a[i] = abw ? (float)abw[i] : 1.0f;
       if (a[i] != 1.0f) ps.uniform_abw = false;
       if (!(a[i] > 0.0f) || !std::isfinite(a[i]))
         fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
@@ -248,16 +270,21 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
     hip_check(hipMemcpyAsync(ps.m.ptr, m.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload masses");
     hip_check(hipMemcpyAsync(ps.abw.ptr, a.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload abw");
     if (dim <= 2) {
-      std::vector<float> x(n), y(n, 0.0f);
-      for (size_t i = 0; i < n; ++i) {
-        x[i] = (float)coords[i * dim];
-        if (dim == 2) y[i] = (float)coords[i * dim + 1];
-      }
-      float lo = x[0], hi = x[0];
-      for (size_t i = 0; i < n; ++i) {
-        lo = std::min(lo, std::min(x[i], y[i]));
-        hi = std::max(hi, std::max(x[i], y[i]));
-      }
+      std::vector<float> x(n), y(n);
+      std::mutex part_mu;
+      float lo = (float)coords[0], hi = lo;
+      host_parallel(n, [&](size_t b, size_t e) {
+        float l = (float)coords[b * dim], h = l;
+        for (size_t i = b; i < e; ++i) {
+          x[i] = (float)coords[i * dim];
+          y[i] = dim == 2 ? (float)coords[i * dim + 1] : 0.0f;
+          l = std::min(l, std::min(x[i], y[i]));
+          h = std::max(h, std::max(x[i], y[i]));
+        }
+        std::lock_guard<std::mutex> g(part_mu);  // min and max do not depend on the order
+        lo = std::min(lo, l);
+        hi = std::max(hi, h);
+      });
       ps.box_lo = std::isfinite(lo) ? lo : -1.0f;
       ps.box_hi = std::isfinite(hi) && hi > ps.box_lo ? hi : ps.box_lo + 1.0f;
       ps.x.reserve(n * sizeof(float));

@@ -19,8 +19,10 @@
 
 #include <bbrcit_hip.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -39,7 +41,8 @@ inline void check(int status) {
   throw std::runtime_error("bbrcit_hip: " + msg);
 }
 
-// Host points -> row-major coordinates, masses, abw (as double).
+// Host points -> row-major coordinates, masses, abw (as double); large sets
+// on several host threads (disjoint slices, so the result is the same).
 template <int D, typename HostPointT>
 void flatten(const std::vector<HostPointT> &pts, std::vector<double> &xy, std::vector<double> &m,
              std::vector<double> &a) {
@@ -47,11 +50,22 @@ void flatten(const std::vector<HostPointT> &pts, std::vector<double> &xy, std::v
   xy.resize(n * D);
   m.resize(n);
   a.resize(n);
-  for (size_t i = 0; i < n; ++i) {
-    for (int d = 0; d < D; ++d) xy[i * D + d] = (double)pts[i][d];
-    m[i] = (double)pts[i].attributes().mass();
-    a[i] = (double)pts[i].attributes().abw();
+  auto slice = [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+      for (int d = 0; d < D; ++d) xy[i * D + d] = (double)pts[i][d];
+      m[i] = (double)pts[i].attributes().mass();
+      a[i] = (double)pts[i].attributes().abw();
+    }
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t t = n < (size_t(1) << 18) || hw < 2 ? 1 : std::min<size_t>(hw, 16);
+  if (t == 1) {
+    slice(0, n);
+    return;
   }
+  std::vector<std::thread> workers;
+  for (size_t c = 0; c < t; ++c) workers.emplace_back(slice, n * c / t, n * (c + 1) / t);
+  for (auto &w : workers) w.join();
 }
 
 }  // namespace hip_detail
