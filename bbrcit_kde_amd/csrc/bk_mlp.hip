@@ -2,7 +2,7 @@
 // trained on (query, log-density) pairs from the exact direct sum and served
 // with a sampled check against the exact sum (bk_mlp_*, declared in
 // include/bbrcit_hip.h).  The reference has no counterpart; the exact direct
-// sum stays both the fallback and the judge of the surrogate's error.
+// sum stays both the fallback and the arbiter of the surrogate's error.
 //
 // Weights are BF16 (FP32 master copies updated by Adam; every forward pass,
 // in training and serving, uses the BF16-rounded weights), accumulation is

@@ -349,7 +349,7 @@ int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
  * BF16 (FP32 master copies for Adam), biases FP32, accumulation FP32.
  * Parameter order (floats): W1[128][2], b1[128], W2[128][128], b2[128],
  * W3[128][128], b3[128], W4[128], b4 (W[out][in]).  The exact direct sum is
- * both the fallback and the judge of the surrogate (bk_mlp_serve_device). */
+ * both the fallback and the arbiter of the surrogate (bk_mlp_serve_device). */
 #define BK_MLP_NUM_PARAMS 33537
 
 typedef struct bk_mlp bk_mlp;
