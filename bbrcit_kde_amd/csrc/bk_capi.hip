@@ -74,7 +74,7 @@ struct bk_ctx {
   DevBuf ord_list, ord_keys_a, ord_keys_b, ord_tmp, ord_x, ord_y, ord_out;  // spatial query order
   // bounded evaluation: tile summaries of the reference set (per version),
   // per-block thresholds, pruning counters, bounds planes
-  DevBuf tsum, bnd_mask, bnd_lo, bnd_gap, bnd_counts, bnd_exact, bnd_hi;
+  DevBuf tsum, bnd_mask, bnd_lo, bnd_gap, bnd_counts, bnd_exact, bnd_hi, bnd_vals, bnd_order;
   unsigned long long tsum_version = 0;
 
   // staging
@@ -770,6 +770,17 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
       dst = ctx->partial.as<double>();
     }
     bk::BoundArgs ba{mask, nwords};
+    // (block, split) pairs with the most listed tiles start first, so the
+    // launch does not end on a few long workgroups
+    const long long nblocks = (cn + qpb - 1) / qpb;
+    if (nblocks * splits > 1 && nblocks * splits <= 0x7fffffffLL) {
+      ctx->bnd_order.reserve((size_t)(nblocks * splits) * sizeof(int));
+      const long long split_tiles = bk::split_length(0, nr, splits, tile_of(path)) / bk::kTile;
+      bk::listed_order(mask, nwords, (int)nblocks, splits, split_tiles, ctx->bnd_order.as<int>(), ctx->ord_keys_a,
+                       ctx->ord_keys_b, ctx->bnd_vals, ctx->ord_tmp, s);
+      ba.order = ctx->bnd_order.as<int>();
+      ba.nblocks = (int)nblocks;
+    }
     hipError_t e;
     if (path == Path::GAUSS32)
       e = bk::launch_gauss(p, qx32, qy32, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(),
@@ -963,7 +974,7 @@ int bk_destroy(bk_ctx *ctx) {
                         &ctx->weights, &ctx->scratch, &ctx->scalar, &ctx->ord_list, &ctx->ord_keys_a,
                         &ctx->ord_keys_b, &ctx->ord_tmp, &ctx->ord_x, &ctx->ord_y, &ctx->ord_out, &ctx->tsum,
                         &ctx->bnd_mask, &ctx->bnd_lo, &ctx->bnd_gap, &ctx->bnd_counts, &ctx->bnd_exact,
-                        &ctx->bnd_hi})
+                        &ctx->bnd_hi, &ctx->bnd_vals, &ctx->bnd_order})
         b->release();
       if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }

@@ -86,7 +86,15 @@ constexpr int kTileSummary = 2;
 struct BoundArgs {
   const unsigned long long *mask;  // per query block, nwords words: bit t set = tile t is summed
   long long nwords;
+  // optional dispatch order of the (query block, reference split) pairs:
+  // workgroup k takes pair order[k] = block + split * nblocks (from
+  // listed_order: most listed tiles first); nullptr = the 2-D grid's order
+  const int *order = nullptr;
+  int nblocks = 0;
 };
+// Reference split length of the range kernels (a multiple of `tile`).
+long long split_length(long long r_i, long long r_end, int splits, int tile);
+
 hipError_t launch_tile_summary_f32(const float *x, const float *y, const float *m, const float *abw, size_t n,
                                    double4 *tsum, hipStream_t s);
 hipError_t launch_tile_summary_f64(const double *x, const double *y, const double *m, const double *abw, size_t n,
@@ -226,6 +234,11 @@ void spatial_sort(const float *qx, const float *qy, const long long *list, long 
 void spatial_sort(const double *qx, const double *qy, const long long *list, long long n, long long index_limit,
                   float lo, float hi, long long *list_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &tmp,
                   hipStream_t s);
+// order_out[k]: the (block, split) pairs, block + split * nblocks, sorted by
+// the number of tiles listed in `mask` within the split, most first (ties in
+// pair order), so a bounded launch starts its longest workgroups first.
+void listed_order(const unsigned long long *mask, long long nwords, int nblocks, int splits, long long split_tiles,
+                  int *order_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &vals, DevBuf &tmp, hipStream_t s);
 // gx[k] = qx[list[k]], gy[k] = qy[list[k]] for k < n.
 hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
                             float *gy, hipStream_t s);

@@ -501,14 +501,20 @@ __global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
     const float4 *__restrict__ tiles, long long r_i, long long r_end,
     long long split_len, long long q_i, long long nq, double *__restrict__ dst,
     double out_scale, int self_excl, float rho, double gx, double gy, int mode, BoundArgs ba) {
-  const long long qb = (long long)blockIdx.x * kGaussQueriesPerBlock;
+  long long bx = blockIdx.x, by = blockIdx.y;
+  if (BOUNDED && ba.order) {
+    const int it = ba.order[blockIdx.x];
+    bx = it % ba.nblocks;
+    by = it / ba.nblocks;
+  }
+  const long long qb = bx * kGaussQueriesPerBlock;
   long long rb, re;
-  split_bounds<kGaussTile>(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
+  split_bounds<kGaussTile>(r_i, r_end, split_len, (int)by, rb, re);
   const long long left = nq - qb;
   const int q_count = (int)(left < kGaussQueriesPerBlock ? left : kGaussQueriesPerBlock);
-  const long long off = (long long)blockIdx.y * nq + qb;
+  const long long off = by * nq + qb;
   gauss_item<ISO, BOUNDED>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
-                           gx, gy, mode, BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
+                           gx, gy, mode, BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
 }
 
 template <bool ISO>
@@ -797,14 +803,20 @@ __global__ __launch_bounds__(kBlock) void epan_range_kernel(
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_i, long long r_end, long long split_len, long long q_i,
     long long nq, double *__restrict__ dst, double out_scale, int self_excl, BoundArgs ba) {
-  const long long qb = (long long)blockIdx.x * kQueriesPerBlock;
+  long long bx = blockIdx.x, by = blockIdx.y;
+  if (BOUNDED && ba.order) {
+    const int it = ba.order[blockIdx.x];
+    bx = it % ba.nblocks;
+    by = it / ba.nblocks;
+  }
+  const long long qb = bx * kQueriesPerBlock;
   long long rb, re;
-  split_bounds<kTile>(r_i, r_end, split_len, (int)blockIdx.y, rb, re);
+  split_bounds<kTile>(r_i, r_end, split_len, (int)by, rb, re);
   const long long left = nq - qb;
   const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
-  const long long off = (long long)blockIdx.y * nq + qb;
+  const long long off = by * nq + qb;
   epan_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
-                          BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
+                          BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
 }
 
 template <int FAM>
@@ -1068,16 +1080,21 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
     long long nq, double *__restrict__ dst, double out_scale, int self_excl,
     double rho, BoundArgs ba) {
   constexpr int kQb = kBlock64 * kQpt;
-  const long long qb = (long long)blockIdx.x * kQb;
-  const long long rb = r_i + (long long)blockIdx.y * split_len;
+  long long bx = blockIdx.x, by = blockIdx.y;
+  if (BOUNDED && ba.order) {
+    const int it = ba.order[blockIdx.x];
+    bx = it % ba.nblocks;
+    by = it / ba.nblocks;
+  }
+  const long long qb = bx * kQb;
+  const long long rb = r_i + by * split_len;
   long long re = rb + split_len;
   if (re > r_end) re = r_end;
   const long long left = nq - qb;
   const int q_count = (int)(left < kQb ? left : kQb);
-  const long long off = (long long)blockIdx.y * nq + qb;
+  const long long off = by * nq + qb;
   direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
-                                rho,
-                                BOUNDED ? ba.mask + (long long)blockIdx.x * ba.nwords : nullptr);
+                                rho, BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
 }
 
 template <int FAM>
@@ -1542,12 +1559,6 @@ __global__ void adapt_lambda_kernel(const double *__restrict__ S, long long n,
 
 // Split length: a multiple of `tile`, counted from the tile boundary at or
 // below r_i.
-long long split_length(long long r_i, long long r_end, int splits, int tile) {
-  const long long base = (r_i / tile) * tile;
-  long long len = (r_end - base + splits - 1) / splits;
-  len = (len + tile - 1) / tile * tile;
-  return len;
-}
 
 // ---- bounded evaluation: tile summaries, first pass, bounds planes ----------------
 // Block-wide (kTile threads) FP64 reduction: min, or sum.
@@ -1675,6 +1686,13 @@ __global__ void bounds_combine_kernel(const double *__restrict__ exact, const do
 
 }  // namespace
 
+long long split_length(long long r_i, long long r_end, int splits, int tile) {
+  const long long base = (r_i / tile) * tile;
+  long long len = (r_end - base + splits - 1) / splits;
+  len = (len + tile - 1) / tile * tile;
+  return len;
+}
+
 // ---- launchers -----------------------------------------------------------------
 
 hipError_t launch_tile_summary_f32(const float *x, const float *y, const float *m, const float *abw, size_t n,
@@ -1754,8 +1772,9 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
   } else {
     if (nq <= 0) return hipSuccess;
     const long long len = split_length(r_i, r_end, splits, kGaussTile);
-    const dim3 grid((unsigned)((nq + kGaussQueriesPerBlock - 1) / kGaussQueriesPerBlock), (unsigned)splits);
+    dim3 grid((unsigned)((nq + kGaussQueriesPerBlock - 1) / kGaussQueriesPerBlock), (unsigned)splits);
     const BoundArgs b = ba ? *ba : BoundArgs{};
+    if (b.order) grid = dim3(grid.x * grid.y, 1);
 #define BK_GR(I, B)                                                                                              \
   gauss_range_kernel<I, B><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst, out_scale, \
                                                    self_excl, rho, gx, gy, mode, b)
@@ -1806,8 +1825,9 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
   } else {
     if (nq <= 0) return hipSuccess;
     const long long len = split_length(r_i, r_end, splits, kTile);
-    const dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
+    dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
     const BoundArgs b = ba ? *ba : BoundArgs{};
+    if (b.order) grid = dim3(grid.x * grid.y, 1);
 #define BK_ER(F, B)                                                                                 \
   epan_range_kernel<F, B><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, \
                                                   self_excl, b)
@@ -1867,8 +1887,9 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
     if (nq <= 0) return hipSuccess;
     long long len = (r_end - r_i + splits - 1) / splits;
     len = (len + kTile64 - 1) / kTile64 * kTile64;
-    const dim3 grid((unsigned)((nq + kBlock64 * kQpt - 1) / (kBlock64 * kQpt)), (unsigned)splits);
+    dim3 grid((unsigned)((nq + kBlock64 * kQpt - 1) / (kBlock64 * kQpt)), (unsigned)splits);
     const BoundArgs b = ba ? *ba : BoundArgs{};
+    if (b.order) grid = dim3(grid.x * grid.y, 1);
 #define BK_R64(F, B)                                                          \
   direct_f64_range_kernel<F, B><<<grid, kBlock64, 0, s>>>(                    \
       qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho, b)

@@ -89,6 +89,32 @@ __global__ void lexico_keys_kernel(const double *__restrict__ coords, int dim, i
   keys[k] = lexico_key(coords[order[k] * dim + d]);
 }
 
+// keys[i] = ~(tiles listed for pair i = b + sp * nblocks within split sp's
+// tile range), vals[i] = i
+__global__ void listed_count_kernel(const unsigned long long *__restrict__ mask, long long nwords, int nblocks,
+                                    int splits, long long split_tiles, unsigned int *__restrict__ keys,
+                                    int *__restrict__ vals) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nblocks * splits) return;
+  const long long b = i % nblocks, sp = i / nblocks;
+  const unsigned long long *m = mask + b * nwords;
+  const long long t0 = sp * split_tiles;
+  long long t1 = t0 + split_tiles;
+  if (t1 > nwords * 64) t1 = nwords * 64;
+  unsigned int c = 0;
+  for (long long t = t0; t < t1;) {
+    const long long w = t >> 6;
+    const long long stop = (w + 1) * 64 < t1 ? (w + 1) * 64 : t1;
+    const int len = (int)(stop - t);
+    unsigned long long bits = m[w] >> (t & 63);
+    if (len < 64) bits &= (1ULL << len) - 1ULL;
+    c += (unsigned int)__builtin_popcountll(bits);
+    t = stop;
+  }
+  keys[i] = ~c;
+  vals[i] = (int)i;
+}
+
 __global__ void iota_kernel(long long *__restrict__ v, long long n) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) v[k] = k;
@@ -114,6 +140,27 @@ hipError_t launch_scatter_f64(const long long *list, long long n, const double *
   if (n <= 0) return hipSuccess;
   scatter_f64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(list, n, vals, out);
   return hipGetLastError();
+}
+
+void listed_order(const unsigned long long *mask, long long nwords, int nblocks, int splits, long long split_tiles,
+                  int *order_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &vals, DevBuf &tmp, hipStream_t s) {
+  const long long n = (long long)nblocks * splits;
+  if (n <= 0) return;
+  if (n > 0x7fffffffLL) fail(BK_ERR_INVALID_ARG, "listed_order: too many (block, split) pairs");
+  keys_a.reserve((size_t)n * sizeof(unsigned int));
+  keys_b.reserve((size_t)n * sizeof(unsigned int));
+  vals.reserve((size_t)n * sizeof(int));
+  size_t tmp_bytes = 0;
+  hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_a.as<unsigned int>(), keys_b.as<unsigned int>(),
+                                               vals.as<int>(), order_out, (int)n, 0, 32, s),
+            "radix sort size query");
+  tmp.reserve(tmp_bytes + 256);
+  listed_count_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(mask, nwords, nblocks, splits, split_tiles,
+                                                                   keys_a.as<unsigned int>(), vals.as<int>());
+  hip_check(hipGetLastError(), "listed count launch");
+  hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys_a.as<unsigned int>(),
+                                               keys_b.as<unsigned int>(), vals.as<int>(), order_out, (int)n, 0, 32, s),
+            "radix sort");
 }
 
 namespace {
