@@ -74,6 +74,10 @@ def test_null_arguments_are_rejected(bk):
     assert lib.bk_eval_range(None, ctypes.byref(d), 0, 0, 0, 0, out.ctypes.data_as(bk._DBL_P), 0) == \
         bk.BK_ERR_INVALID_ARG
     assert lib.bk_create(ctypes.byref(ctypes.c_void_p()), 0, 7) == bk.BK_ERR_INVALID_ARG
+    # the device sort checks its arguments before touching a device
+    assert lib.bk_lexico_order(0, 17, 4, None, None) == bk.BK_ERR_INVALID_ARG
+    assert lib.bk_lexico_order(0, 2, 4, None, None) == bk.BK_ERR_INVALID_ARG
+    assert lib.bk_lexico_order(0, 2, 0, None, None) == bk.BK_OK  # nothing to order
 
 
 def test_python_view_raises_without_library(monkeypatch, tmp_path):
