@@ -1274,13 +1274,6 @@ __global__ void prep_general_kernel(EvalParams p, const float *__restrict__ abw,
 // M, queries on N, so a lane's four results belong to its own query.
 typedef __attribute__((__vector_size__(4 * sizeof(double)))) double v4d;
 
-// 2^e for e <= 0 over the FP64 range: FP32 exp2 of the fraction, exact
-// power of two by ldexp in FP64.
-__device__ __forceinline__ double exp2_wide(double e) {
-  const double n = __builtin_fmax(__builtin_floor(e), -2000.0);
-  return __builtin_ldexp((double)__builtin_amdgcn_exp2f((float)(e - n)), (int)n);
-}
-
 constexpr int kGramGroups = 4;  // groups of 16 queries per wavefront (64 queries)
 
 // Terms of one 16 x 16 block of (reference, query) pairs from d = C_r +
