@@ -74,7 +74,7 @@ struct bk_ctx {
   DevBuf ord_list, ord_keys_a, ord_keys_b, ord_tmp, ord_x, ord_y, ord_out;  // spatial query order
   // bounded evaluation: tile summaries of the reference set (per version),
   // per-block thresholds, pruning counters, bounds planes
-  DevBuf tsum, bnd_mask, bnd_lo, bnd_gap, bnd_counts, bnd_exact, bnd_hi, bnd_vals, bnd_order;
+  DevBuf tsum, bnd_mask, bnd_lo, bnd_gap, bnd_counts, bnd_exact, bnd_hi, bnd_vals, bnd_order, live_boxes;
   unsigned long long tsum_version = 0;
 
   // staging
@@ -534,24 +534,24 @@ double path_scale(const bk_ctx *ctx, Path path, const EvalParams &p, int normali
 // or work-item mode.
 hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkItem *items, int n_items,
                        long long r_i, long long r_end, long long q_i, long long nq, int splits, double *dst,
-                       double scale, int self_excl) {
+                       double scale, int self_excl, const bk::BoundArgs *order = nullptr) {
   const hipStream_t s = ctx->stream;
   const PointSet &r = ctx->refs;
   const PointSet &q = query_set(ctx);
   switch (path) {
     case Path::F64:
       return bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
-                                   ctx->f1.as<double4>(), ctx->tiles.as<double4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
-                                   self_excl, s);
+                                   ctx->f1.as<double4>(), ctx->tiles.as<double4>(), items, n_items, r_i, r_end, q_i,
+                                   nq, splits, dst, scale, self_excl, s, order);
     case Path::GAUSS32:
       return bk::launch_gauss(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
                               ctx->tiles.as<float4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
                               self_excl,
-                              ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s);
+                              ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, order);
     case Path::EPAN64:
       return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<double4>(), ctx->f1.as<double4>(),
                              ctx->tiles.as<double4>(),
-                             items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
+                             items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s, order);
     case Path::GRAM:
       return bk::launch_gram(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double4>(), items, n_items,
                              r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
@@ -560,6 +560,41 @@ hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkIt
       return bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double2>(),
                                        items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
   }
+}
+
+// Dispatch order of a culled direct launch (queries in Z-curve order, or self
+// sums over kd-ordered points): the (block, split) pairs sorted by the tiles
+// they keep, most first, so the launch does not end on a few long
+// workgroups.  Results do not depend on it.  Returns an order-free
+// BoundArgs where it does not apply.  The FP32 Gaussian path is left in grid
+// order: its blocks are four times smaller, so the count pass (one test per
+// block and tile) costs more than the balance gains (measured: the sorted
+// 6.9M-query surrogate fallback 43 -> 46 ms); the FP64 kernels gain (1M x 1M
+// Epanechnikov product in Z-curve order 4.3 -> 3.7 ms).
+bk::BoundArgs live_order(bk_ctx *ctx, Path path, const EvalParams &p, const float *qx32, const float *qy32,
+                         const double *qx64, const double *qy64, long long nq, long long r_i, long long r_end,
+                         int splits) {
+  bk::BoundArgs ba{nullptr, 0};
+  if (path != Path::EPAN64 && path != Path::F64) return ba;
+  const int qpb = queries_per_item(path);
+  const long long nblocks = (nq + qpb - 1) / qpb;
+  const long long n = nblocks * splits;
+  if (n <= 1 || n > 0x7fffffffLL) return ba;
+  const hipStream_t s = ctx->stream;
+  const int kind = path == Path::GAUSS32 ? 0 : path == Path::EPAN64 ? 1 : 2;
+  ctx->live_boxes.reserve((size_t)nblocks * sizeof(double4));
+  ctx->ord_keys_a.reserve((size_t)n * sizeof(unsigned int));
+  ctx->bnd_vals.reserve((size_t)n * sizeof(int));
+  ctx->bnd_order.reserve((size_t)n * sizeof(int));
+  hip_check(bk::launch_live_counts(p, kind, qx32, qy32, qx64, qy64, nq, qpb, ctx->tiles.ptr, r_i, r_end, splits,
+                                   ctx->live_boxes.as<double4>(), ctx->ord_keys_a.as<unsigned int>(),
+                                   ctx->bnd_vals.as<int>(), s),
+            "live count launch");
+  bk::order_by_keys(ctx->ord_keys_a.as<unsigned int>(), ctx->bnd_vals.as<int>(), n, ctx->bnd_order.as<int>(),
+                    ctx->ord_keys_b, ctx->ord_tmp, s);
+  ba.order = ctx->bnd_order.as<int>();
+  ba.nblocks = (int)nblocks;
+  return ba;
 }
 
 // Direct sums over one range pair into device memory `out` (nq doubles).
@@ -604,8 +639,10 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                        ctx->ord_tmp, s);
       double *ox = ctx->ord_x.as<double>(), *oy = ctx->ord_y.as<double>();
       hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
+      const bk::BoundArgs ord = live_order(ctx, path, p, nullptr, nullptr, ox, oy, nq, (long long)r_i, r_end, splits);
       e = bk::launch_direct_f64(p, ox, oy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
-                                nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s);
+                                nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s,
+                                ord.order ? &ord : nullptr);
       hip_check(e, "direct kernel launch");
       if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, res, s), "reduce launch");
       hip_check(bk::launch_scatter_f64(list, nq, res, out, s), "scatter launch");
@@ -616,13 +653,15 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                      ctx->ord_tmp, s);
     float *ox = ctx->ord_x.as<float>(), *oy = ctx->ord_y.as<float>();
     hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
+    const bk::BoundArgs ord = live_order(ctx, path, p, ox, oy, nullptr, nullptr, nq, (long long)r_i, r_end, splits);
+    const bk::BoundArgs *op = ord.order ? &ord : nullptr;
     if (path == Path::GAUSS32)
       e = bk::launch_gauss(p, ox, oy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
                            0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0,
-                           ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s);
+                           ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, op);
     else
       e = bk::launch_epan(p, ox, oy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
-                          nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s);
+                          nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s, op);
     hip_check(e, "direct kernel launch");
     if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, res, s), "reduce launch");
     hip_check(bk::launch_scatter_f64(list, nq, res, out, s), "scatter launch");
@@ -633,8 +672,17 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
     ctx->partial.reserve((size_t)splits * nq * sizeof(double));
     dst = ctx->partial.as<double>();
   }
+  bk::BoundArgs ord{nullptr, 0};
+  if (self_excl) {
+    // self sums over kd-ordered points cull without any option
+    const PointSet &qs = query_set(ctx);
+    const bool f64 = path == Path::F64;
+    ord = live_order(ctx, path, p, f64 ? nullptr : qs.x.as<float>() + q_i, f64 ? nullptr : qs.y.as<float>() + q_i,
+                     f64 ? qs.xd.as<double>() + q_i : nullptr, f64 ? qs.yd.as<double>() + q_i : nullptr, nq,
+                     (long long)r_i, r_end, splits);
+  }
   hip_check(launch_path(ctx, path, p, nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits, dst, scale,
-                        self_excl),
+                        self_excl, ord.order ? &ord : nullptr),
             "direct kernel launch");
   if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, ctx->stream), "reduce launch");
 }
@@ -913,15 +961,17 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
       dst = ctx->partial.as<double>();
     }
     const double scale = path_scale(ctx, path, p, 1);
+    // the callers pass queries in Z-curve order: longest workgroups first
+    const BoundArgs ord = live_order(ctx, path, p, qx, qy, nullptr, nullptr, (long long)nq, 0, nr, splits);
+    const BoundArgs *op = ord.order ? &ord : nullptr;
     hipError_t e;
     if (path == Path::GAUSS32)
       e = launch_gauss(p, qx, qy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr, 0, 0,
                        nr, 0, (long long)nq, splits, dst, scale, 0,
-                       ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? GAUSS_DIRECT : GAUSS_AUTO, s);
+                       ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? GAUSS_DIRECT : GAUSS_AUTO, s, op);
     else
       e = launch_epan(p, qx, qy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(), nullptr, 0,
-                      0, nr, 0, (long long)nq,
-                      splits, dst, scale, 0, s);
+                      0, nr, 0, (long long)nq, splits, dst, scale, 0, s, op);
     hip_check(e, "direct kernel launch");
     if (splits > 1) hip_check(launch_reduce_splits(dst, splits, (long long)nq, out, s), "reduce launch");
   } catch (...) {
@@ -974,7 +1024,7 @@ int bk_destroy(bk_ctx *ctx) {
                         &ctx->weights, &ctx->scratch, &ctx->scalar, &ctx->ord_list, &ctx->ord_keys_a,
                         &ctx->ord_keys_b, &ctx->ord_tmp, &ctx->ord_x, &ctx->ord_y, &ctx->ord_out, &ctx->tsum,
                         &ctx->bnd_mask, &ctx->bnd_lo, &ctx->bnd_gap, &ctx->bnd_counts, &ctx->bnd_exact,
-                        &ctx->bnd_hi, &ctx->bnd_vals, &ctx->bnd_order})
+                        &ctx->bnd_hi, &ctx->bnd_vals, &ctx->bnd_order, &ctx->live_boxes})
         b->release();
       if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }

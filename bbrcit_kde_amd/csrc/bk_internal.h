@@ -94,6 +94,15 @@ struct BoundArgs {
 };
 // Reference split length of the range kernels (a multiple of `tile`).
 long long split_length(long long r_i, long long r_end, int splits, int tile);
+// For a culled direct launch over nq queries (blocks of qpb) and references
+// [r_i, r_end) in `splits` splits: keys[i] = ~(tiles of pair i = block +
+// split * nblocks that survive the kernel's own culling test), vals[i] = i.
+// kind: 0 FP32 Gaussian (meta = tile meta), 1 Epanechnikov on FP32 points,
+// 2 FP64 context (meta = tb); boxes receives one double4 per block.
+hipError_t launch_live_counts(const EvalParams &p, int kind, const float *qx32, const float *qy32, const double *qx64,
+                              const double *qy64, long long nq, int qpb, const void *meta, long long r_i,
+                              long long r_end, int splits, double4 *boxes, unsigned int *keys, int *vals,
+                              hipStream_t s);
 
 hipError_t launch_tile_summary_f32(const float *x, const float *y, const float *m, const float *abw, size_t n,
                                    double4 *tsum, hipStream_t s);
@@ -239,6 +248,10 @@ void spatial_sort(const double *qx, const double *qy, const long long *list, lon
 // pair order), so a bounded launch starts its longest workgroups first.
 void listed_order(const unsigned long long *mask, long long nwords, int nblocks, int splits, long long split_tiles,
                   int *order_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &vals, DevBuf &tmp, hipStream_t s);
+// order_out = vals sorted by keys (ascending, stable): with keys ~count, the
+// pairs with the most work first.
+void order_by_keys(unsigned int *keys, int *vals, long long n, int *order_out, DevBuf &keys_b, DevBuf &tmp,
+                   hipStream_t s);
 // gx[k] = qx[list[k]], gy[k] = qy[list[k]] for k < n.
 hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
                             float *gy, hipStream_t s);

@@ -331,6 +331,17 @@ __device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cn
     gauss_tile_loop<FORM, ISO, false>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
 }
 
+// A Gaussian tile is out of reach of a query box when W_min Q2 > kFarExp
+// between the boxes (its terms are below 2^-kFarExp of their masses).
+__device__ __forceinline__ bool gauss_tile_far(const float4 *__restrict__ tiles, long long tile_idx, float bx0,
+                                               float bx1, float by0, float by1, double gx, double gy) {
+  const float4 *m = tiles + tile_idx * kTileMeta;
+  const float4 c = m[0], b = m[kTileMeta - 1];
+  const double dx = fmax(0.0, fmax((double)c.x - (double)b.x - (double)bx1, (double)bx0 - (double)c.x - (double)b.x));
+  const double dy = fmax(0.0, fmax((double)c.y - (double)b.y - (double)by1, (double)by0 - (double)c.y - (double)b.y));
+  return (double)b.z * __builtin_fma(gy * dy, dy, gx * dx * dx) > (double)kFarExp;
+}
+
 // BOUNDED: only the tiles listed in bmask (the block's tile mask) are summed;
 // the bounds of the others are added by the caller.
 template <bool ISO, bool BOUNDED>
@@ -367,17 +378,8 @@ __device__ __forceinline__ void gauss_item(
   // first tile at or after `tl` that is not culled (or listed), or >= r_end
   auto live_from = [&](long long tl) {
     if constexpr (BOUNDED) return next_listed_tile<T>(bmask, tl, r_end);
-    for (; tl < r_end; tl += T) {
-      {
-        const float4 *m = tiles + (tl / T) * kTileMeta;
-        const float4 c = m[0], b = m[kTileMeta - 1];
-        const double dx =
-            fmax(0.0, fmax((double)c.x - (double)b.x - (double)bx1, (double)bx0 - (double)c.x - (double)b.x));
-        const double dy =
-            fmax(0.0, fmax((double)c.y - (double)b.y - (double)by1, (double)by0 - (double)c.y - (double)b.y));
-        if ((double)b.z * __builtin_fma(gy * dy, dy, gx * dx * dx) <= (double)kFarExp) break;
-      }
-    }
+    for (; tl < r_end; tl += T)
+      if (!gauss_tile_far(tiles, tl / T, bx0, bx1, by0, by1, gx, gy)) break;
     return tl;
   };
 
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
     long long split_len, long long q_i, long long nq, double *__restrict__ dst,
     double out_scale, int self_excl, float rho, double gx, double gy, int mode, BoundArgs ba) {
   long long bx = blockIdx.x, by = blockIdx.y;
-  if (BOUNDED && ba.order) {
+  if (ba.order) {
     const int it = ba.order[blockIdx.x];
     bx = it % ba.nblocks;
     by = it / ba.nblocks;
@@ -804,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void epan_range_kernel(
     long long r_i, long long r_end, long long split_len, long long q_i,
     long long nq, double *__restrict__ dst, double out_scale, int self_excl, BoundArgs ba) {
   long long bx = blockIdx.x, by = blockIdx.y;
-  if (BOUNDED && ba.order) {
+  if (ba.order) {
     const int it = ba.order[blockIdx.x];
     bx = it % ba.nblocks;
     by = it / ba.nblocks;
@@ -1081,7 +1083,7 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
     double rho, BoundArgs ba) {
   constexpr int kQb = kBlock64 * kQpt;
   long long bx = blockIdx.x, by = blockIdx.y;
-  if (BOUNDED && ba.order) {
+  if (ba.order) {
     const int it = ba.order[blockIdx.x];
     bx = it % ba.nblocks;
     by = it / ba.nblocks;
@@ -1677,6 +1679,82 @@ __global__ void bounds_combine_kernel(const double *__restrict__ exact, const do
   upper[k] = lo + factor * gap_b[b];
 }
 
+// ---- dispatch order of culled direct launches --------------------------------
+// Box of each block's queries as doubles (exact for FP32 and FP64
+// coordinates); padding slots repeat the block's first query, as the
+// evaluation kernels do.
+template <typename T>
+__global__ __launch_bounds__(256) void block_box_kernel(const T *__restrict__ qx, const T *__restrict__ qy,
+                                                        long long nq, int qpb, double4 *__restrict__ boxes) {
+  __shared__ double sh[4][256];
+  const long long q0 = (long long)blockIdx.x * qpb;
+  const double inf = __builtin_inf();
+  double x0 = inf, x1 = -inf, y0 = inf, y1 = -inf;
+  for (int k = threadIdx.x; k < qpb; k += 256) {
+    const long long q = q0 + k < nq ? q0 + k : q0;
+    const double x = (double)qx[q], y = (double)qy[q];
+    x0 = fmin(x0, x);
+    x1 = fmax(x1, x);
+    y0 = fmin(y0, y);
+    y1 = fmax(y1, y);
+  }
+  sh[0][threadIdx.x] = x0;
+  sh[1][threadIdx.x] = x1;
+  sh[2][threadIdx.x] = y0;
+  sh[3][threadIdx.x] = y1;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sh[0][threadIdx.x] = fmin(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+      sh[2][threadIdx.x] = fmin(sh[2][threadIdx.x], sh[2][threadIdx.x + o]);
+      sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) boxes[blockIdx.x] = make_double4(sh[0][0], sh[1][0], sh[2][0], sh[3][0]);
+}
+
+// keys[i] = ~(tiles of pair i = block + split * nblocks that the kernel's
+// own culling test keeps), vals[i] = i.  KIND: 0 FP32 Gaussian (tile meta
+// float4), 1 Epanechnikov on FP32 points, 2 FP64 context (splits start at
+// r_i instead of the tile below it).
+template <int KIND, int FAM>
+__global__ void live_count_kernel(const double4 *__restrict__ boxes, const void *__restrict__ meta, long long r_i,
+                                  long long r_end, long long len, int nblocks, int splits, double gx, double gy,
+                                  double rho, unsigned int *__restrict__ keys, int *__restrict__ vals) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nblocks * splits) return;
+  const long long b = i % nblocks, sp = i / nblocks;
+  const double4 box = boxes[b];
+  constexpr int T = KIND == 0 ? kGaussTile : kTile;
+  long long rb, re;
+  if (KIND == 2) {
+    rb = r_i + sp * len;
+    re = rb + len < r_end ? rb + len : r_end;
+  } else {
+    const long long base = (r_i / T) * T;
+    rb = base + sp * len;
+    re = rb + len < r_end ? rb + len : r_end;
+    if (rb < r_i) rb = r_i;
+  }
+  unsigned int c = 0;
+  for (long long tl = (rb / T) * T; tl < re; tl += T) {
+    bool far;
+    if constexpr (KIND == 0)
+      far = gauss_tile_far((const float4 *)meta, tl / T, (float)box.x, (float)box.y, (float)box.z, (float)box.w, gx,
+                           gy);
+    else if constexpr (KIND == 1)
+      far = epan_tile_far<FAM>((const double4 *)meta, tl / T, (float)box.x, (float)box.y, (float)box.z,
+                               (float)box.w);
+    else
+      far = f64_tile_far<FAM>((const double4 *)meta, tl / T, box.x, box.y, box.z, box.w, rho);
+    c += far ? 0u : 1u;
+  }
+  keys[i] = ~c;
+  vals[i] = (int)i;
+}
+
 }  // namespace
 
 long long split_length(long long r_i, long long r_end, int splits, int tile) {
@@ -1685,6 +1763,46 @@ long long split_length(long long r_i, long long r_end, int splits, int tile) {
   len = (len + tile - 1) / tile * tile;
   return len;
 }
+
+hipError_t launch_live_counts(const EvalParams &p, int kind, const float *qx32, const float *qy32, const double *qx64,
+                              const double *qy64, long long nq, int qpb, const void *meta, long long r_i,
+                              long long r_end, int splits, double4 *boxes, unsigned int *keys, int *vals,
+                              hipStream_t s) {
+  const long long nblocks = (nq + qpb - 1) / qpb;
+  if (nblocks <= 0) return hipSuccess;
+  if (kind == 2)
+    block_box_kernel<double><<<(unsigned)nblocks, 256, 0, s>>>(qx64, qy64, nq, qpb, boxes);
+  else
+    block_box_kernel<float><<<(unsigned)nblocks, 256, 0, s>>>(qx32, qy32, nq, qpb, boxes);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  long long len;
+  if (kind == 2) {
+    len = (r_end - r_i + splits - 1) / splits;
+    len = (len + kTile64 - 1) / kTile64 * kTile64;
+  } else {
+    len = split_length(r_i, r_end, splits, kind == 0 ? kGaussTile : kTile);
+  }
+  const long long n = nblocks * splits;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  const double rho = p.gy / p.gx;
+#define BK_LC(K, F)                                                                                            \
+  live_count_kernel<K, F><<<grid, 256, 0, s>>>(boxes, meta, r_i, r_end, len, (int)nblocks, splits, p.gx, p.gy, \
+                                               rho, keys, vals)
+  if (kind == 0) {
+    BK_LC(0, FAM_GAUSS);
+  } else {
+    switch (p.family) {
+      case FAM_GAUSS: BK_LC(2, FAM_GAUSS); break;
+      case FAM_EPAN: if (kind == 1) BK_LC(1, FAM_EPAN); else BK_LC(2, FAM_EPAN); break;
+      case FAM_EPANPROD: if (kind == 1) BK_LC(1, FAM_EPANPROD); else BK_LC(2, FAM_EPANPROD); break;
+      default: if (kind == 1) BK_LC(1, FAM_EPANCONV); else BK_LC(2, FAM_EPANCONV); break;
+    }
+  }
+#undef BK_LC
+  return hipGetLastError();
+}
+
 
 // ---- launchers -----------------------------------------------------------------
 
@@ -1771,7 +1889,7 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
 #define BK_GR(I, B)                                                                                              \
   gauss_range_kernel<I, B><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq, dst, out_scale, \
                                                    self_excl, rho, gx, gy, mode, b)
-    if (ba) {
+    if (ba && ba->mask) {
       if (iso)
         BK_GR(true, true);
       else
@@ -1824,7 +1942,7 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
 #define BK_ER(F, B)                                                                                 \
   epan_range_kernel<F, B><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, \
                                                   self_excl, b)
-    if (ba) {
+    if (ba && ba->mask) {
       switch (p.family) {
         case FAM_EPAN: BK_ER(FAM_EPAN, true); break;
         case FAM_EPANPROD: BK_ER(FAM_EPANPROD, true); break;
@@ -1886,7 +2004,7 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
 #define BK_R64(F, B)                                                          \
   direct_f64_range_kernel<F, B><<<grid, kBlock64, 0, s>>>(                    \
       qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho, b)
-    if (ba) {
+    if (ba && ba->mask) {
       switch (p.family) {
         case FAM_GAUSS: BK_R64(FAM_GAUSS, true); break;
         case FAM_EPAN: BK_R64(FAM_EPAN, true); break;

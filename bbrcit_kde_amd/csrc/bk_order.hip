@@ -142,6 +142,21 @@ hipError_t launch_scatter_f64(const long long *list, long long n, const double *
   return hipGetLastError();
 }
 
+void order_by_keys(unsigned int *keys, int *vals, long long n, int *order_out, DevBuf &keys_b, DevBuf &tmp,
+                   hipStream_t s) {
+  if (n <= 0) return;
+  if (n > 0x7fffffffLL) fail(BK_ERR_INVALID_ARG, "order_by_keys: too many items");
+  keys_b.reserve((size_t)n * sizeof(unsigned int));
+  size_t tmp_bytes = 0;
+  hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_b.as<unsigned int>(), vals, order_out,
+                                               (int)n, 0, 32, s),
+            "radix sort size query");
+  tmp.reserve(tmp_bytes + 256);
+  hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys, keys_b.as<unsigned int>(), vals, order_out,
+                                               (int)n, 0, 32, s),
+            "radix sort");
+}
+
 void listed_order(const unsigned long long *mask, long long nwords, int nblocks, int splits, long long split_tiles,
                   int *order_out, DevBuf &keys_a, DevBuf &keys_b, DevBuf &vals, DevBuf &tmp, hipStream_t s) {
   const long long n = (long long)nblocks * splits;
