@@ -36,7 +36,7 @@ namespace bk {
 namespace {
 
 constexpr float kMaskedLog = -1.0e30f;  // base-2 log mass of a masked reference
-constexpr float kGramMaxRho2 = 1.0f;      // compactness bound of the Gram form
+constexpr float kGramMaxRho2 = 4.0f;      // compactness bound of the Gram form
 #ifndef BK_FACTORED_RHO2
 #define BK_FACTORED_RHO2 9.0f
 #endif
