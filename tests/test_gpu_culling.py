@@ -125,23 +125,57 @@ def test_epanechnikov_culling_far_from_origin(bk, oracle, kind, h):
     assert (want > 0).mean() > 0.05
 
 
+@pytest.mark.parametrize("adaptive", [False, True])
 @pytest.mark.parametrize("kind,h", CASES)
-def test_f64_spatial_order_and_culling(bk, oracle, kind, h):
+def test_f64_spatial_order_and_culling(bk, oracle, kind, h, adaptive):
     # FP64 contexts: the same culling on FP64 boxes (Epanechnikov terms of a
     # culled tile are exactly zero, Gaussian ones below 2^-100 of their mass),
-    # held to the FP64 bar of 1e-12 in both orders
+    # held to the FP64 bar of 1e-12 in both orders; with adaptive bandwidths
+    # the reach of a tile follows its largest abw
     n_ref, n_q = 20_000, 7_000
     pts, mass = kd_mixture(n_ref, 900 + kind)
     q = ds.uniform_box(n_q, -4.0, 4.0, 950 + kind)
+    abw = np.random.default_rng(17).uniform(0.6, 1.6, n_ref) if adaptive else None
     desc = bk.KernelDesc(kind, 2, *h)
-    with bk.DirectKde(pts, q, mass, precision=bk.BK_PREC_F64) as kde:
+    with bk.DirectKde(pts, q, mass, abw, precision=bk.BK_PREC_F64) as kde:
         given = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_GIVEN)
         spatial = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_SPATIAL)
-    want = oracle.direct(kind, h[0], h[1], pts, q, mass=mass, normalize=True)
+    want = oracle.direct(kind, h[0], h[1], pts, q, mass=mass, abw=abw, normalize=True)
     bound = oracle.normalization(kind, 2, *h) * mass.sum()
-    assert_parity(spatial, want, bound, rtol=1e-12, what=f"f64 spatial kind {kind}")
-    assert_parity(given, want, bound, rtol=1e-12, what=f"f64 given kind {kind}")
+    assert_parity(spatial, want, bound, rtol=1e-12, what=f"f64 spatial kind {kind} adaptive {adaptive}")
+    assert_parity(given, want, bound, rtol=1e-12, what=f"f64 given kind {kind} adaptive {adaptive}")
     if kind in EPAN_KINDS:
+        np.testing.assert_array_equal(spatial, given)
+    assert (want > 0).mean() > 0.05
+
+
+CASES_1D = [(GAUSS, (0.02, 0.02)), (EPAN, (0.04, 0.04)), (EPANCONV1, (0.04, 0.04)), (GAUSSCONV1, (0.02, 0.02))]
+
+
+@pytest.mark.parametrize("f64", [False, True])
+@pytest.mark.parametrize("kind,h", CASES_1D)
+def test_1d_spatial_order_and_culling(bk, oracle, kind, h, f64):
+    # 1-D sets: the Z-curve order is the order along x; the culling and the
+    # bars are those of the 2-D cases (bit-identical orders for the
+    # Epanechnikov kinds, 1e-5 / 1e-12 against the oracle)
+    n_ref, n_q = 20_000, 7_000
+    pts, w = ds.gaussian_mixture(n_ref, seed=1100 + kind)
+    pts = pts[:, :1]
+    perm = ds.kd_order(pts, 64)
+    pts, mass = pts[perm], (w / w.sum())[perm]
+    q = ds.uniform_box(n_q, -4.0, 4.0, 1150 + kind, dim=1)
+    desc = bk.KernelDesc(kind, 1, *h)
+    prec = bk.BK_PREC_F64 if f64 else bk.BK_PREC_F32
+    with bk.DirectKde(pts, q, mass, precision=prec) as kde:
+        given = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_GIVEN)
+        spatial = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_SPATIAL)
+    P, Q, M = (pts, q, mass) if f64 else (f32(pts), f32(q), f32(mass))
+    want = oracle.direct(kind, h[0], h[1], P, Q, mass=M, normalize=True)
+    bound = oracle.normalization(kind, 1, *h) * mass.sum()
+    rtol = 1e-12 if f64 else 1e-5
+    assert_parity(spatial, want, bound, rtol=rtol, what=f"1-D spatial kind {kind} f64 {f64}")
+    assert_parity(given, want, bound, rtol=rtol, what=f"1-D given kind {kind} f64 {f64}")
+    if kind in (EPAN, EPANCONV1):
         np.testing.assert_array_equal(spatial, given)
     assert (want > 0).mean() > 0.05
 
