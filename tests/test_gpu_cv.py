@@ -5,7 +5,9 @@ LCV and LSCV are FP64 reductions over FP32 per-point sums; each per-point
 sum carries <= 1e-5 relative error, so the scores are compared with an
 absolute tolerance of 1e-5 * sum_i w_i |term_i| (a bound on the propagated
 error).  lambda of the adaptive pass is a power -alpha of the pilot, so its
-relative error is <= alpha * 1e-5.
+relative error is <= alpha * 1e-5.  FP64 contexts (the reference's double
+kernels) are held to 1e-12 on per-point sums, 1e-10 on the scores and 1e-11 on
+the adaptive pass, against the oracle on the FP64 points.
 """
 from __future__ import annotations
 
@@ -101,6 +103,68 @@ def test_likelihood_and_lsq_cv(bk, oracle, kind, dim, h):
     else:
         assert abs(lcv - want_lcv) <= 1e-5 * (1 + abs(want_lcv))
     assert abs(lscv - want_lscv) <= 1e-5 * abs(want_lscv) + 1e-8
+
+
+def test_f64_leaf_pairs_and_self_sums(bk, oracle):
+    # FP64 contexts (the reference's double kernels): leaf pairs and the
+    # self sums with and without the self pair, at the FP64 bar of 1e-12
+    pts, mass = mixture(5000, 131)
+    q = ds.uniform_box(3000, -3, 3, 132)
+    rng = np.random.default_rng(3)
+    pairs = []
+    for _ in range(20):
+        r_i = int(rng.integers(0, 5000))
+        r_j = int(rng.integers(r_i, min(5000, r_i + 2500)))
+        q_i = int(rng.integers(0, 3000))
+        q_j = int(rng.integers(q_i, min(3000, q_i + 2000)))
+        pairs.append((r_i, r_j, q_i, q_j))
+    pairs.append((0, 4999, 0, 2999))
+    for kind, h in ((GAUSSPROD, (0.09, 0.13)), (EPANPROD, (0.2, 0.25))):
+        desc = bk.KernelDesc(kind, 2, *h)
+        with bk.DirectKde(pts, q, mass, precision=bk.BK_PREC_F64) as kde:
+            outs = kde.eval_leaf_pairs(desc, pairs)
+        for (r_i, r_j, q_i, q_j), got in zip(pairs, outs):
+            want = oracle.direct(kind, *h, pts[r_i:r_j + 1], q[q_i:q_j + 1], mass=mass[r_i:r_j + 1])
+            assert_parity(got, want, mass[r_i:r_j + 1].sum(), rtol=1e-12, what=f"f64 kind {kind} pair {r_i, r_j}")
+        with bk.DirectKde(pts, None, mass, precision=bk.BK_PREC_F64) as kde:
+            full = kde.self_eval(desc, exclude_self=False, normalize=False)
+            loo = kde.self_eval(desc, exclude_self=True, normalize=False)
+        assert_parity(full, oracle.direct(kind, *h, pts, pts, mass=mass), mass.sum(), rtol=1e-12, what="f64 full")
+        assert_parity(loo, oracle.direct(kind, *h, pts, pts, mass=mass, exclude_self=True), mass.sum(),
+                      rtol=1e-12, what="f64 loo")
+
+
+@pytest.mark.parametrize("kind,dim,h", [(GAUSSPROD, 2, (0.05, 0.07)), (EPANPROD, 2, (0.15, 0.2)),
+                                        (GAUSS, 1, (0.05, 0.05)), (EPAN, 1, (0.12, 0.12))])
+def test_f64_likelihood_and_lsq_cv(bk, oracle, kind, dim, h):
+    # per-point sums within 1e-12: the FP64 scores within 1e-10 relative
+    pts, w = mixture(6000, 160 + kind)
+    pts = pts[:, :dim]
+    desc = bk.KernelDesc(kind, dim, *h)
+    with bk.DirectKde(pts, None, w, precision=bk.BK_PREC_F64) as kde:
+        lcv = kde.likelihood_cv(desc, w)
+        lscv = kde.lsq_conv_cv(desc, w)
+    want_lcv = oracle.likelihood_cv(kind, *h, pts, w, mass=w)
+    want_lscv = oracle.lsq_conv_cv(kind, *h, pts, w, mass=w)
+    if np.isinf(want_lcv):
+        assert lcv == want_lcv
+    else:
+        assert abs(lcv - want_lcv) <= 1e-10 * (1 + abs(want_lcv))
+    assert abs(lscv - want_lscv) <= 1e-10 * abs(want_lscv) + 1e-12
+
+
+def test_f64_adapt_matches_oracle(bk, oracle):
+    pts, w = mixture(6000, 177)
+    desc = bk.KernelDesc(GAUSSPROD, 2, 0.05, 0.05)
+    with bk.DirectKde(pts, None, w, precision=bk.BK_PREC_F64) as kde:
+        g, lam = kde.adapt(desc, 0.5, w)
+        after = kde.self_eval(desc, exclude_self=False, normalize=True)
+    g_want, lam_want = oracle.adapt(GAUSSPROD, 0.05, 0.05, pts, w, 0.5)
+    assert g == pytest.approx(g_want, rel=1e-11)
+    np.testing.assert_allclose(lam, lam_want, rtol=1e-11)
+    want_after = oracle.direct(GAUSSPROD, 0.05, 0.05, pts, pts, mass=w * lam ** -2.0, abw=lam, normalize=True)
+    assert_parity(after, want_after, oracle.normalization(GAUSSPROD, 2, 0.05, 0.05) * (w * lam ** -2.0).sum(),
+                  rtol=1e-12, what="f64 adapted self density")
 
 
 def test_adapt_matches_oracle(bk, oracle):
