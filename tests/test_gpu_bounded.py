@@ -73,15 +73,18 @@ def test_bounded_2d_meets_contract(bk, oracle, kind, h, adaptive, f64):
     assert st["excluded"] + st["approximated"] > 0 and st["evaluated"] > 0
 
 
+@pytest.mark.parametrize("f64", [False, True])
 @pytest.mark.parametrize("kind,h", CASES_1D)
-def test_bounded_1d_meets_contract(bk, oracle, kind, h):
+def test_bounded_1d_meets_contract(bk, oracle, kind, h, f64):
     pts, mass = kd_points(12_000, 900 + kind, 1)
     q = np.sort(ds.uniform_box(5_000, -4.0, 4.0, 901 + kind, dim=1), axis=0)
     desc = bk.KernelDesc(kind, 1, *h)
-    with bk.DirectKde(pts, q, mass) as kde:
+    prec = bk.BK_PREC_F64 if f64 else bk.BK_PREC_F32
+    with bk.DirectKde(pts, q, mass, precision=prec) as kde:
         lo, up, st = kde.eval_bounded(desc, 0, len(q) - 1, 1e-3, 1e-7)
-    exact = oracle.direct(kind, h[0], h[1], f32(pts), f32(q), mass=f32(mass), normalize=True)
-    check_bounds(lo, up, exact, oracle.normalization(kind, 1, *h), 1e-3, 1e-7, False, f"1-D kind {kind}")
+    P, Q, M = (pts, q, mass) if f64 else (f32(pts), f32(q), f32(mass))
+    exact = oracle.direct(kind, h[0], h[1], P, Q, mass=M, normalize=True)
+    check_bounds(lo, up, exact, oracle.normalization(kind, 1, *h), 1e-3, 1e-7, f64, f"1-D kind {kind} f64 {f64}")
     assert st["evaluated"] > 0
 
 
