@@ -11,6 +11,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -60,15 +61,19 @@ def build_library(verbose: bool = False) -> Path:
     LIB_DIR.mkdir(parents=True, exist_ok=True)
     build_dir = PKG_DIR / "_build"
     build_dir.mkdir(exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in HIP_SOURCES:
         obj = build_dir / (Path(src).stem + ".o")
         cmd = [hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src, []), "-I", str(REPO_DIR / "include"), "-c",
                str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(str(obj))
+    # the sources compile independently: one hipcc per source, at most 4 at once
+    with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB_PATH.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:
