@@ -161,6 +161,9 @@ def load_library() -> ctypes.CDLL:
         "bk_eval_range_device": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
                                                 ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p,
                                                 ctypes.c_int]),
+        "bk_eval_range_device_noalloc": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
+                                                        ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                                        ctypes.c_void_p, ctypes.c_int]),
         "bk_eval_leaf_pairs": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.POINTER(RangePair),
                                               ctypes.c_size_t, _DBL_P, ctypes.c_int]),
         "bk_eval_bounded": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t, ctypes.c_size_t,
@@ -343,6 +346,12 @@ class DirectKde:
         """Queue an evaluation writing doubles to device memory (no sync)."""
         _check(self._lib.bk_eval_range_device(self._ctx, ctypes.byref(desc), r_i, r_j, q_i, q_j,
                                               ctypes.c_void_p(int(out_dev_ptr)), int(bool(normalize))))
+
+    def eval_device_noalloc(self, desc: KernelDesc, r_i: int, r_j: int, q_i: int, q_j: int, out_dev_ptr: int,
+                            normalize: bool = True) -> None:
+        """As eval_device, through the entry point that allocates nothing."""
+        _check(self._lib.bk_eval_range_device_noalloc(self._ctx, ctypes.byref(desc), r_i, r_j, q_i, q_j,
+                                                      ctypes.c_void_p(int(out_dev_ptr)), int(bool(normalize))))
 
     def eval_leaf_pairs(self, desc: KernelDesc, pairs: Sequence[Sequence[int]], normalize: bool = False) -> list:
         """Evaluate many (r_i, r_j, q_i, q_j) pairs in one launch; one array per pair."""

@@ -415,6 +415,80 @@ int tile_of(Path path) {
   return (path == Path::GENERAL || path == Path::GRAM) ? 128 : 256;
 }
 
+// Device addresses of the per-reference features of one kernel descriptor
+// (f1 and tiles may be unused by a path).
+struct FeatPtrs {
+  void *f0 = nullptr, *f1 = nullptr, *tiles = nullptr;
+};
+
+FeatPtrs ctx_feats(const bk_ctx *ctx) { return FeatPtrs{ctx->f0.ptr, ctx->f1.ptr, ctx->tiles.ptr}; }
+
+// Bytes of the three feature arrays of a path for n references.
+struct FeatBytes {
+  size_t f0 = 0, f1 = 0, tiles = 0;
+};
+
+FeatBytes feature_bytes(Path path, size_t n) {
+  FeatBytes b;
+  switch (path) {
+    case Path::F64:
+      b.f0 = b.f1 = n * sizeof(double4);
+      b.tiles = (n + bk::kTile - 1) / bk::kTile * 2 * sizeof(double4);
+      break;
+    case Path::GAUSS32:
+      b.f0 = b.f1 = n * sizeof(float4);
+      b.tiles = (n + bk::kGaussTile - 1) / bk::kGaussTile * bk::kGaussTileMeta * sizeof(float4);
+      break;
+    case Path::EPAN64:
+      b.f0 = b.f1 = n * sizeof(double4);
+      b.tiles = (n + bk::kTile - 1) / bk::kTile * bk::kEpanTileMeta * sizeof(double4);
+      break;
+    case Path::GENERAL:
+      b.f0 = n * sizeof(double2);
+      break;
+    case Path::GRAM:
+      b.f0 = n * sizeof(double4);
+      break;
+  }
+  return b;
+}
+
+// Prepare the features of `p` for the context's reference set into fp.
+void launch_prep(bk_ctx *ctx, Path path, const EvalParams &p, const FeatPtrs &fp) {
+  PointSet &r = ctx->refs;
+  const size_t n = r.n;
+  const hipStream_t s = ctx->stream;
+  switch (path) {
+    case Path::F64:
+      hip_check(bk::launch_prep_f64(p, r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
+                                    r.abwd.as<double>(), n, static_cast<double4 *>(fp.f0),
+                                    static_cast<double4 *>(fp.f1), static_cast<double4 *>(fp.tiles), s),
+                "prep_f64 launch");
+      break;
+    case Path::GAUSS32:
+      hip_check(bk::launch_prep_gauss(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
+                                      mass_scale(r), static_cast<float4 *>(fp.f0), static_cast<float4 *>(fp.f1),
+                                      static_cast<float4 *>(fp.tiles), s),
+                "prep_gauss launch");
+      break;
+    case Path::EPAN64:
+      hip_check(bk::launch_prep_epan(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
+                                     static_cast<double4 *>(fp.f0), static_cast<double4 *>(fp.f1),
+                                     static_cast<double4 *>(fp.tiles), s),
+                "prep_epan launch");
+      break;
+    case Path::GENERAL:
+      hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, static_cast<double2 *>(fp.f0), s),
+                "prep_general launch");
+      break;
+    case Path::GRAM:
+      hip_check(bk::launch_prep_gram(p, r.coords.as<float>(), r.abw.as<float>(), r.m.as<float>(), n,
+                                     static_cast<double4 *>(fp.f0), s),
+                "prep_gram launch");
+      break;
+  }
+}
+
 // Make sure the per-reference features of `desc` are on the device.
 void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &p) {
   PointSet &r = ctx->refs;
@@ -425,50 +499,11 @@ void ensure_features(bk_ctx *ctx, const bk_kernel_desc *desc, const EvalParams &
   ctx->feat_valid = false;
   const size_t n = r.n;
   if (n == 0) return;
-  const hipStream_t s = ctx->stream;
-  switch (want) {
-    case Path::F64:
-      ctx->f0.reserve(n * sizeof(double4));
-      ctx->f1.reserve(n * sizeof(double4));
-      ctx->tiles.reserve((n + bk::kTile - 1) / bk::kTile * 2 * sizeof(double4));
-      hip_check(bk::launch_prep_f64(p, r.xd.as<double>(), r.yd.as<double>(), r.md.as<double>(),
-                                    r.abwd.as<double>(), n, ctx->f0.as<double4>(), ctx->f1.as<double4>(),
-                                    ctx->tiles.as<double4>(), s),
-                "prep_f64 launch");
-      break;
-    case Path::GAUSS32: {
-      const size_t ntiles = (n + bk::kGaussTile - 1) / bk::kGaussTile;
-      ctx->f0.reserve(n * sizeof(float4));
-      ctx->f1.reserve(n * sizeof(float4));
-      ctx->tiles.reserve(ntiles * bk::kGaussTileMeta * sizeof(float4));
-      hip_check(bk::launch_prep_gauss(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
-                                      mass_scale(r), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
-                                      ctx->tiles.as<float4>(), s),
-                "prep_gauss launch");
-      break;
-    }
-    case Path::EPAN64: {
-      const size_t ntiles = (n + bk::kTile - 1) / bk::kTile;
-      ctx->f0.reserve(n * sizeof(double4));
-      ctx->f1.reserve(n * sizeof(double4));
-      ctx->tiles.reserve(ntiles * bk::kEpanTileMeta * sizeof(double4));
-      hip_check(bk::launch_prep_epan(p, r.x.as<float>(), r.y.as<float>(), r.m.as<float>(), r.abw.as<float>(), n,
-                                     ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(), s),
-                "prep_epan launch");
-      break;
-    }
-    case Path::GENERAL:
-      ctx->f0.reserve(n * sizeof(double2));
-      hip_check(bk::launch_prep_general(p, r.abw.as<float>(), r.m.as<float>(), n, ctx->f0.as<double2>(), s),
-                "prep_general launch");
-      break;
-    case Path::GRAM:
-      ctx->f0.reserve(n * sizeof(double4));
-      hip_check(bk::launch_prep_gram(p, r.coords.as<float>(), r.abw.as<float>(), r.m.as<float>(), n,
-                                     ctx->f0.as<double4>(), s),
-                "prep_gram launch");
-      break;
-  }
+  const FeatBytes fb = feature_bytes(want, n);
+  ctx->f0.reserve(fb.f0);
+  if (fb.f1) ctx->f1.reserve(fb.f1);
+  if (fb.tiles) ctx->tiles.reserve(fb.tiles);
+  launch_prep(ctx, want, p, ctx_feats(ctx));
   ctx->feat_path = (int)want;
   ctx->feat_desc = *desc;
   ctx->feat_version = r.version;
@@ -531,34 +566,35 @@ double path_scale(const bk_ctx *ctx, Path path, const EvalParams &p, int normali
 }
 
 // Launch the evaluation kernels of one call: range mode (items == nullptr)
-// or work-item mode.
-hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const WorkItem *items, int n_items,
-                       long long r_i, long long r_end, long long q_i, long long nq, int splits, double *dst,
-                       double scale, int self_excl, const bk::BoundArgs *order = nullptr) {
+// or work-item mode, with the features at fp.
+hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const FeatPtrs &fp, const WorkItem *items,
+                       int n_items, long long r_i, long long r_end, long long q_i, long long nq, int splits,
+                       double *dst, double scale, int self_excl, const bk::BoundArgs *order = nullptr) {
   const hipStream_t s = ctx->stream;
   const PointSet &r = ctx->refs;
   const PointSet &q = query_set(ctx);
   switch (path) {
     case Path::F64:
-      return bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), ctx->f0.as<double4>(),
-                                   ctx->f1.as<double4>(), ctx->tiles.as<double4>(), items, n_items, r_i, r_end, q_i,
-                                   nq, splits, dst, scale, self_excl, s, order);
+      return bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), static_cast<const double4 *>(fp.f0),
+                                   static_cast<const double4 *>(fp.f1), static_cast<const double4 *>(fp.tiles),
+                                   items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s, order);
     case Path::GAUSS32:
-      return bk::launch_gauss(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<float4>(), ctx->f1.as<float4>(),
-                              ctx->tiles.as<float4>(), items, n_items, r_i, r_end, q_i, nq, splits, dst, scale,
-                              self_excl,
+      return bk::launch_gauss(p, q.x.as<float>(), q.y.as<float>(), static_cast<const float4 *>(fp.f0),
+                              static_cast<const float4 *>(fp.f1), static_cast<const float4 *>(fp.tiles), items,
+                              n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl,
                               ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, order);
     case Path::EPAN64:
-      return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), ctx->f0.as<double4>(), ctx->f1.as<double4>(),
-                             ctx->tiles.as<double4>(),
-                             items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s, order);
+      return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), static_cast<const double4 *>(fp.f0),
+                             static_cast<const double4 *>(fp.f1), static_cast<const double4 *>(fp.tiles), items,
+                             n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s, order);
     case Path::GRAM:
-      return bk::launch_gram(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double4>(), items, n_items,
-                             r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
+      return bk::launch_gram(p, q.coords.as<float>(), r.coords.as<float>(), static_cast<const double4 *>(fp.f0),
+                             items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
     case Path::GENERAL:
     default:
-      return bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(), ctx->f0.as<double2>(),
-                                       items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
+      return bk::launch_direct_general(p, q.coords.as<float>(), r.coords.as<float>(),
+                                       static_cast<const double2 *>(fp.f0), items, n_items, r_i, r_end, q_i, nq,
+                                       splits, dst, scale, self_excl, s);
   }
 }
 
@@ -681,8 +717,8 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                      f64 ? qs.xd.as<double>() + q_i : nullptr, f64 ? qs.yd.as<double>() + q_i : nullptr, nq,
                      (long long)r_i, r_end, splits);
   }
-  hip_check(launch_path(ctx, path, p, nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits, dst, scale,
-                        self_excl, ord.order ? &ord : nullptr),
+  hip_check(launch_path(ctx, path, p, ctx_feats(ctx), nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits,
+                        dst, scale, self_excl, ord.order ? &ord : nullptr),
             "direct kernel launch");
   if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, ctx->stream), "reduce launch");
 }
@@ -982,6 +1018,80 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
 }
 }  // namespace bk
 
+// ---- allocation-free range evaluation ------------------------------------------
+// bk_eval_range_device_noalloc keeps the reference features and the split
+// partial sums of a call in one file-scope device array per GPU instead of
+// buffers the context allocates.  Host code never passes the array's name to
+// a kernel: its device address is looked up once per device (in bk_create,
+// with that device current) and kept in scratch_ptr[device].
+namespace {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB per device
+constexpr int kScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex scratch_mu;
+void *scratch_ptr[kScratchDevices] = {};
+
+// Device address of `device`'s scratch array; `device` must be current.
+void *scratch_for(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    fail(BK_ERR_UNSUPPORTED, "device index " + std::to_string(device) + " beyond the scratch table");
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void *ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) fail(BK_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    scratch_ptr[device] = ptr;
+  }
+  return scratch_ptr[device];
+}
+
+size_t round_up_256(size_t b) { return (b + 255) / 256 * 256; }
+
+// As eval_range_dev (queries in their stored order, no self exclusion), with
+// every buffer beyond the context's points and `out` inside the scratch: the
+// features of the whole reference set first, the split partial sums after
+// them (the split count is lowered until they fit; one split writes `out`
+// directly).  Nothing here allocates, and nothing needs clearing: the
+// features and every partial sum are written before they are read.
+void eval_range_noalloc(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t r_j, size_t q_i, size_t q_j,
+                        double *out, int normalize) {
+  const EvalParams p = params_of(desc);
+  check_compatible(ctx, p);
+  check_range(ctx, r_i, r_j, q_i, q_j);
+  const Path path = path_of(ctx, p);
+  const FeatBytes fb = feature_bytes(path, ctx->refs.n);
+  const size_t feat = round_up_256(fb.f0) + round_up_256(fb.f1) + round_up_256(fb.tiles);
+  if (feat > kScratchBytes)
+    fail(BK_ERR_INVALID_ARG, "bk_eval_range_device_noalloc: the features of " + std::to_string(ctx->refs.n) +
+                                 " references need " + std::to_string(feat) + " bytes, more than the " +
+                                 std::to_string(kScratchBytes) + "-byte scratch (use bk_eval_range_device)");
+  char *base = static_cast<char *>(scratch_for(ctx->device));
+  FeatPtrs fp;
+  fp.f0 = base;
+  fp.f1 = base + round_up_256(fb.f0);
+  fp.tiles = base + round_up_256(fb.f0) + round_up_256(fb.f1);
+  launch_prep(ctx, path, p, fp);
+  const long long nq = (long long)(q_j - q_i + 1);
+  const long long nr = (long long)(r_j - r_i + 1);
+  const long long r_end = (long long)r_j + 1;
+  const double scale = path_scale(ctx, path, p, normalize);
+  const long long slots =
+      path == Path::GAUSS32 ? (long long)bk::gauss_resident_blocks() * bk::device_cu_count() : 0;
+  long long splits = choose_splits(nq, nr, queries_per_item(path), tile_of(path), slots);
+  const long long fit = (long long)((kScratchBytes - feat) / ((size_t)nq * sizeof(double)));
+  if (splits > fit) splits = fit >= 2 ? fit : 1;
+  double *dst = splits > 1 ? reinterpret_cast<double *>(base + feat) : out;
+  hip_check(launch_path(ctx, path, p, fp, nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, (int)splits, dst,
+                        scale, 0),
+            "direct kernel launch");
+  if (splits > 1) hip_check(bk::launch_reduce_splits(dst, (int)splits, nq, out, ctx->stream), "reduce launch");
+}
+
+}  // namespace
+
 // ---- C ABI -----------------------------------------------------------------
 
 extern "C" {
@@ -1008,6 +1118,13 @@ int bk_create(bk_ctx **ctx_out, int device, int precision) {
       fail(BK_ERR_HIP, "hipStreamCreateWithFlags failed");
     }
     c->stream = c->own_stream;
+    try {
+      (void)scratch_for(device);  // the device address of this GPU's scratch array
+    } catch (...) {
+      (void)hipStreamDestroy(c->own_stream);
+      delete c;
+      throw;
+    }
     *ctx_out = c;
   });
 }
@@ -1129,6 +1246,16 @@ int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, si
   });
 }
 
+int bk_eval_range_device_noalloc(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t r_j, size_t q_i,
+                                 size_t q_j, double *out_dev, int normalize) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!out_dev) fail(BK_ERR_INVALID_ARG, "null output array");
+    DeviceGuard g(ctx->device);
+    eval_range_noalloc(ctx, desc, r_i, r_j, q_i, q_j, out_dev, normalize);
+  });
+}
+
 int bk_eval_bounded_device(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_t q_j, double rel_err,
                            double abs_err, double *lower_dev, double *upper_dev, int normalize,
                            bk_bound_stats *stats) {
@@ -1191,8 +1318,8 @@ int bk_eval_leaf_pairs(bk_ctx *ctx, const bk_kernel_desc *desc, const bk_range_p
                              s),
               "upload work items");
     double *dev = staging_out(ctx, (size_t)total);
-    const hipError_t e = launch_path(ctx, path, p, ctx->items.as<WorkItem>(), (int)items.size(), 0, 0, 0, 0, 1,
-                                     dev, path_scale(ctx, path, p, normalize), 0);
+    const hipError_t e = launch_path(ctx, path, p, ctx_feats(ctx), ctx->items.as<WorkItem>(), (int)items.size(), 0,
+                                     0, 0, 0, 1, dev, path_scale(ctx, path, p, normalize), 0);
     hip_check(e, "leaf-pair kernel launch");
     copy_to_host(ctx, out_host, dev, (size_t)total);
   });

@@ -179,6 +179,28 @@ int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc,
                          size_t r_i, size_t r_j, size_t q_i, size_t q_j,
                          double *out_dev, int normalize);
 
+/* As bk_eval_range_device, allocating nothing: no hipMalloc, hipFree or
+ * allocating library call, on any call.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * The reference features of desc and the split partial sums live in one
+ * 64 MiB device array per GPU that the library holds until the process
+ * ends (its address is looked up in bk_create).  Limit: the features of the
+ * context's whole reference set must fit in it (FP32 Gaussian kinds
+ * 32.4 bytes per reference, about 2.07M references; Epanechnikov kinds and
+ * FP64 contexts 64.25 bytes, about 1.04M; D >= 3 16 or 32 bytes); larger
+ * sets fail with BK_ERR_INVALID_ARG before any work is queued.  The split
+ * count is lowered until the partial sums fit beside the features, so
+ * results agree with bk_eval_range_device up to the FP64 order of the split
+ * combination.  Queries are always taken in their stored order
+ * (BK_OPT_QUERY_ORDER is ignored).  Every call on one GPU shares that GPU's
+ * array, which keeps the last call's contents: calls that overlap on one
+ * GPU (other streams, other host threads, other contexts) race; order them
+ * on one stream or synchronise between them. */
+int bk_eval_range_device_noalloc(bk_ctx *ctx, const bk_kernel_desc *desc,
+                                 size_t r_i, size_t r_j, size_t q_i, size_t q_j,
+                                 double *out_dev, int normalize);
+
 /* Batched leaf-pair evaluation: all n_pairs (reference range, query range)
  * pairs in one launch.  The results of pair p are written contiguously,
  * pair after pair: out_host[off_p + (k - q_i)] with off_p the sum of the
