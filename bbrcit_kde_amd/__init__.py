@@ -48,6 +48,7 @@ BK_HIGH_DIM_DIRECT = 1
 BK_OPT_QUERY_ORDER = 3
 BK_QUERY_ORDER_GIVEN = 0
 BK_QUERY_ORDER_SPATIAL = 1
+BK_OPT_COUNT_TERMS = 4
 
 BK_MLP_NUM_PARAMS = 33537
 
@@ -161,6 +162,8 @@ def load_library() -> ctypes.CDLL:
         "bk_eval_range_device": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
                                                 ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p,
                                                 ctypes.c_int]),
+        "bk_term_count": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(ctypes.c_ulonglong),
+                                         ctypes.POINTER(ctypes.c_ulonglong)]),
         "bk_eval_range_device_noalloc": (ctypes.c_int, [c_ctx_p, ctypes.POINTER(KernelDesc), ctypes.c_size_t,
                                                         ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                                                         ctypes.c_void_p, ctypes.c_int]),
@@ -346,6 +349,13 @@ class DirectKde:
         """Queue an evaluation writing doubles to device memory (no sync)."""
         _check(self._lib.bk_eval_range_device(self._ctx, ctypes.byref(desc), r_i, r_j, q_i, q_j,
                                               ctypes.c_void_p(int(out_dev_ptr)), int(bool(normalize))))
+
+    def term_count(self) -> tuple[int, int]:
+        """(lane terms, real-query terms) evaluated since counting was switched on
+        (set_option(BK_OPT_COUNT_TERMS, 1)) or last read; resets them."""
+        a, b = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        _check(self._lib.bk_term_count(self._ctx, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
 
     def eval_device_noalloc(self, desc: KernelDesc, r_i: int, r_j: int, q_i: int, q_j: int, out_dev_ptr: int,
                             normalize: bool = True) -> None:

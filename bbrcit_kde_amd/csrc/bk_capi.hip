@@ -71,6 +71,8 @@ struct bk_ctx {
   int high_dim_form = BK_HIGH_DIM_AUTO;
   int feat_path = -1;  // Path of the cached features
   int query_order = BK_QUERY_ORDER_GIVEN;
+  int count_terms = 0;  // BK_OPT_COUNT_TERMS
+  DevBuf term_count;    // two counters of the instrumented Gaussian kernel
   DevBuf ord_list, ord_keys_a, ord_keys_b, ord_tmp, ord_x, ord_y, ord_out;  // spatial query order
   // bounded evaluation: tile summaries of the reference set (per version),
   // per-block thresholds, pruning counters, bounds planes
@@ -689,8 +691,9 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                      ctx->ord_tmp, s);
     float *ox = ctx->ord_x.as<float>(), *oy = ctx->ord_y.as<float>();
     hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
-    const bk::BoundArgs ord = live_order(ctx, path, p, ox, oy, nullptr, nullptr, nq, (long long)r_i, r_end, splits);
-    const bk::BoundArgs *op = ord.order ? &ord : nullptr;
+    bk::BoundArgs ord = live_order(ctx, path, p, ox, oy, nullptr, nullptr, nq, (long long)r_i, r_end, splits);
+    if (ctx->count_terms && path == Path::GAUSS32) ord.count = ctx->term_count.as<unsigned long long>();
+    const bk::BoundArgs *op = (ord.order || ord.count) ? &ord : nullptr;
     if (path == Path::GAUSS32)
       e = bk::launch_gauss(p, ox, oy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
                            0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0,
@@ -717,8 +720,9 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                      f64 ? qs.xd.as<double>() + q_i : nullptr, f64 ? qs.yd.as<double>() + q_i : nullptr, nq,
                      (long long)r_i, r_end, splits);
   }
+  if (ctx->count_terms && path == Path::GAUSS32) ord.count = ctx->term_count.as<unsigned long long>();
   hip_check(launch_path(ctx, path, p, ctx_feats(ctx), nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits,
-                        dst, scale, self_excl, ord.order ? &ord : nullptr),
+                        dst, scale, self_excl, (ord.order || ord.count) ? &ord : nullptr),
             "direct kernel launch");
   if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, out, ctx->stream), "reduce launch");
 }
@@ -998,8 +1002,9 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
     }
     const double scale = path_scale(ctx, path, p, 1);
     // the callers pass queries in Z-curve order: longest workgroups first
-    const BoundArgs ord = live_order(ctx, path, p, qx, qy, nullptr, nullptr, (long long)nq, 0, nr, splits);
-    const BoundArgs *op = ord.order ? &ord : nullptr;
+    BoundArgs ord = live_order(ctx, path, p, qx, qy, nullptr, nullptr, (long long)nq, 0, nr, splits);
+    if (ctx->count_terms && path == Path::GAUSS32) ord.count = ctx->term_count.as<unsigned long long>();
+    const BoundArgs *op = (ord.order || ord.count) ? &ord : nullptr;
     hipError_t e;
     if (path == Path::GAUSS32)
       e = launch_gauss(p, qx, qy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr, 0, 0,
@@ -1141,7 +1146,7 @@ int bk_destroy(bk_ctx *ctx) {
                         &ctx->weights, &ctx->scratch, &ctx->scalar, &ctx->ord_list, &ctx->ord_keys_a,
                         &ctx->ord_keys_b, &ctx->ord_tmp, &ctx->ord_x, &ctx->ord_y, &ctx->ord_out, &ctx->tsum,
                         &ctx->bnd_mask, &ctx->bnd_lo, &ctx->bnd_gap, &ctx->bnd_counts, &ctx->bnd_exact,
-                        &ctx->bnd_hi, &ctx->bnd_vals, &ctx->bnd_order, &ctx->live_boxes})
+                        &ctx->bnd_hi, &ctx->bnd_vals, &ctx->bnd_order, &ctx->live_boxes, &ctx->term_count})
         b->release();
       if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     }
@@ -1174,6 +1179,16 @@ int bk_set_option(bk_ctx *ctx, int option, int value) {
         if (value != BK_QUERY_ORDER_GIVEN && value != BK_QUERY_ORDER_SPATIAL)
           fail(BK_ERR_INVALID_ARG, "BK_OPT_QUERY_ORDER takes BK_QUERY_ORDER_GIVEN or BK_QUERY_ORDER_SPATIAL");
         ctx->query_order = value;
+        break;
+      case BK_OPT_COUNT_TERMS:
+        if (value != 0 && value != 1) fail(BK_ERR_INVALID_ARG, "BK_OPT_COUNT_TERMS takes 0 or 1");
+        if (value) {
+          DeviceGuard g(ctx->device);
+          ctx->term_count.reserve(2 * sizeof(unsigned long long));
+          hip_check(hipMemsetAsync(ctx->term_count.ptr, 0, 2 * sizeof(unsigned long long), ctx->stream),
+                    "clear term counters");
+        }
+        ctx->count_terms = value;
         break;
       default:
         fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));
@@ -1219,6 +1234,20 @@ int bk_update_refs(bk_ctx *ctx, const double *mass, const double *abw) {
     require_ctx(ctx);
     DeviceGuard g(ctx->device);
     update_refs(ctx, mass, abw);
+  });
+}
+
+int bk_term_count(bk_ctx *ctx, unsigned long long *terms_out, unsigned long long *query_terms_out) {
+  return guarded([&] {
+    require_ctx(ctx);
+    if (!ctx->count_terms) fail(BK_ERR_STATE, "term counting is off (BK_OPT_COUNT_TERMS)");
+    DeviceGuard g(ctx->device);
+    unsigned long long c[2] = {0, 0};
+    hip_check(hipMemcpyAsync(c, ctx->term_count.ptr, sizeof(c), hipMemcpyDeviceToHost, ctx->stream), "read counters");
+    hip_check(hipMemsetAsync(ctx->term_count.ptr, 0, sizeof(c), ctx->stream), "clear term counters");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (terms_out) *terms_out = c[0];
+    if (query_terms_out) *query_terms_out = c[1];
   });
 }
 

@@ -91,6 +91,10 @@ struct BoundArgs {
   // listed_order: most listed tiles first); nullptr = the 2-D grid's order
   const int *order = nullptr;
   int nblocks = 0;
+  // FP32 Gaussian range launches (not BOUNDED): when set, an instrumented
+  // variant adds the pair terms its wavefronts evaluate to count[0] (every
+  // lane) and count[1] (lanes holding a real query)
+  unsigned long long *count = nullptr;
 };
 // Reference split length of the range kernels (a multiple of `tile`).
 long long split_length(long long r_i, long long r_end, int splits, int tile);

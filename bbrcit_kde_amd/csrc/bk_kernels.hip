@@ -343,14 +343,18 @@ __device__ __forceinline__ bool gauss_tile_far(const float4 *__restrict__ tiles,
 }
 
 // BOUNDED: only the tiles listed in bmask (the block's tile mask) are summed;
-// the bounds of the others are added by the caller.
-template <bool ISO, bool BOUNDED>
+// the bounds of the others are added by the caller.  COUNT (an instrumented
+// variant, never the default launch): count[0] += the pair terms the
+// wavefronts evaluate (64 lanes per reference they loop over), count[1] +=
+// those of lanes that hold a real query.
+template <bool ISO, bool BOUNDED, bool COUNT = false>
 __device__ __forceinline__ void gauss_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
     const float4 *__restrict__ tiles, long long r_begin, long long r_end,
     long long q_begin, int q_count, double *__restrict__ dst, double out_scale,
-    int self_excl, float rho, double gx, double gy, int mode, const unsigned long long *__restrict__ bmask) {
+    int self_excl, float rho, double gx, double gy, int mode, const unsigned long long *__restrict__ bmask,
+    unsigned long long *__restrict__ count = nullptr) {
   constexpr int T = kGaussTile;
   static_assert(!BOUNDED || kGaussTile == kTile, "tile masks are per kTile references");
   __shared__ float4 sf[2][T];
@@ -372,6 +376,7 @@ __device__ __forceinline__ void gauss_item(
   }
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
   const long long q_self0 = q_begin + t;
+  unsigned long long evals = 0;  // COUNT: references this wavefront loops over
 
   float bx0, bx1, by0, by1;
   block_query_box<kGaussQpt>(px, py, sbox, bx0, bx1, by0, by1);
@@ -443,7 +448,10 @@ __device__ __forceinline__ void gauss_item(
           s[k] = 0.0f;
         }
         // a factor of zero for every query of the wavefront: the sub-tile adds nothing
-        if (__any(any_near)) gauss_tile<2, ISO>(sf[buf] + c0, cn, tile + c0, diag, q_self0, px, py, ax, ay, aq, rho, s);
+        if (__any(any_near)) {
+          gauss_tile<2, ISO>(sf[buf] + c0, cn, tile + c0, diag, q_self0, px, py, ax, ay, aq, rho, s);
+          if constexpr (COUNT) evals += (unsigned long long)cn;
+        }
 #pragma unroll
         for (int k = 0; k < kGaussQpt; ++k) acc[k] = __builtin_fma((double)s[k], sfac[k], acc[k]);
       }
@@ -472,11 +480,16 @@ __device__ __forceinline__ void gauss_item(
     }
     if (form == 2) {
       // a factor of zero for every query of the wavefront: the tile adds nothing
-      if (__any(any_near)) gauss_tile<2, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+      if (__any(any_near)) {
+        gauss_tile<2, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+        if constexpr (COUNT) evals += (unsigned long long)cnt;
+      }
     } else if (form == 1) {
       gauss_tile<1, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+      if constexpr (COUNT) evals += (unsigned long long)cnt;
     } else {
       gauss_tile<0, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+      if constexpr (COUNT) evals += (unsigned long long)cnt;
     }
 #pragma unroll
     for (int k = 0; k < kGaussQpt; ++k) acc[k] = __builtin_fma((double)s[k], fac[k], acc[k]);
@@ -487,6 +500,15 @@ __device__ __forceinline__ void gauss_item(
     const int qi = t + k * kBlock;
     if (qi < q_count) dst[qi] = acc[k] * out_scale;
   }
+  if constexpr (COUNT) {
+    int real = 0;
+#pragma unroll
+    for (int k = 0; k < kGaussQpt; ++k) real += __popcll(__ballot(t + k * kBlock < q_count));
+    if ((t & 63) == 0 && evals > 0) {
+      atomicAdd(count, evals * 64ULL * (unsigned long long)kGaussQpt);
+      atomicAdd(count + 1, evals * (unsigned long long)real);
+    }
+  }
 }
 
 // Minimum waves per SIMD of the range kernel: caps its registers at 64 so
@@ -496,7 +518,7 @@ __device__ __forceinline__ void gauss_item(
 #define BK_GAUSS_WAVES 8
 #endif
 
-template <bool ISO, bool BOUNDED>
+template <bool ISO, bool BOUNDED, bool COUNT = false>
 __global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const float4 *__restrict__ fd, const float4 *__restrict__ fg,
@@ -515,8 +537,8 @@ __global__ __launch_bounds__(kBlock, BK_GAUSS_WAVES) void gauss_range_kernel(
   const long long left = nq - qb;
   const int q_count = (int)(left < kGaussQueriesPerBlock ? left : kGaussQueriesPerBlock);
   const long long off = by * nq + qb;
-  gauss_item<ISO, BOUNDED>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl, rho,
-                           gx, gy, mode, BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
+  gauss_item<ISO, BOUNDED, COUNT>(qx, qy, fd, fg, tiles, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
+                                  rho, gx, gy, mode, BOUNDED ? ba.mask + bx * ba.nwords : nullptr, ba.count);
 }
 
 template <bool ISO>
@@ -1894,6 +1916,14 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
         BK_GR(true, true);
       else
         BK_GR(false, true);
+    } else if (ba && ba->count) {
+      if (iso)
+        gauss_range_kernel<true, false, true><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i, nq,
+                                                                      dst, out_scale, self_excl, rho, gx, gy, mode, b);
+      else
+        gauss_range_kernel<false, false, true><<<grid, kBlock, 0, s>>>(qx, qy, fd, fg, tiles, r_i, r_end, len, q_i,
+                                                                       nq, dst, out_scale, self_excl, rho, gx, gy,
+                                                                       mode, b);
     } else {
       if (iso)
         BK_GR(true, false);

@@ -86,6 +86,11 @@ extern "C" {
                                   /*   (D <= 2), so each workgroup                  */
                                   /*   holds nearby queries and skips the reference */
                                   /*   tiles beyond the kernel's reach              */
+#define BK_OPT_COUNT_TERMS     4  /* 1: FP32 Gaussian range evaluations (D <= 2)   */
+                                  /* run an instrumented variant of their kernel  */
+                                  /* that counts the pair terms it evaluates      */
+                                  /* (read with bk_term_count); slower, for       */
+                                  /* measurement only.  0 (default): off          */
 
 /* ---- POD types -------------------------------------------------------- */
 typedef struct bk_ctx bk_ctx;
@@ -159,6 +164,16 @@ int bk_update_refs(bk_ctx *ctx, const double *mass, const double *abw);
  * least significant coordinate first; needs no context.  Replaces the
  * std::sort that starts Kdtree construction (include/Kdtree.h:369-371). */
 int bk_lexico_order(int device, int dim, size_t n, const double *coords, long long *order_out);
+
+/* Pair terms evaluated since counting was switched on or last read
+ * (BK_OPT_COUNT_TERMS): terms_out = wavefront lanes x references they loop
+ * over (the vector work), query_terms_out = the same for lanes that hold a
+ * real query.  Tiles that a whole wavefront or workgroup passes over (all its
+ * terms below 2^-49, resp. 2^-100, of their mass) are not counted.  Resets
+ * both counters.  No counterpart in the reference, whose kernel evaluates
+ * every pair (include/CudaDirectKde.h:147-164). */
+int bk_term_count(bk_ctx *ctx, unsigned long long *terms_out,
+                  unsigned long long *query_terms_out);
 
 /* ---- direct sums ----------------------------------------------------- */
 

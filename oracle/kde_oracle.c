@@ -119,6 +119,7 @@ typedef struct {
   const double *qry;
   size_t q_begin, q_end;
   int exclude_self;
+  const long long *self_idx; /* non-NULL: query q is ref[self_idx[q]] and pair (self_idx[q], q) is left out */
   double *out;
 } direct_job;
 
@@ -132,9 +133,10 @@ static void *direct_worker(void *arg) {
   double dx[64];
   for (size_t q = j->q_begin; q < j->q_end; ++q) {
     double s = 0.0;
-    const double *xq = j->qry + q * (size_t)j->dim;
+    const size_t self = j->self_idx ? (size_t)j->self_idx[q] : q;
+    const double *xq = j->self_idx ? j->ref + self * (size_t)j->dim : j->qry + q * (size_t)j->dim;
     for (size_t r = 0; r < j->nr; ++r) {
-      if (j->exclude_self && r == q) continue;
+      if (j->exclude_self && r == self) continue;
       const double *xr = j->ref + r * (size_t)j->dim;
       for (int d = 0; d < j->dim; ++d) dx[d] = xq[d] - xr[d];
       const double a = j->abw ? j->abw[r] : 1.0;
@@ -148,9 +150,9 @@ static void *direct_worker(void *arg) {
 
 /* out[q] = sum_r mass_r U(x_q - x_r; h abw_r), q < nq (unnormalised).
  * Points are row-major (n x dim).  nthreads <= 0 uses one thread. */
-int bko_direct(int kind, int dim, double hx, double hy, const double *ref, const double *mass,
-               const double *abw, size_t nr, const double *qry, size_t nq, int exclude_self, double *out,
-               int nthreads) {
+static int direct_run(int kind, int dim, double hx, double hy, const double *ref, const double *mass,
+                      const double *abw, size_t nr, const double *qry, size_t nq, int exclude_self,
+                      const long long *self_idx, double *out, int nthreads) {
   if (dim < 1 || dim > 64) return -1;
   if (nthreads < 1) nthreads = 1;
   if ((size_t)nthreads > nq) nthreads = nq > 0 ? (int)nq : 1;
@@ -176,6 +178,7 @@ int bko_direct(int kind, int dim, double hx, double hy, const double *ref, const
     j->q_begin = (size_t)t * chunk < nq ? (size_t)t * chunk : nq;
     j->q_end = j->q_begin + chunk < nq ? j->q_begin + chunk : nq;
     j->exclude_self = exclude_self;
+    j->self_idx = self_idx;
     j->out = out;
   }
   int *joined = (int *)calloc((size_t)nthreads, sizeof(int));
@@ -193,6 +196,22 @@ int bko_direct(int kind, int dim, double hx, double hy, const double *ref, const
   free(jobs);
   free(th);
   return 0;
+}
+
+int bko_direct(int kind, int dim, double hx, double hy, const double *ref, const double *mass,
+               const double *abw, size_t nr, const double *qry, size_t nq, int exclude_self, double *out,
+               int nthreads) {
+  return direct_run(kind, dim, hx, hy, ref, mass, abw, nr, qry, nq, exclude_self, NULL, out, nthreads);
+}
+
+/* Leave-one-out sums at a subset of the points: out[k] = sum over r != idx[k]
+ * of mass_r U(ref[idx[k]] - ref[r]; h abw_r), the entries idx[k] of
+ * bko_direct(..., ref, nr, 1, ...) over the whole set (same loop, same order). */
+int bko_direct_loo_at(int kind, int dim, double hx, double hy, const double *ref, const double *mass,
+                      const double *abw, size_t nr, const long long *idx, size_t nq, double *out, int nthreads) {
+  for (size_t k = 0; k < nq; ++k)
+    if (idx[k] < 0 || (size_t)idx[k] >= nr) return -3;
+  return direct_run(kind, dim, hx, hy, ref, mass, abw, nr, NULL, nq, 1, idx, out, nthreads);
 }
 
 /* KernelDensity::likelihood_cross_validate, KernelDensityImpl.h:215-252:

@@ -75,6 +75,9 @@ class Oracle:
         lib.bko_kernel_u.argtypes = [_I, _I, _D, _D, _DP, _D]
         lib.bko_direct.restype = _I
         lib.bko_direct.argtypes = [_I, _I, _D, _D, _DP, _DP, _DP, _SZ, _DP, _SZ, _I, _DP, _I]
+        lib.bko_direct_loo_at.restype = _I
+        lib.bko_direct_loo_at.argtypes = [_I, _I, _D, _D, _DP, _DP, _DP, _SZ, ctypes.POINTER(ctypes.c_longlong),
+                                          _SZ, _DP, _I]
         for name in ("bko_likelihood_cv", "bko_lsq_conv_cv"):
             f = getattr(lib, name)
             f.restype = _D
@@ -104,6 +107,23 @@ class Oracle:
             raise RuntimeError(f"bko_direct failed ({rc})")
         if normalize:
             out *= self.normalization(kind, dim, hx, hy)
+        return out
+
+    def direct_loo_at(self, kind, hx, hy, pts, idx, mass=None, abw=None, normalize=False,
+                      threads: Optional[int] = None) -> np.ndarray:
+        """Leave-one-out sums of the points pts[idx] over all of pts (pair (i, i) left out)."""
+        p = _pts(pts)
+        ii = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+        hy = hx if hy is None else hy
+        out = np.empty(len(ii), dtype=np.float64)
+        rc = self.lib.bko_direct_loo_at(kind, p.shape[1], hx, hy, _p(p), _p(_vec(mass, len(p))),
+                                        _p(_vec(abw, len(p))), len(p),
+                                        ii.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(ii), _p(out),
+                                        threads or default_threads())
+        if rc != 0:
+            raise RuntimeError(f"bko_direct_loo_at failed ({rc})")
+        if normalize:
+            out *= self.normalization(kind, p.shape[1], hx, hy)
         return out
 
     def likelihood_cv(self, kind, hx, hy, pts, weight, mass=None, abw=None, threads=None) -> float:
@@ -140,6 +160,9 @@ class Reference:
         lib.ref_kernel.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _D, _DP, _DP]
         lib.ref_direct_eval.restype = _I
         lib.ref_direct_eval.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, _DP, _SZ, _DP]
+        lib.ref_direct_eval_timed.restype = _I
+        lib.ref_direct_eval_timed.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, _DP, _SZ, _I, _DP,
+                                              ctypes.POINTER(_D)]
         lib.ref_tree_points.restype = ctypes.c_long
         lib.ref_tree_points.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, _DP, _DP, _DP, _DP]
         lib.ref_tree_eval.restype = ctypes.c_long
@@ -172,6 +195,19 @@ class Reference:
         if rc != 0:
             raise RuntimeError(f"ref_direct_eval failed ({rc})")
         return out
+
+    def direct_eval_timed(self, kind, kernel_float, hx, hy, pts, w, queries, threads=1, leaf_nmax=2):
+        """direct_eval with the KernelDensity built outside the timed region and
+        the queries split over `threads` threads: (values, seconds)."""
+        p, q = _pts(pts), _pts(queries)
+        out = np.empty(len(q))
+        sec = _D()
+        rc = self.lib.ref_direct_eval_timed(kind, p.shape[1], kernel_float, hx, hx if hy is None else hy, _p(p),
+                                            _p(_vec(w, len(p))), len(p), leaf_nmax, _p(q), len(q), int(threads),
+                                            _p(out), ctypes.byref(sec))
+        if rc != 0:
+            raise RuntimeError(f"ref_direct_eval_timed failed ({rc})")
+        return out, sec.value
 
     def tree_points(self, kind, kernel_float, hx, hy, pts, w, leaf_nmax=2):
         p = _pts(pts)

@@ -28,7 +28,9 @@
 #include <Kernels/GaussianProductConvKernel2d.h>
 #include <Kernels/GaussianProductKernel2d.h>
 
+#include <chrono>
 #include <cstddef>
+#include <thread>
 #include <exception>
 #include <iostream>
 #include <sstream>
@@ -61,17 +63,24 @@ void set_bw(EpanechnikovConvKernel1d<T> &k, double hx, double) { k.set_hx(hx); }
 template <class T>
 void set_bw(GaussianConvKernel1d<T> &k, double hx, double) { k.set_hx(hx); }
 
-// Every (kind, dim) the reference supports.
+// Every (kind, dim) the reference supports for D <= 2, and the generic
+// Gaussian and Epanechnikov kernels at D = 3, 8 and 16.
 template <class T, class Fn>
 int dispatch_t(int kind, int dim, Fn fn) {
   switch (kind) {
     case 1:
       if (dim == 1) return fn(Tag<1, EpanechnikovKernel<1, T>>());
       if (dim == 2) return fn(Tag<2, EpanechnikovKernel<2, T>>());
+      if (dim == 3) return fn(Tag<3, EpanechnikovKernel<3, T>>());
+      if (dim == 8) return fn(Tag<8, EpanechnikovKernel<8, T>>());
+      if (dim == 16) return fn(Tag<16, EpanechnikovKernel<16, T>>());
       break;
     case 2:
       if (dim == 1) return fn(Tag<1, GaussianKernel<1, T>>());
       if (dim == 2) return fn(Tag<2, GaussianKernel<2, T>>());
+      if (dim == 3) return fn(Tag<3, GaussianKernel<3, T>>());
+      if (dim == 8) return fn(Tag<8, GaussianKernel<8, T>>());
+      if (dim == 16) return fn(Tag<16, GaussianKernel<16, T>>());
       break;
     case 3:
       if (dim == 2) return fn(Tag<2, EpanechnikovProductKernel2d<T>>());
@@ -192,6 +201,40 @@ int ref_direct_eval(int kind, int dim, int kernel_float, double hx, double hy, c
       set_bw(kde.kernel(), hx, hy);
       auto q = make_points<KDE>(qry, nullptr, nq);
       kde.direct_eval(q);
+      for (size_t i = 0; i < nq; ++i) out[i] = q[i].attributes().value();
+      return 0;
+    });
+  });
+}
+
+// Timing of KernelDensity::direct_eval alone: the KernelDensity (its kd tree)
+// and the query points are built first, outside the timed region; then the
+// queries are split into `threads` contiguous chunks evaluated concurrently
+// with the per-point direct_eval, as test/test_kde_cppthread.cc:88-106 splits
+// its queries over std::thread workers.  *seconds_out = wall time of the
+// evaluation only; out receives the values in query order.
+int ref_direct_eval_timed(int kind, int dim, int kernel_float, double hx, double hy, const double *pts,
+                          const double *w, size_t n, int leaf_nmax, const double *qry, size_t nq, int threads,
+                          double *out, double *seconds_out) {
+  return guarded([&] {
+    return dispatch(kind, dim, kernel_float, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      auto q = make_points<KDE>(qry, nullptr, nq);
+      const int t = threads < 1 ? 1 : threads;
+      auto segment = [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) kde.direct_eval(q[i]);
+      };
+      const auto t0 = std::chrono::steady_clock::now();
+      std::vector<std::thread> workers;
+      for (int c = 1; c < t; ++c) workers.emplace_back(segment, nq * c / t, nq * (c + 1) / t);
+      segment(0, nq / t);
+      for (auto &th : workers) th.join();
+      const auto t1 = std::chrono::steady_clock::now();
+      *seconds_out = std::chrono::duration<double>(t1 - t0).count();
       for (size_t i = 0; i < nq; ++i) out[i] = q[i].attributes().value();
       return 0;
     });

@@ -5,7 +5,8 @@ The values come from the reference's own CPU implementation
 oracle/_ref/libbbrcit_ref.so).  Inputs are seeded synthetic points; each
 fixture stores inputs and expected outputs only.
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py             # every fixture
+    make -C oracle && python tests/golden/make_golden.py high_dim    # direct_hd_ref.npz only
 """
 from __future__ import annotations
 
@@ -45,8 +46,30 @@ def grid_search_case():
     return {"bw_x0": 0.2, "bw_x1": 0.6, "bw_y0": 0.25, "bw_y1": 0.7}
 
 
+# (kind, dim, h) of the generic GaussianKernel<D> / EpanechnikovKernel<D> at D > 2
+HIGH_DIM_CASES = [(2, 3, 0.7), (1, 3, 1.5), (2, 8, 0.7), (1, 8, 3.0), (2, 16, 0.7), (1, 16, 5.0)]
+
+
+def write_high_dim(ref) -> None:
+    """direct_eval of the generic kernels at D = 3, 8, 16 (FP64 kernels):
+    direct_hd_ref.npz (300 standard-normal references, 50 queries)."""
+    rng = np.random.default_rng(3)
+    out = {"cases": np.array(HIGH_DIM_CASES, dtype=np.float64)}
+    for i, (kind, dim, h) in enumerate(HIGH_DIM_CASES):
+        p = rng.normal(size=(300, dim))
+        w = rng.uniform(0.5, 1.5, 300)
+        q = rng.normal(size=(50, dim)) * 1.2
+        out[f"pts_{i}"], out[f"w_{i}"], out[f"q_{i}"] = p, w, q
+        out[f"direct_{i}"] = ref.direct_eval(kind, 0, h, h, p, w, q)
+    np.savez_compressed(OUT / "direct_hd_ref.npz", **out)
+
+
 def main() -> int:
     ref = Reference()
+    if sys.argv[1:] == ["high_dim"]:
+        write_high_dim(ref)
+        return 0
+    write_high_dim(ref)
     pts, w, q = inputs()
     out = {"pts": pts, "w": w, "q": q, "cases": np.array(DIRECT_CASES, dtype=np.float64)}
     for i, (kind, dim, hx, hy) in enumerate(DIRECT_CASES):

@@ -312,3 +312,33 @@ def test_epanechnikov_general_d_normalisation(oracle):
         want = 0.5 * (D + 2) / (pi ** (D / 2) / gamma(1 + D / 2))
         assert oracle.normalization(EPAN, D, 1.0) == pytest.approx(want, rel=1e-14)
         assert oracle.normalization(GAUSS, D, 0.5) == pytest.approx((2 * pi) ** (-D / 2) / 0.5 ** D, rel=1e-14)
+
+
+def test_loo_at_subset_equals_full_leave_one_out(oracle):
+    # bko_direct_loo_at(idx) must return exactly the idx entries of the
+    # whole-set leave-one-out sums (same loop and order), for every kind used
+    # by the full-size GPU checks, with adaptive bandwidths
+    rng = np.random.default_rng(31)
+    pts = rng.normal(size=(700, 2))
+    mass = rng.uniform(0.5, 1.5, 700)
+    abw = rng.uniform(0.5, 2.0, 700)
+    idx = np.array([0, 5, 699, 350, 5, 123])
+    for kind, (hx, hy) in ((2, (0.2, 0.2)), (4, (0.15, 0.25)), (3, (0.3, 0.4))):
+        full = oracle.direct(kind, hx, hy, pts, pts, mass=mass, abw=abw, exclude_self=True)
+        sub = oracle.direct_loo_at(kind, hx, hy, pts, idx, mass=mass, abw=abw)
+        np.testing.assert_array_equal(sub, full[idx])
+    with pytest.raises(RuntimeError):
+        oracle.direct_loo_at(2, 0.2, 0.2, pts, [700])
+
+
+def test_direct_high_dim_matches_reference_fixture(oracle):
+    # generic GaussianKernel<D> / EpanechnikovKernel<D>, D = 3, 8, 16, written by
+    # the reference's own KernelDensity::direct_eval (weights normalised to sum 1)
+    f = np.load(GOLDEN / "direct_hd_ref.npz")
+    for i, (kind, dim, h) in enumerate(f["cases"]):
+        p, w, q = f[f"pts_{i}"], f[f"w_{i}"], f[f"q_{i}"]
+        assert p.shape[1] == int(dim)
+        want = f[f"direct_{i}"]
+        got = oracle.direct(int(kind), h, h, p, q, mass=w / w.sum(), normalize=True)
+        assert (want > 0).mean() > 0.9
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
