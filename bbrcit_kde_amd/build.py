@@ -20,7 +20,7 @@ CSRC = PKG_DIR / "csrc"
 LIB_DIR = PKG_DIR / "lib"
 LIB_PATH = LIB_DIR / "libbbrcit_hip.so"
 
-HIP_SOURCES = ["bk_kernels.hip", "bk_capi.hip", "bk_cvgrid.hip", "bk_mlp.hip", "bk_order.hip"]
+HIP_SOURCES = ["bk_kernels.hip", "bk_capi.hip", "bk_cvgrid.hip", "bk_mlp.hip", "bk_order.hip", "bk_sample.hip"]
 
 # -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32.
 HIPCC_FLAGS = [
@@ -38,7 +38,9 @@ HIPCC_FLAGS = [
 # Per-source additions.  bk_kernels.hip: MFMA results go straight to VGPRs
 # (the Gram kernels read every result with the VALU; the AGPR form would add
 # two v_accvgpr_read per pair term).
-SOURCE_FLAGS = {"bk_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# bk_sample.hip: no contraction of a * b + c into FMAs anywhere in the file
+# (its samples must round each product and sum as the reference's host code).
+SOURCE_FLAGS = {"bk_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"], "bk_sample.hip": ["-ffp-contract=off"]}
 
 
 def hipcc() -> str:

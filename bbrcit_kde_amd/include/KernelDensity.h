@@ -236,6 +236,44 @@ class KernelDensity {
     for (int i = 0; i < D; ++i) p[i] = ref[i] + (FloatType)k[i];
   }
 
+  // n samples at once into out (row-major, n x D): the same values as n
+  // calls of simulate(e, p).  The draws are taken from e on the host in the
+  // order those calls take them (the pick's uniform, then the kernel's unit
+  // draws, sample after sample); the picks (a binary search over the
+  // cumulative weights per sample) and the shifts run on the GPU
+  // (bk_sampler_draw).  Not in the reference, which samples one point per call.
+  template <typename RNG>
+  void simulate(RNG &e, size_t n, std::vector<FloatType> &out) const {
+    require_data();
+    static_assert(D <= 2, "batched sampling supports D = 1 and 2");
+    out.resize(n * D);
+    if (n == 0) return;
+    std::uniform_real_distribution<FloatType> pick(0, 1);
+    std::vector<double> u(n), z(n * D);
+    KernelFloatType unit[D];
+    for (size_t k = 0; k < n; ++k) {
+      u[k] = (double)pick(e);
+      kernel_.draw_unit(e, unit);
+      for (int d = 0; d < D; ++d) z[k * D + d] = (double)unit[d];
+    }
+    const auto &pts = points();
+    std::vector<double> xy(pts.size() * D), cum(cum_weights_.begin(), cum_weights_.end()), abw(pts.size());
+    for (size_t i = 0; i < pts.size(); ++i) {
+      for (int d = 0; d < D; ++d) xy[i * D + d] = (double)pts[i][d];
+      abw[i] = (double)pts[i].attributes().abw();
+    }
+    const bk_kernel_desc kd = kernel_.desc();
+    const double h[2] = {kd.hx, kd.hy};
+    bk_sampler *s = nullptr;
+    hip_detail::check(bk_sampler_create(0, D, pts.size(), xy.data(), cum.data(), abw.data(), &s));
+    std::vector<double> res(n * D);
+    const int st = bk_sampler_draw(s, std::is_same<KernelFloatType, float>::value ? 1 : 0, h, n, u.data(), z.data(),
+                                   res.data(), nullptr);
+    bk_sampler_destroy(s);
+    hip_detail::check(st);
+    for (size_t k = 0; k < n * D; ++k) out[k] = (FloatType)res[k];
+  }
+
   template <typename RNG>
   DataPointType simulate(RNG &e) const {
     std::vector<FloatType> p;

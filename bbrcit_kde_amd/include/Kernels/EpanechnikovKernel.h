@@ -44,6 +44,17 @@ class EpanechnikovKernel {
   template <typename RNG>
   void simulate(RNG &e, std::vector<T> &p, T a = ConstantTraits<T>::one()) const;
 
+  // The unit draw of simulate (D = 1): the Epanechnikov choice of three
+  // uniform(-1, 1) draws taken in the order v3, v2, v1, the order in which
+  // g++ evaluates the reference's epanechnikov_choice(d(e), d(e), d(e)).
+  template <typename RNG>
+  void draw_unit(RNG &e, T *z) const {
+    static_assert(D == 1, "EpanechnikovKernel sampling is defined for D = 1");
+    std::uniform_real_distribution<T> u(T(-1), T(1));
+    const T v3 = u(e), v2 = u(e), v1 = u(e);
+    z[0] = epanechnikov_choice(v1, v2, v3);
+  }
+
   T bandwidth() const { return h_; }
   void set_bandwidth(T h) { h_ = h; }
 
@@ -60,10 +71,10 @@ template <int D, typename T>
 template <typename RNG>
 void EpanechnikovKernel<D, T>::simulate(RNG &e, std::vector<T> &p, T a) const {
   static_assert(D == 1, "EpanechnikovKernel::simulate is defined for D = 1");
-  std::uniform_real_distribution<T> u(T(-1), T(1));
+  T z[1];
+  draw_unit(e, z);
   p.resize(1);
-  const T v1 = u(e), v2 = u(e), v3 = u(e);
-  p[0] = a * h_ * epanechnikov_choice(v1, v2, v3);
+  p[0] = a * h_ * z[0];
 }
 
 }  // namespace bbrcit

@@ -40,6 +40,19 @@ class EpanechnikovProductKernel2d {
   template <typename RNG>
   void simulate(RNG &e, std::vector<T> &p, T a = ConstantTraits<T>::one()) const;
 
+  // The unit draws of simulate: per axis (x, then y) the Epanechnikov choice
+  // of three uniform(-1, 1) draws, taken in the order v3, v2, v1 -- the
+  // order in which g++ evaluates the reference's
+  // epanechnikov_choice(d(e), d(e), d(e)) (arguments right to left).
+  template <typename RNG>
+  void draw_unit(RNG &e, T *z) const {
+    std::uniform_real_distribution<T> u(T(-1), T(1));
+    for (int i = 0; i < 2; ++i) {
+      const T v3 = u(e), v2 = u(e), v1 = u(e);
+      z[i] = epanechnikov_choice(v1, v2, v3);
+    }
+  }
+
   T hx() const { return hx_; }
   T hy() const { return hy_; }
   void set_hx(T h) { hx_ = h; }
@@ -59,12 +72,11 @@ class EpanechnikovProductKernel2d {
 template <typename T>
 template <typename RNG>
 void EpanechnikovProductKernel2d<T>::simulate(RNG &e, std::vector<T> &p, T a) const {
-  std::uniform_real_distribution<T> u(T(-1), T(1));
+  T z[2];
+  draw_unit(e, z);
   p.resize(2);
-  for (int i = 0; i < 2; ++i) {
-    const T v1 = u(e), v2 = u(e), v3 = u(e);
-    p[i] = a * (i == 0 ? hx_ : hy_) * epanechnikov_choice(v1, v2, v3);
-  }
+  p[0] = a * hx_ * z[0];
+  p[1] = a * hy_ * z[1];
 }
 
 }  // namespace bbrcit

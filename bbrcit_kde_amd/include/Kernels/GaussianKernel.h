@@ -44,6 +44,16 @@ class GaussianKernel {
   template <typename RNG>
   void simulate(RNG &e, std::vector<T> &p, T a = ConstantTraits<T>::one()) const;
 
+  // The unit draws of simulate (D standard normals; simulate scales them by
+  // a * h).  One distribution object serves every call, as the reference's
+  // function-local static does (GaussianKernel.h:146-151): for odd D a call
+  // uses the second normal of the pair the previous call drew.
+  template <typename RNG>
+  void draw_unit(RNG &e, T *z) const {
+    static std::normal_distribution<T> n(T(0), T(1));
+    for (int i = 0; i < D; ++i) z[i] = n(e);
+  }
+
   T bandwidth() const { return h_; }
   void set_bandwidth(T h) { h_ = h; }
 
@@ -58,9 +68,10 @@ class GaussianKernel {
 template <int D, typename T>
 template <typename RNG>
 void GaussianKernel<D, T>::simulate(RNG &e, std::vector<T> &p, T a) const {
-  std::normal_distribution<T> n(T(0), T(1));
+  T z[D];
+  draw_unit(e, z);
   p.resize(D);
-  for (int i = 0; i < D; ++i) p[i] = a * h_ * n(e);
+  for (int i = 0; i < D; ++i) p[i] = a * h_ * z[i];
 }
 
 }  // namespace bbrcit

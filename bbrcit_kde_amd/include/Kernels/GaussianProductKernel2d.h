@@ -40,6 +40,15 @@ class GaussianProductKernel2d {
   template <typename RNG>
   void simulate(RNG &e, std::vector<T> &p, T a = ConstantTraits<T>::one()) const;
 
+  // The unit draws of simulate (two standard normals, x then y, from one
+  // distribution object shared by every call as the reference's static).
+  template <typename RNG>
+  void draw_unit(RNG &e, T *z) const {
+    static std::normal_distribution<T> n(T(0), T(1));
+    z[0] = n(e);
+    z[1] = n(e);
+  }
+
   T hx() const { return hx_; }
   T hy() const { return hy_; }
   void set_hx(T h) { hx_ = h; }
@@ -59,10 +68,11 @@ class GaussianProductKernel2d {
 template <typename T>
 template <typename RNG>
 void GaussianProductKernel2d<T>::simulate(RNG &e, std::vector<T> &p, T a) const {
-  std::normal_distribution<T> n(T(0), T(1));
+  T z[2];
+  draw_unit(e, z);
   p.resize(2);
-  p[0] = a * hx_ * n(e);
-  p[1] = a * hy_ * n(e);
+  p[0] = a * hx_ * z[0];
+  p[1] = a * hy_ * z[1];
 }
 
 }  // namespace bbrcit

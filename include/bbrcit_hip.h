@@ -376,6 +376,31 @@ int bk_cv_grid_get_option(const bk_cv_grid *plan, int option, int *value);
 int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
                            double *total_ms, long long *rescued);
 
+/* ---- sampling ------------------------------------------------------------ */
+/* The device part of KernelDensity::simulate (include/KernelDensityImpl.h:
+ * 99-138), for batches of samples.  A sampler holds, on HIP device `device`,
+ * the n points of a density (row-major coords, dim 1 or 2, FP64), their
+ * cumulative weights (non-decreasing; the reference's cum_weights_, last
+ * entry 1) and their bandwidth factors abw.  Sample k, with the caller's
+ * uniform draw u[k] in [0, 1) and the kernel's unit draws z[k*dim + d] (the
+ * standard-normal or Epanechnikov draws the reference's kernel::simulate
+ * multiplies by a * h), is
+ *     i = the first index with cum_weights[i] > u[k] (n - 1 if none),
+ *     out[k*dim + d] = coords[i*dim + d] + (a_i * h[d]) * z[k*dim + d],
+ * the product rounded in the kernel's float type (kernel_float = 1: FP32,
+ * as the reference's float kernels compute it; 0: FP64), the sum in FP64:
+ * bit-identical to the reference's simulate for the same draws.  index_out
+ * (n_samples values) may be NULL.  No counterpart exists for the batch in the
+ * reference, which samples one point per call on the host. */
+typedef struct bk_sampler bk_sampler;
+int bk_sampler_create(int device, int dim, size_t n, const double *coords,
+                      const double *cum_weights, const double *abw,
+                      bk_sampler **out);
+int bk_sampler_destroy(bk_sampler *sampler);
+int bk_sampler_draw(bk_sampler *sampler, int kernel_float, const double *h,
+                    size_t n_samples, const double *u, const double *z,
+                    double *out, long long *index_out);
+
 /* ---- learned density surrogate ------------------------------------------ */
 /* No counterpart in the reference.  An MLP 2 -> 128 -> 128 -> 128 -> 1 with
  * SiLU activations maps a 2-D query q to y, and the surrogate density is

@@ -163,6 +163,8 @@ class Reference:
         lib.ref_direct_eval_timed.restype = _I
         lib.ref_direct_eval_timed.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, _DP, _SZ, _I, _DP,
                                               ctypes.POINTER(_D)]
+        lib.ref_simulate.restype = _I
+        lib.ref_simulate.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, ctypes.c_ulonglong, _SZ, _DP]
         lib.ref_tree_points.restype = ctypes.c_long
         lib.ref_tree_points.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, _DP, _DP, _DP, _DP]
         lib.ref_tree_eval.restype = ctypes.c_long
@@ -208,6 +210,16 @@ class Reference:
         if rc != 0:
             raise RuntimeError(f"ref_direct_eval_timed failed ({rc})")
         return out, sec.value
+
+    def simulate(self, kind, kernel_float, hx, hy, pts, w, seed, n_samples, leaf_nmax=2):
+        """n_samples calls of KernelDensity::simulate(e, p), e = std::mt19937_64(seed)."""
+        p = _pts(pts)
+        out = np.empty((n_samples, p.shape[1]))
+        rc = self.lib.ref_simulate(kind, p.shape[1], kernel_float, hx, hx if hy is None else hy, _p(p),
+                                   _p(_vec(w, len(p))), len(p), leaf_nmax, int(seed), int(n_samples), _p(out))
+        if rc != 0:
+            raise RuntimeError(f"ref_simulate failed ({rc})")
+        return out
 
     def tree_points(self, kind, kernel_float, hx, hy, pts, w, leaf_nmax=2):
         p = _pts(pts)

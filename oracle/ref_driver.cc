@@ -165,6 +165,23 @@ int guarded(F f) {
   }
 }
 
+// The kinds whose kernels can be sampled (simulate): EpanechnikovKernel<1>,
+// GaussianKernel<1|2>, EpanechnikovProductKernel2d, GaussianProductKernel2d.
+template <class T, class Fn>
+int dispatch_sim_t(int kind, int dim, Fn fn) {
+  if (kind == 1 && dim == 1) return fn(Tag<1, EpanechnikovKernel<1, T>>());
+  if (kind == 2 && dim == 1) return fn(Tag<1, GaussianKernel<1, T>>());
+  if (kind == 2 && dim == 2) return fn(Tag<2, GaussianKernel<2, T>>());
+  if (kind == 3 && dim == 2) return fn(Tag<2, EpanechnikovProductKernel2d<T>>());
+  if (kind == 4 && dim == 2) return fn(Tag<2, GaussianProductKernel2d<T>>());
+  return -1;
+}
+
+template <class Fn>
+int dispatch_sim(int kind, int dim, int kernel_float, Fn fn) {
+  return kernel_float ? dispatch_sim_t<float>(kind, dim, fn) : dispatch_sim_t<double>(kind, dim, fn);
+}
+
 }  // namespace
 
 extern "C" {
@@ -236,6 +253,28 @@ int ref_direct_eval_timed(int kind, int dim, int kernel_float, double hx, double
       const auto t1 = std::chrono::steady_clock::now();
       *seconds_out = std::chrono::duration<double>(t1 - t0).count();
       for (size_t i = 0; i < nq; ++i) out[i] = q[i].attributes().value();
+      return 0;
+    });
+  });
+}
+
+// n_samples calls of the reference's KernelDensity::simulate(e, p) with
+// std::mt19937_64(seed): out row-major (n_samples x dim).
+int ref_simulate(int kind, int dim, int kernel_float, double hx, double hy, const double *pts, const double *w,
+                 size_t n, int leaf_nmax, unsigned long long seed, size_t n_samples, double *out) {
+  return guarded([&] {
+    return dispatch_sim(kind, dim, kernel_float, [&](auto tag) -> int {
+      using K = typename decltype(tag)::Kernel;
+      constexpr int D = decltype(tag)::dim;
+      using KDE = KernelDensity<D, K, double>;
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      set_bw(kde.kernel(), hx, hy);
+      std::mt19937_64 e(seed);
+      std::vector<double> p;
+      for (size_t k = 0; k < n_samples; ++k) {
+        kde.simulate(e, p);
+        for (int d = 0; d < D; ++d) out[k * D + d] = p[d];
+      }
       return 0;
     });
   });

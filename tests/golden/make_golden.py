@@ -7,6 +7,7 @@ fixture stores inputs and expected outputs only.
 
     make -C oracle && python tests/golden/make_golden.py             # every fixture
     make -C oracle && python tests/golden/make_golden.py high_dim    # direct_hd_ref.npz only
+    make -C oracle && python tests/golden/make_golden.py simulate    # simulate_ref.npz only
 """
 from __future__ import annotations
 
@@ -64,12 +65,32 @@ def write_high_dim(ref) -> None:
     np.savez_compressed(OUT / "direct_hd_ref.npz", **out)
 
 
+# (kind, dim, kernel_float, hx, hy) of the samplable kernels
+SIMULATE_CASES = [(1, 1, 0, 0.3, 0.3), (1, 1, 1, 0.3, 0.3), (2, 1, 0, 0.2, 0.2), (2, 2, 0, 0.25, 0.25),
+                  (2, 2, 1, 0.25, 0.25), (3, 2, 0, 0.3, 0.45), (3, 2, 1, 0.3, 0.45), (4, 2, 0, 0.2, 0.35),
+                  (4, 2, 1, 0.2, 0.35)]
+
+
+def write_simulate(ref) -> None:
+    """KernelDensity::simulate, 2000 samples per case from std::mt19937_64(seed):
+    simulate_ref.npz (the same 600 weighted points as direct_ref.npz)."""
+    pts, w, _ = inputs()
+    out = {"pts": pts, "w": w, "cases": np.array(SIMULATE_CASES, dtype=np.float64)}
+    for i, (kind, dim, kf, hx, hy) in enumerate(SIMULATE_CASES):
+        out[f"sim_{i}"] = ref.simulate(kind, kf, hx, hy, pts[:, :dim], w, seed=100 + i, n_samples=2000)
+    np.savez_compressed(OUT / "simulate_ref.npz", **out)
+
+
 def main() -> int:
     ref = Reference()
     if sys.argv[1:] == ["high_dim"]:
         write_high_dim(ref)
         return 0
+    if sys.argv[1:] == ["simulate"]:
+        write_simulate(ref)
+        return 0
     write_high_dim(ref)
+    write_simulate(ref)
     pts, w, q = inputs()
     out = {"pts": pts, "w": w, "q": q, "cases": np.array(DIRECT_CASES, dtype=np.float64)}
     for i, (kind, dim, hx, hy) in enumerate(DIRECT_CASES):

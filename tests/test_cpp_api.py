@@ -53,7 +53,15 @@ def api():
     lib.api_cv_grid.argtypes = [_I, _DP, _DP, _SZ, _I, _DP, _DP, _SZ, _I, _D, _DP, _DP, _DP, _DP]
     lib.api_grid_layouts.argtypes = [_DP, _SZ, _DP, _SZ, _D, _D, _I, _D, _D, _I, _DP, _DP]
     lib.api_marginal_eval.argtypes = [_I, _I, _D, _D, _DP, _DP, _SZ, _I, _D, _I, _DP, _SZ, _DP, _DP, _SZ, _DP]
+    lib.api_simulate.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, ctypes.c_ulonglong, _SZ, _I, _DP]
     return lib
+
+
+def api_simulate(api, kind, dim, kf, hx, hy, pts, w, seed, n, batched, leaf_nmax=2):
+    out = np.empty((n, dim))
+    rc = api.api_simulate(kind, dim, kf, hx, hy, _p(pts), _p(w), len(pts), leaf_nmax, seed, n, int(batched), _p(out))
+    assert rc == 0, _err(api)
+    return out
 
 
 def _err(api):
@@ -61,6 +69,19 @@ def _err(api):
 
 
 # ---- host only ------------------------------------------------------------------------
+def test_simulate_matches_reference_fixture_on_the_host(api):
+    # n single-point KernelDensity::simulate calls with std::mt19937_64(seed),
+    # against the same calls of the reference (tests/golden/simulate_ref.npz):
+    # bit-identical (same draws in the same order, same arithmetic)
+    f = np.load(GOLDEN / "simulate_ref.npz")
+    pts, w = f["pts"], f["w"]
+    for i, (kind, dim, kf, hx, hy) in enumerate(f["cases"]):
+        kind, dim, kf = int(kind), int(dim), int(kf)
+        got = api_simulate(api, kind, dim, kf, hx, hy, pts[:, :dim], w, 100 + i, 2000, batched=False)
+        np.testing.assert_array_equal(got, f[f"sim_{i}"], err_msg=f"case {i}")
+
+
+
 def test_kdtree_order_matches_reference(api, reference):
     rng = np.random.default_rng(3)
     pts = rng.normal(size=(3000, 2))
