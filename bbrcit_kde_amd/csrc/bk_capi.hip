@@ -850,8 +850,11 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
                                     ctx->tsum.as<double4>(), ntiles, rel_err, abs_u, m_total, nwords, mask, lo_b,
                                     gap_b, counts, s),
               "bound pass launch");
-    // the summed tiles' exact part
-    const int splits = choose_splits(cn, nr, qpb, tile_of(path), slots);
+    // the summed tiles' exact part; the split partial sums are held within
+    // 256 MiB (the pruned evaluation sums few tiles per block, so a few
+    // splits keep the device busy and the buffer small)
+    const long long split_cap = std::max<long long>(1, (256LL << 20) / (cn * (long long)sizeof(double)));
+    const int splits = (int)std::min<long long>(choose_splits(cn, nr, qpb, tile_of(path), slots), split_cap);
     double *dst = exact + c0;
     if (splits > 1) {
       ctx->partial.reserve((size_t)splits * cn * sizeof(double));

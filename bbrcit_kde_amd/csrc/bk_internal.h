@@ -256,6 +256,12 @@ void listed_order(const unsigned long long *mask, long long nwords, int nblocks,
 // pairs with the most work first.
 void order_by_keys(unsigned int *keys, int *vals, long long n, int *order_out, DevBuf &keys_b, DevBuf &tmp,
                    hipStream_t s);
+// Stable lexicographic order (ExactLexicoLess, -0 equal to +0, ties in index
+// order) of n device points (row-major, dim per point): a radix sort per
+// coordinate, least significant first.  The order is left in va; ka, kb, vb
+// and tmp are workspaces the function grows.
+void lexico_sort(const double *coords, int dim, long long n, DevBuf &ka, DevBuf &kb, DevBuf &va, DevBuf &vb,
+                 DevBuf &tmp, hipStream_t s);
 // gx[k] = qx[list[k]], gy[k] = qy[list[k]] for k < n.
 hipError_t launch_gather_xy(const float *qx, const float *qy, const long long *list, long long n, float *gx,
                             float *gy, hipStream_t s);

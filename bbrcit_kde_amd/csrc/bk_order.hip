@@ -229,6 +229,34 @@ void spatial_sort(const double *qx, const double *qy, const long long *list, lon
 
 // Lexicographic order of n points by a stable device radix sort, least
 // significant coordinate first (see include/bbrcit_hip.h).
+namespace bk {
+void lexico_sort(const double *coords, int dim, long long n, DevBuf &ka, DevBuf &kb, DevBuf &va, DevBuf &vb,
+                 DevBuf &tmp, hipStream_t s) {
+  ka.reserve((size_t)n * sizeof(unsigned long long));
+  kb.reserve((size_t)n * sizeof(unsigned long long));
+  va.reserve((size_t)n * sizeof(long long));
+  vb.reserve((size_t)n * sizeof(long long));
+  size_t tmp_bytes = 0;
+  hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, ka.as<unsigned long long>(),
+                                               kb.as<unsigned long long>(), va.as<long long>(), vb.as<long long>(),
+                                               (int)n, 0, 64, s),
+            "radix sort size query");
+  tmp.reserve(tmp_bytes + 256);
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  iota_kernel<<<grid, 256, 0, s>>>(va.as<long long>(), n);
+  hip_check(hipGetLastError(), "iota launch");
+  for (int d = dim - 1; d >= 0; --d) {
+    lexico_keys_kernel<<<grid, 256, 0, s>>>(coords, dim, d, va.as<long long>(), n, ka.as<unsigned long long>());
+    hip_check(hipGetLastError(), "key launch");
+    hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, ka.as<unsigned long long>(),
+                                                 kb.as<unsigned long long>(), va.as<long long>(),
+                                                 vb.as<long long>(), (int)n, 0, 64, s),
+              "radix sort");
+    std::swap(va, vb);
+  }
+}
+}  // namespace bk
+
 extern "C" int bk_lexico_order(int device, int dim, size_t n, const double *coords, long long *order_out) {
   using namespace bk;
   return guarded([&] {
@@ -249,33 +277,10 @@ extern "C" int bk_lexico_order(int device, int dim, size_t n, const double *coor
       }
     } w;
     hip_check(hipStreamCreate(&w.s), "hipStreamCreate");
-    const long long nn = (long long)n;
     w.c.reserve(n * (size_t)dim * sizeof(double));
-    w.ka.reserve(n * sizeof(unsigned long long));
-    w.kb.reserve(n * sizeof(unsigned long long));
-    w.va.reserve(n * sizeof(long long));
-    w.vb.reserve(n * sizeof(long long));
-    size_t tmp_bytes = 0;
-    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, w.ka.as<unsigned long long>(),
-                                                 w.kb.as<unsigned long long>(), w.va.as<long long>(),
-                                                 w.vb.as<long long>(), (int)n, 0, 64, w.s),
-              "radix sort size query");
-    w.tmp.reserve(tmp_bytes + 256);
     hip_check(hipMemcpyAsync(w.c.ptr, coords, n * (size_t)dim * sizeof(double), hipMemcpyHostToDevice, w.s),
               "upload coordinates");
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    iota_kernel<<<grid, 256, 0, w.s>>>(w.va.as<long long>(), nn);
-    hip_check(hipGetLastError(), "iota launch");
-    for (int d = dim - 1; d >= 0; --d) {
-      lexico_keys_kernel<<<grid, 256, 0, w.s>>>(w.c.as<double>(), dim, d, w.va.as<long long>(), nn,
-                                                w.ka.as<unsigned long long>());
-      hip_check(hipGetLastError(), "key launch");
-      hip_check(hipcub::DeviceRadixSort::SortPairs(w.tmp.ptr, tmp_bytes, w.ka.as<unsigned long long>(),
-                                                   w.kb.as<unsigned long long>(), w.va.as<long long>(),
-                                                   w.vb.as<long long>(), (int)n, 0, 64, w.s),
-                "radix sort");
-      std::swap(w.va, w.vb);
-    }
+    lexico_sort(w.c.as<double>(), dim, (long long)n, w.ka, w.kb, w.va, w.vb, w.tmp, w.s);
     hip_check(hipMemcpyAsync(order_out, w.va.ptr, n * sizeof(long long), hipMemcpyDeviceToHost, w.s),
               "download order");
     hip_check(hipStreamSynchronize(w.s), "hipStreamSynchronize");

@@ -30,10 +30,12 @@
 #include <memory>
 #include <stack>
 #include <thread>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
 #include <bbrcit_hip.h>
+#include "Attributes/AdaKdeAttributes.h"
 #include "Attributes/PointWeights.h"
 #include "DecoratedPoint.h"
 #include "FloatUtils.h"
@@ -43,6 +45,60 @@ namespace bbrcit {
 
 template <int D, typename KT, typename FT, typename AT>
 class KernelDensity;
+
+namespace kdtree_detail {
+
+// Attribute types whose merge the device reproduces exactly: their values
+// merge by summation, minimum or maximum and are held as double.  pack()
+// writes a point's values into the three groups, unpack() rebuilds the
+// attributes from merged values.
+template <typename AttrT>
+struct DeviceMerge {
+  static constexpr bool ok = false;
+};
+
+template <>
+struct DeviceMerge<PointWeights<double>> {
+  static constexpr bool ok = true;
+  static constexpr int n_sum = 1, n_min = 0, n_max = 0;
+  static void pack(const PointWeights<double> &a, double *s, double *, double *) { s[0] = a.weight(); }
+  static PointWeights<double> unpack(const double *s, const double *, const double *) {
+    return PointWeights<double>(s[0]);
+  }
+};
+
+template <>
+struct DeviceMerge<AdaKdeAttributes<double>> {
+  static constexpr bool ok = true;
+  static constexpr int n_sum = 2, n_min = 2, n_max = 2;
+  static void pack(const AdaKdeAttributes<double> &a, double *s, double *mn, double *mx) {
+    s[0] = a.weight();
+    s[1] = a.mass();
+    mn[0] = a.lower_abw();
+    mn[1] = a.lower();
+    mx[0] = a.upper_abw();
+    mx[1] = a.upper();
+  }
+  static AdaKdeAttributes<double> unpack(const double *s, const double *mn, const double *mx) {
+    AdaKdeAttributes<double> a(s[0]);
+    a.set_mass(s[1]);
+    a.set_lower_abw(mn[0]);
+    a.set_lower(mn[1]);
+    a.set_upper_abw(mx[0]);
+    a.set_upper(mx[1]);
+    return a;
+  }
+};
+
+}  // namespace kdtree_detail
+
+// Whether large trees (2^18 points and more) build their duplicate merge,
+// root box and node attributes on the HIP device when there is one (default
+// true); the host path gives the same tree.  For tests and comparisons.
+inline bool &kdtree_device_build() {
+  static bool on = true;
+  return on;
+}
 
 template <int D, typename AttrT = PointWeights<int>, typename FloatT = double>
 class Kdtree {
@@ -110,6 +166,24 @@ class Kdtree {
     for (IndexType i : idx) os << points_[i] << std::endl;
   }
 
+  // f(start, end, split, cell, attributes, is_leaf) for every node, in
+  // pre-order (an addition to the reference's interface, for inspection)
+  template <typename F>
+  void visit_nodes(F &&f) const {
+    if (!root_) return;
+    std::stack<const Node *> st;
+    st.push(root_);
+    while (!st.empty()) {
+      const Node *n = st.top();
+      st.pop();
+      f(n->start_idx_, n->end_idx_, n->split_, n->bbox_, n->attr_, n->is_leaf());
+      if (!n->is_leaf()) {
+        st.push(n->right_);
+        st.push(n->left_);
+      }
+    }
+  }
+
   // index ranges [start, end] of the leaves, left to right
   void report_leaves(std::vector<std::pair<IndexType, IndexType>> &out) const {
     if (!root_) return;
@@ -142,9 +216,16 @@ class Kdtree {
   Node *root_;
   int leaf_nmax_;
 
-  // recompute node attributes below n from the points' attributes
+  // recompute node attributes below n from the points' attributes (the
+  // whole tree of a large set with sibling subtrees on host threads)
   void refresh_node_attributes(Node *n) {
     if (!n) return;
+    if (n == root_ && points_.size() >= kParallelMin) {
+      int spawn_depth = 0;
+      while ((1 << spawn_depth) < build_threads()) ++spawn_depth;
+      refresh_parallel(n, spawn_depth);
+      return;
+    }
     if (n->is_leaf()) {
       n->attr_ = merged_attributes(n->start_idx_, n->end_idx_);
     } else {
@@ -163,6 +244,7 @@ class Kdtree {
   }
 
   void build() {
+    if (use_device_build() && device_build()) return;
     merge_duplicates();
     if (points_.empty()) return;
     int spawn_depth = 0;
@@ -209,6 +291,184 @@ class Kdtree {
     });
     points_.swap(sorted);
     return true;
+  }
+
+  using Merge = kdtree_detail::DeviceMerge<AttrT>;
+
+  bool use_device_build() const {
+    return std::is_same<FloatT, double>::value && kdtree_device_build() && points_.size() >= kParallelMin &&
+           points_.size() <= 0x7fffffffULL && device_sort_available();
+  }
+
+  // body(a, b) over [0, n) on up to build_threads() host threads
+  template <typename F>
+  static void on_threads(size_t n, F &&body) {
+    const int t = build_threads();
+    std::vector<std::future<void>> jobs;
+    for (int c = 0; c < t; ++c) {
+      const size_t a = n * (size_t)c / (size_t)t, b = n * (size_t)(c + 1) / (size_t)t;
+      jobs.push_back(std::async(std::launch::async, [&body, a, b] { body(a, b); }));
+    }
+    for (auto &j : jobs) j.get();
+  }
+
+  // A point's coordinates and its index among the distinct points: the
+  // partition runs on these records, whose comparisons (coordinate d) and
+  // moves are those of the same std::nth_element calls on the points, so the
+  // order is the same; the points move into it once at the end.
+  struct Record {
+    FloatT c[D];
+    IndexType i;
+  };
+
+  // Merged attributes of the runs of duplicates (run u: positions
+  // start[u] .. end - 1 of order), on the device for the attribute types it
+  // merges (bk_run_merge), else with the host merge; dup lists the runs.
+  void merge_runs(const std::vector<long long> &order, const std::vector<long long> &start, size_t n,
+                  const std::vector<size_t> &dup, std::vector<AttrT> &merged) const {
+    merged.resize(dup.size());
+    if (dup.empty()) return;
+    auto run_end = [&](size_t u) { return u + 1 < start.size() ? (size_t)start[u + 1] : n; };
+    if (merge_runs_device(order, start, run_end, dup, merged, std::integral_constant<bool, Merge::ok>())) return;
+    for (size_t k = 0; k < dup.size(); ++k) {
+      const size_t u = dup[k];
+      AttrT a = points_[(size_t)order[(size_t)start[u]]].attributes();
+      for (size_t p = (size_t)start[u] + 1; p < run_end(u); ++p) a = merge(a, points_[(size_t)order[p]].attributes());
+      merged[k] = a;
+    }
+  }
+
+  template <typename RunEnd>
+  bool merge_runs_device(const std::vector<long long> &, const std::vector<long long> &, RunEnd &&,
+                         const std::vector<size_t> &, std::vector<AttrT> &, std::false_type) const {
+    return false;
+  }
+  template <typename RunEnd>
+  bool merge_runs_device(const std::vector<long long> &order, const std::vector<long long> &start, RunEnd &&run_end,
+                         const std::vector<size_t> &dup, std::vector<AttrT> &merged, std::true_type) const {
+    constexpr int ns = Merge::n_sum, nmn = Merge::n_min, nmx = Merge::n_max;
+    std::vector<long long> off(dup.size() + 1, 0);
+    for (size_t k = 0; k < dup.size(); ++k) off[k + 1] = off[k] + (long long)(run_end(dup[k]) - (size_t)start[dup[k]]);
+    const size_t rows = (size_t)off.back();
+    std::vector<double> s(rows * ns), mn(rows * nmn), mx(rows * nmx);
+    double ds[ns > 0 ? ns : 1], dmn[nmn > 0 ? nmn : 1], dmx[nmx > 0 ? nmx : 1];
+    for (size_t k = 0, r = 0; k < dup.size(); ++k)
+      for (size_t p = (size_t)start[dup[k]]; p < run_end(dup[k]); ++p, ++r) {
+        Merge::pack(points_[(size_t)order[p]].attributes(), ds, dmn, dmx);
+        for (int j = 0; j < ns; ++j) s[r * ns + j] = ds[j];
+        for (int j = 0; j < nmn; ++j) mn[r * nmn + j] = dmn[j];
+        for (int j = 0; j < nmx; ++j) mx[r * nmx + j] = dmx[j];
+      }
+    std::vector<double> os(dup.size() * ns), omn(dup.size() * nmn), omx(dup.size() * nmx);
+    if (bk_run_merge(0, dup.size(), off.data(), ns, ns ? s.data() : nullptr, nmn, nmn ? mn.data() : nullptr, nmx,
+                     nmx ? mx.data() : nullptr, ns ? os.data() : nullptr, nmn ? omn.data() : nullptr,
+                     nmx ? omx.data() : nullptr) != BK_OK)
+      return false;
+    for (size_t k = 0; k < dup.size(); ++k)
+      merged[k] = Merge::unpack(ns ? &os[k * ns] : nullptr, nmn ? &omn[k * nmn] : nullptr,
+                                nmx ? &omx[k * nmx] : nullptr);
+    return true;
+  }
+
+  // Large sets with a HIP device: the sort, the runs of duplicates and the
+  // root box in one device pass (bk_lexico_runs), the runs' attribute merge
+  // on the device (bk_run_merge), the partition on host threads over
+  // compact records, the points moved into tree order once, node attributes
+  // on host threads.  false: nothing done.
+  bool device_build() {
+    const size_t n = points_.size();
+    std::vector<double> xy(n * D);
+    on_threads(n, [&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i)
+        for (int d = 0; d < D; ++d) xy[i * D + d] = (double)points_[i][d];
+    });
+    size_t nu = 0;
+    std::vector<long long> order(n), start(n);
+    std::vector<double> box(2 * D);
+    if (bk_lexico_runs(0, D, n, xy.data(), order.data(), &nu, start.data(), box.data()) != BK_OK) return false;
+    start.resize(nu);
+    std::vector<size_t> dup, dup_of(nu, (size_t)-1);
+    for (size_t u = 0; u < nu; ++u) {
+      const size_t e = u + 1 < nu ? (size_t)start[u + 1] : n;
+      if (e - (size_t)start[u] > 1) {
+        dup_of[u] = dup.size();
+        dup.push_back(u);
+      }
+    }
+    std::vector<AttrT> merged;
+    merge_runs(order, start, n, dup, merged);
+    std::vector<Record> rec(nu);
+    on_threads(nu, [&](size_t a, size_t b) {
+      for (size_t u = a; u < b; ++u) {
+        const size_t i = (size_t)order[(size_t)start[u]];
+        for (int d = 0; d < D; ++d) rec[u].c[d] = (FloatT)xy[i * D + d];  // exact: FloatT is double here
+        rec[u].i = u;
+      }
+    });
+    RectangleType cell;
+    for (int d = 0; d < D; ++d) cell.resize(d, typename RectangleType::EdgeType(box[2 * d], box[2 * d + 1]));
+    int spawn_depth = 0;
+    while ((1 << spawn_depth) < build_threads()) ++spawn_depth;
+    Node *root = make_node_rec(rec, 0, nu - 1, 0, cell, spawn_depth);
+    std::vector<DataPointType> out(nu);
+    on_threads(nu, [&](size_t a, size_t b) {
+      for (size_t k = a; k < b; ++k) {
+        const size_t u = rec[k].i;
+        out[k] = std::move(points_[(size_t)order[(size_t)start[u]]]);
+        if (dup_of[u] != (size_t)-1) out[k].set_attributes(merged[dup_of[u]]);
+      }
+    });
+    points_.swap(out);
+    root_ = root;
+    refresh_parallel(root_, spawn_depth);
+    return true;
+  }
+
+  // the partition of make_node over records; node attributes are set after
+  // the points move into tree order
+  Node *make_node_rec(std::vector<Record> &rec, IndexType i, IndexType j, int d, const RectangleType &cell,
+                      int spawn_depth) {
+    Node *n = new Node();
+    n->start_idx_ = i;
+    n->end_idx_ = j;
+    n->bbox_ = cell;
+    if (j - i + 1 <= (IndexType)leaf_nmax_) return n;
+    const IndexType m = i + (j - i) / 2;
+    std::nth_element(rec.begin() + i, rec.begin() + m, rec.begin() + j + 1,
+                     [d](const Record &a, const Record &b) { return a.c[d] < b.c[d]; });
+    n->split_ = rec[m].c[d];
+    const RectangleType lo_cell = cell.lower_halfspace(d, n->split_), hi_cell = cell.upper_halfspace(d, n->split_);
+    if (spawn_depth > 0) {
+      auto left = std::async(std::launch::async, [this, &rec, i, m, d, &lo_cell, spawn_depth] {
+        return make_node_rec(rec, i, m, (d + 1) % D, lo_cell, spawn_depth - 1);
+      });
+      n->right_ = make_node_rec(rec, m + 1, j, (d + 1) % D, hi_cell, spawn_depth - 1);
+      n->left_ = left.get();
+    } else {
+      n->left_ = make_node_rec(rec, i, m, (d + 1) % D, lo_cell, 0);
+      n->right_ = make_node_rec(rec, m + 1, j, (d + 1) % D, hi_cell, 0);
+    }
+    return n;
+  }
+
+  // node attributes below n (leaves: their points merged in index order;
+  // internal nodes: left merged with right), sibling subtrees on their own
+  // threads down to spawn_depth: the same values as the serial refresh
+  void refresh_parallel(Node *n, int spawn_depth) {
+    if (!n) return;
+    if (n->is_leaf()) {
+      n->attr_ = merged_attributes(n->start_idx_, n->end_idx_);
+      return;
+    }
+    if (spawn_depth > 0) {
+      auto left = std::async(std::launch::async, [this, n, spawn_depth] { refresh_parallel(n->left_, spawn_depth - 1); });
+      refresh_parallel(n->right_, spawn_depth - 1);
+      left.get();
+    } else {
+      refresh_parallel(n->left_, 0);
+      refresh_parallel(n->right_, 0);
+    }
+    n->attr_ = merge(n->left_->attr_, n->right_->attr_);
   }
 
   // chunks sorted on their own threads, then merged pairwise

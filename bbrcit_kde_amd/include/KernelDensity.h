@@ -36,6 +36,7 @@
 #include <memory>
 #include <random>
 #include <stdexcept>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -307,13 +308,31 @@ class KernelDensity {
   void init(std::vector<DataPointType> &&pts, int leaf_nmax) {
     FloatT total = ConstantTraits<FloatT>::zero();
     for (const auto &p : pts) total += p.attributes().weight();
-    for (auto &p : pts) {
-      auto &a = p.attributes();
-      a.set_weight(a.weight() / total);
-      a.set_mass(a.weight() * std::pow(a.abw(), -D));
+    // per point: weight / total, mass = weight * abw^-D; independent per
+    // point, so large sets split over host threads give the same values
+    // (pow(1, -D) is exactly 1, so a unit factor skips the call)
+    auto normalise = [&pts, total](size_t lo, size_t hi) {
+      for (size_t i = lo; i < hi; ++i) {
+        auto &a = pts[i].attributes();
+        a.set_weight(a.weight() / total);
+        const FloatT f = a.abw() == ConstantTraits<FloatT>::one() ? ConstantTraits<FloatT>::one()
+                                                                   : (FloatT)std::pow(a.abw(), -D);
+        a.set_mass(a.weight() * f);
+      }
+    };
+    const size_t n = pts.size();
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t t = n < (size_t(1) << 18) || hw < 2 ? 1 : std::min<size_t>(hw, 16);
+    if (t == 1) {
+      normalise(0, n);
+    } else {
+      std::vector<std::thread> workers;
+      for (size_t c = 0; c < t; ++c) workers.emplace_back(normalise, n * c / t, n * (c + 1) / t);
+      for (auto &w : workers) w.join();
     }
     data_tree_ = KdtreeType(std::move(pts), leaf_nmax);
     cum_weights_.clear();
+    cum_weights_.reserve(points().size());
     FloatT c = 0;
     for (const auto &p : points()) {
       c += p.attributes().weight();
@@ -362,19 +381,45 @@ class KernelDensity {
     dev.kernel() = kernel;
     std::vector<double> lo, up;
     dev.unnormalized_eval_bounded(0, qtree.size() - 1, (double)rel_err, (double)(abs_err / norm), lo, up);
-    for (size_t i = 0; i < qtree.points_.size(); ++i) {
-      auto &a = qtree.points_[i].attributes();
-      a.set_lower((FloatType)lo[i] * norm);
-      a.set_upper((FloatType)up[i] * norm);
-      report_error(std::cerr, qtree.points_[i].point(), a.upper(), a.lower(), rel_err, abs_err);
+    // bounds into the query points (large sets on host threads), then the
+    // reference's precision report for the queries that miss the tolerance,
+    // in query order
+    auto &qp = qtree.points_;
+    auto store = [&](size_t a0, size_t a1) {
+      for (size_t i = a0; i < a1; ++i) {
+        auto &a = qp[i].attributes();
+        a.set_lower((FloatType)lo[i] * norm);
+        a.set_upper((FloatType)up[i] * norm);
+      }
+    };
+    const size_t nq = qp.size();
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t t = nq < (size_t(1) << 18) || hw < 2 ? 1 : std::min<size_t>(hw, 16);
+    if (t == 1) {
+      store(0, nq);
+    } else {
+      std::vector<std::thread> workers;
+      for (size_t c = 0; c < t; ++c) workers.emplace_back(store, nq * c / t, nq * (c + 1) / t);
+      for (auto &w : workers) w.join();
     }
+    for (size_t i = 0; i < nq; ++i) {
+      const auto &a = qp[i].attributes();
+      if (!within_tolerance(a.upper(), a.lower(), rel_err, abs_err))
+        report_error(std::cerr, qp[i].point(), a.upper(), a.lower(), rel_err, abs_err);
+    }
+  }
+
+  // the tolerance test of report_error
+  static bool within_tolerance(FloatType upper, FloatType lower, FloatType rel_err, FloatType abs_err) {
+    const FloatType gap = std::abs(upper - lower);
+    if (gap <= abs_err) return true;
+    return lower && std::abs((upper - lower) / lower) <= rel_err;
   }
 
   void report_error(std::ostream &os, const GeomPointType &p, FloatType upper, FloatType lower, FloatType rel_err,
                     FloatType abs_err) const {
+    if (within_tolerance(upper, lower, rel_err, abs_err)) return;
     const FloatType gap = std::abs(upper - lower);
-    if (gap <= abs_err) return;
-    if (lower && std::abs((upper - lower) / lower) <= rel_err) return;
     os << std::setprecision(6) << (lower ? "Relative" : "Absolute") << " precision loss when querying " << p
        << ": " << std::endl
        << std::setprecision(15) << "\tlower:   " << lower << std::endl

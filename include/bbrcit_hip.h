@@ -175,6 +175,33 @@ int bk_lexico_order(int device, int dim, size_t n, const double *coords, long lo
 int bk_term_count(bk_ctx *ctx, unsigned long long *terms_out,
                   unsigned long long *query_terms_out);
 
+/* The first steps of Kdtree construction on HIP device `device`
+ * (include/Kdtree.h:364-390 merge_duplicates, 396-420 compute_bounding_box):
+ * order_out (n values) receives the order of bk_lexico_order; *n_runs_out
+ * the number of distinct points; run_start_out[u] (n rows) the position in
+ * that order of the first point of the u-th run of exactly equal points
+ * (ExactEqual: every coordinate ==), so run u holds positions
+ * run_start_out[u] .. run_start_out[u + 1] - 1 (the last run ends at n), its
+ * points in input order; box_out[2d], box_out[2d + 1]: the least and greatest
+ * coordinate d, grown from DBL_MAX and DBL_MIN as the reference grows its box
+ * (so the upper edge is at least DBL_MIN). */
+int bk_lexico_runs(int device, int dim, size_t n, const double *coords,
+                   long long *order_out, size_t *n_runs_out,
+                   long long *run_start_out, double *box_out);
+
+/* The attribute merge of runs of duplicates (merge_duplicates,
+ * include/Kdtree.h:364-390) on HIP device `device`.  Attribute values come
+ * packed in three groups per row (0 to 8 values each, row-major, NULL when
+ * empty): n_sum merged by summation, n_min by std::min, n_max by std::max.
+ * Run u holds rows run_offsets[u] .. run_offsets[u + 1] - 1 in merge order;
+ * its result (sums_out / mins_out / maxs_out[u * n_group + j]) is its first
+ * row merged with each next one, left to right: the host merge's exact
+ * operations, so bit-identical to it. */
+int bk_run_merge(int device, size_t n_runs, const long long *run_offsets,
+                 int n_sum, const double *sums, int n_min, const double *mins,
+                 int n_max, const double *maxs, double *sums_out,
+                 double *mins_out, double *maxs_out);
+
 /* ---- direct sums ----------------------------------------------------- */
 
 /* out_host[k - q_i] = S_k (times C(h) if normalize != 0) for k in [q_i, q_j],

@@ -311,6 +311,63 @@ int api_simulate(int kind, int dim, int kf, double hx, double hy, const double *
   });
 }
 
+// The tree of KernelDensity<2, EpanechnikovKernel<2, float>, double> over n
+// points (after adapt_density(alpha) when alpha > 0), built with the device
+// steps on or off (kdtree_device_build).  points_out: 6 values per tree point
+// (x, y, weight, mass, lower abw, upper abw); nodes_out: 13 values per node in
+// pre-order (start, end, split, cell x lo/hi, y lo/hi, weight, mass, lower
+// abw, upper abw, lower, upper).  Returns the node count (nodes_out holds
+// max_nodes rows) or a negative status.
+long api_tree_dump(const double *pts, const double *w, size_t n, int leaf_nmax, int use_device, double alpha,
+                   double *points_out, size_t max_nodes, double *nodes_out) {
+  return guarded([&]() -> int {
+    using KDE = KernelDensity<2, EpanechnikovKernel<2, float>, double>;
+    const bool saved = kdtree_device_build();
+    kdtree_device_build() = use_device != 0;
+    try {
+      KDE kde(make_points<KDE>(pts, w, n), leaf_nmax);
+      kde.kernel().set_bandwidth(0.2f);
+      if (alpha > 0) kde.adapt_density(alpha);
+      kdtree_device_build() = saved;
+      const auto &tp = kde.points();
+      for (size_t i = 0; i < tp.size(); ++i) {
+        const auto &a = tp[i].attributes();
+        double *o = points_out + 6 * i;
+        o[0] = tp[i][0];
+        o[1] = tp[i][1];
+        o[2] = a.weight();
+        o[3] = a.mass();
+        o[4] = a.lower_abw();
+        o[5] = a.upper_abw();
+      }
+      size_t k = 0;
+      kde.data_tree().visit_nodes([&](size_t i, size_t j, double split, const auto &cell, const auto &a, bool) {
+        if (k < max_nodes) {
+          double *o = nodes_out + 13 * k;
+          o[0] = (double)i;
+          o[1] = (double)j;
+          o[2] = split;
+          o[3] = cell[0].lower();
+          o[4] = cell[0].upper();
+          o[5] = cell[1].lower();
+          o[6] = cell[1].upper();
+          o[7] = a.weight();
+          o[8] = a.mass();
+          o[9] = a.lower_abw();
+          o[10] = a.upper_abw();
+          o[11] = a.lower();
+          o[12] = a.upper();
+        }
+        ++k;
+      });
+      return (int)k;
+    } catch (...) {
+      kdtree_device_build() = saved;
+      throw;
+    }
+  });
+}
+
 int api_likelihood_cv(int kind, int dim, int kf, double hx, double hy, const double *pts, const double *w, size_t n,
                       int leaf_nmax, double *out) {
   return guarded([&] {

@@ -54,6 +54,8 @@ def api():
     lib.api_grid_layouts.argtypes = [_DP, _SZ, _DP, _SZ, _D, _D, _I, _D, _D, _I, _DP, _DP]
     lib.api_marginal_eval.argtypes = [_I, _I, _D, _D, _DP, _DP, _SZ, _I, _D, _I, _DP, _SZ, _DP, _DP, _SZ, _DP]
     lib.api_simulate.argtypes = [_I, _I, _I, _D, _D, _DP, _DP, _SZ, _I, ctypes.c_ulonglong, _SZ, _I, _DP]
+    lib.api_tree_dump.restype = ctypes.c_long
+    lib.api_tree_dump.argtypes = [_DP, _DP, _SZ, _I, _I, _D, _DP, _SZ, _DP]
     return lib
 
 
@@ -134,6 +136,30 @@ def test_kdtree_device_sorted_build_matches_reference(api, reference):
         np.testing.assert_array_equal(c[:k], c_ref)
         np.testing.assert_allclose(ww[:k], w_ref, rtol=1e-14)
         np.testing.assert_allclose(m[:k], m_ref, rtol=1e-14)
+
+
+@pytest.mark.gpu
+def test_kdtree_device_build_equals_host_build(api):
+    # the device steps of a large build (sort, runs of duplicates and root
+    # box in bk_lexico_runs, the duplicates' attribute merge in bk_run_merge,
+    # the partition over records, node attributes on host threads, and the
+    # refresh after adapt_density) give the host build's tree bit for bit:
+    # points, merged attributes, splits, cells and every node's attributes
+    rng = np.random.default_rng(12)
+    n = 400_000
+    pts = rng.normal(size=(n, 2))
+    pts[::997] = pts[11]  # exact duplicates, merged
+    pts[5::1999, 0] = pts[7, 0]  # equal x, distinct y
+    w = rng.uniform(0.5, 1.5, n)
+    for leaf, alpha in ((32, 0.0), (4096, 0.0), (256, 0.5)):
+        outs = []
+        for use_device in (1, 0):
+            P, N = np.zeros((n, 6)), np.zeros((100_000, 13))
+            k = api.api_tree_dump(_p(pts), _p(w), n, leaf, use_device, alpha, _p(P), 100_000, _p(N))
+            assert 0 < k <= 100_000, _err(api)
+            outs.append((P, N[:k]))
+        np.testing.assert_array_equal(outs[0][0], outs[1][0], err_msg=f"points, leaf {leaf}, alpha {alpha}")
+        np.testing.assert_array_equal(outs[0][1], outs[1][1], err_msg=f"nodes, leaf {leaf}, alpha {alpha}")
 
 
 @pytest.mark.parametrize("kind,dim", [(EPAN, 1), (EPAN, 2), (GAUSS, 1), (GAUSS, 2), (EPANPROD, 2), (GAUSSPROD, 2),
