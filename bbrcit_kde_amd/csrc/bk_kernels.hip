@@ -42,6 +42,12 @@ constexpr float kGramMaxRho2 = 4.0f;      // compactness bound of the Gram form
 #endif
 constexpr float kFactoredMaxRho2 = BK_FACTORED_RHO2;  // compactness bound of the factored form
 constexpr float kFarExp = 100.0f;         // W Q2 beyond which a factored tile's terms are < 2^-49
+// FP64 contexts: W Q2 beyond which every term of a tile underflows to exactly
+// zero (exp2_f64 of an exponent below -1100 is below 2^-1074 before rounding)
+constexpr double kFarExp64 = 1100.0;
+// FP64 Gaussian culling keeps the culled part of every sum below 2^-kCullRelBits
+// of the sum (see gauss_f64_cull_exponent)
+constexpr int kCullRelBits = 50;
 
 // Evaluation form of an aligned tile (tiles[t].z).
 constexpr float kFormDirect = 0.0f;     // differences q - r
@@ -282,12 +288,13 @@ __device__ __forceinline__ float gauss_gram_term(float ax, float ay, float aq, f
 
 // 2^e for an FP64 exponent e <= 0: exp2 of the fractional part in FP32
 // (|f| <= 1/2, so its FP32 rounding costs < 2^-25 absolute) and the integer
-// part applied exactly by ldexp.
+// part applied exactly by an FP64 ldexp (no FP32 underflow below 2^-126;
+// the result is normal down to 2^-1000).
 __device__ __forceinline__ double exp2_split(double e) {
   double n = rint(e);
   if (n < -1000.0) n = -1000.0;
   const float f = (float)(e - n);
-  return (double)__builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+  return __builtin_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
 }
 
 // Sum of one tile's terms for kGaussQpt queries (self pair excluded when diag).
@@ -991,22 +998,24 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
 }
 
 // Per aligned tile of kTile64 FP64 references, two double4: the box (x0, x1,
-// y0, y1) and the smallest scales (min a.z, min a.w of pair_term_f64's
-// layout; a.w only for FAM_EPANPROD and FAM_EPANCONV).  A tile is culled for
-// a workgroup when its terms are exactly zero (Epanechnikov kinds) or below
-// 2^-kFarExp of their masses (Gaussian) for the whole box of its queries.
-// The test runs the term's own FP64 operations on the per-axis gaps between
-// the boxes with the tile's smallest scales; each is monotone, so the
-// Epanechnikov terms of a culled tile are zero as computed.
+// y0, y1) and (min a.z, min a.w, max a.z, sum |a.w|) of pair_term_f64's
+// layout (FAM_GAUSS: smallest and largest exponent scale, smallest mass, sum
+// of |mass|).  A tile is culled for a workgroup when its terms vanish for
+// the whole box of its queries: the Epanechnikov kinds past their support
+// (exactly zero), the Gaussian when every term's base-2 exponent is at least
+// the workgroup's t_cull (gauss_f64_cull_exponent).  The test runs the
+// term's own FP64 operations on the per-axis gaps between the boxes with the
+// tile's smallest scales; each is monotone, so every Epanechnikov term of a
+// culled tile is zero as computed.
 template <int FAM>
 __device__ __forceinline__ bool f64_tile_far(const double4 *__restrict__ tb, long long tile_idx, double bx0,
-                                             double bx1, double by0, double by1, double rho) {
+                                             double bx1, double by0, double by1, double rho, double t_cull) {
   const double4 b = tb[2 * tile_idx];
   const double4 w = tb[2 * tile_idx + 1];
   const double gx = fmax(0.0, fmax(b.x - bx1, bx0 - b.y));
   const double gy = fmax(0.0, fmax(b.z - by1, by0 - b.w));
   if constexpr (FAM == FAM_GAUSS) {
-    return w.x * __builtin_fma(rho * gy, gy, gx * gx) >= (double)kFarExp;
+    return w.x * __builtin_fma(rho * gy, gy, gx * gx) >= t_cull;
   } else if constexpr (FAM == FAM_EPAN) {
     return __builtin_fma(-w.x, __builtin_fma(gy, gy, gx * gx), 1.0) <= 0.0;
   } else if constexpr (FAM == FAM_EPANPROD) {
@@ -1016,17 +1025,98 @@ __device__ __forceinline__ bool f64_tile_far(const double4 *__restrict__ tb, lon
   }
 }
 
+// Lower bound of a query box's FP64 Gaussian sums and the mass that culling
+// can drop, accumulated over the tiles of a reference range.  A tile held
+// wholly in the range (and, under self exclusion, not holding one of the
+// box's own queries) whose smallest mass is positive adds at least
+// kTile64 m_min 2^-(W_max Q_far) to every query's sum, Q_far taken between
+// the far sides of the two boxes; with no negative mass the sum of these is
+// a lower bound of every query's sum.  Culled terms sum to at most 2^-T
+// times the range's sum of |m| when every culled term's exponent is at least T.
+struct CullBound {
+  double lower = 0.0;  // sum of the per-tile lower bounds
+  double mass = 0.0;   // sum of |m| over the tiles touching the range
+  double min_m = 0.0;  // smallest mass seen (only its sign is used)
+};
+
+__device__ __forceinline__ void cull_bound_add(CullBound &c, const double4 *__restrict__ tb, long long ti,
+                                               long long r_begin, long long r_end, bool own, double bx0, double bx1,
+                                               double by0, double by1, double rho) {
+  const double4 b = tb[2 * ti];
+  const double4 w = tb[2 * ti + 1];
+  c.mass += w.w;
+  c.min_m = fmin(c.min_m, w.y);
+  const long long s = ti * kTile64;
+  if (s >= r_begin && s + kTile64 <= r_end && !own && w.y > 0.0) {
+    const double fx = fmax(bx1 - b.x, b.y - bx0);
+    const double fy = fmax(by1 - b.z, b.w - by0);
+    c.lower += ((double)kTile64 * w.y) * exp2_f64(-w.z * __builtin_fma(rho * fy, fy, fx * fx));
+  }
+}
+
+// Culling exponent T of a box: log2(mass / lower) + kCullRelBits rounded up
+// (frexp's exponent exceeds log2 of the ratio), so the culled part of every
+// query's sum is below 2^-kCullRelBits of it; kFarExp64 (culled terms
+// underflow to zero) when no tile gives a lower bound or a mass is negative.
+__device__ __forceinline__ double cull_exponent(const CullBound &c) {
+  if (!(c.min_m >= 0.0) || !(c.lower > 0.0)) return kFarExp64;
+  const double ratio = c.mass / c.lower;
+  if (!(ratio < 1.0e300)) return kFarExp64;
+  int e = 0;
+  (void)__builtin_frexp(ratio, &e);
+  return fmin((double)e + (double)kCullRelBits, kFarExp64);
+}
+
+// The culling exponent of a workgroup: its threads share the tiles of
+// [r_begin, r_end) and reduce the bound over the workgroup.
+__device__ double gauss_f64_cull_exponent(const double4 *__restrict__ tb, long long r_begin, long long r_end,
+                                          long long q_lo, long long q_hi, int self_excl, double bx0, double bx1,
+                                          double by0, double by1, double rho, double (&sred)[3][kBlock / 64]) {
+  CullBound c;
+  const long long t1 = (r_end + kTile64 - 1) / kTile64;
+  for (long long ti = r_begin / kTile64 + threadIdx.x; ti < t1; ti += kBlock64) {
+    const bool own = self_excl && ti * kTile64 < q_hi && q_lo < ti * kTile64 + kTile64;
+    cull_bound_add(c, tb, ti, r_begin, r_end, own, bx0, bx1, by0, by1, rho);
+  }
+  double lower = c.lower, mass = c.mass;
+  for (int o = 32; o > 0; o >>= 1) {
+    lower += __shfl_xor(lower, o, 64);
+    mass += __shfl_xor(mass, o, 64);
+  }
+  const double min_m = wave_reduce_d(c.min_m, false);
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) {
+    sred[0][t >> 6] = lower;
+    sred[1][t >> 6] = mass;
+    sred[2][t >> 6] = min_m;
+  }
+  __syncthreads();
+  CullBound all;
+  all.lower = sred[0][0];
+  all.mass = sred[1][0];
+  all.min_m = sred[2][0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) {
+    all.lower += sred[0][w];
+    all.mass += sred[1][w];
+    all.min_m = fmin(all.min_m, sred[2][w]);
+  }
+  return cull_exponent(all);
+}
+
 // kQpt queries per thread (kBlock64 * kQpt per workgroup), each reference
 // read once from LDS for all of them; the self pair is excluded only on
 // tiles that hold the workgroup's own queries.  Reference tiles are aligned
 // to kTile64 (entries before r_begin are loaded as zero masses), and tiles out
 // of reach of the workgroup's query box are skipped (tb) or, when BOUNDED,
-// only the tiles listed in bmask are summed.
+// only the tiles listed in bmask are summed.  [c_begin, c_end) is the whole
+// range whose sum [r_begin, r_end) is a part of: Gaussian culling bounds the
+// dropped terms against that sum.
 template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void direct_f64_item(
     const double *__restrict__ qx, const double *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
-    long long r_begin, long long r_end, long long q_begin, int q_count,
+    long long r_begin, long long r_end, long long c_begin, long long c_end, long long q_begin, int q_count,
     double *__restrict__ dst, double out_scale, int self_excl, double rho,
     const unsigned long long *__restrict__ bmask) {
   constexpr int kNf = (FAM == FAM_EPANPROD || FAM == FAM_EPANCONV) ? 2 : 1;
@@ -1034,6 +1124,7 @@ __device__ __forceinline__ void direct_f64_item(
   __shared__ double4 s0[kTile64];
   __shared__ double4 s1[kNf == 2 ? kTile64 : 1];
   __shared__ double sbox[4][kBlock / 64];
+  __shared__ double sred[3][kBlock / 64];
   const int t = threadIdx.x;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
@@ -1050,13 +1141,16 @@ __device__ __forceinline__ void direct_f64_item(
   }
   double bx0, bx1, by0, by1;
   block_query_box_d<kQpt>(px, py, sbox, bx0, bx1, by0, by1);
+  const long long q_lo = q_begin, q_hi = q_begin + q_count;
+  double t_cull = kFarExp64;
+  if constexpr (FAM == FAM_GAUSS && !BOUNDED)
+    t_cull = gauss_f64_cull_exponent(tb, c_begin, c_end, q_lo, q_hi, self_excl, bx0, bx1, by0, by1, rho, sred);
   auto live_from = [&](long long tl) {
     if constexpr (BOUNDED) return next_listed_tile<kTile64>(bmask, tl, r_end);
     for (; tl < r_end; tl += kTile64)
-      if (!f64_tile_far<FAM>(tb, tl / kTile64, bx0, bx1, by0, by1, rho)) break;
+      if (!f64_tile_far<FAM>(tb, tl / kTile64, bx0, bx1, by0, by1, rho, t_cull)) break;
     return tl;
   };
-  const long long q_lo = q_begin, q_hi = q_begin + q_count;
   const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
   for (long long tile = live_from((r_begin / kTile64) * kTile64); tile < r_end;
        tile = live_from(tile + kTile64)) {
@@ -1117,8 +1211,8 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_range_kernel(
   const long long left = nq - qb;
   const int q_count = (int)(left < kQb ? left : kQb);
   const long long off = by * nq + qb;
-  direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
-                                rho, BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
+  direct_f64_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, r_i, r_end, q_i + qb, q_count, dst + off, out_scale,
+                                self_excl, rho, BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
 }
 
 template <int FAM>
@@ -1128,8 +1222,8 @@ __global__ __launch_bounds__(kBlock64) void direct_f64_list_kernel(
     const WorkItem *__restrict__ items, double *__restrict__ dst,
     double out_scale, int self_excl, double rho) {
   const WorkItem it = items[blockIdx.x];
-  direct_f64_item<FAM, false>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count,
-                              dst + it.out_index, out_scale, self_excl, rho, nullptr);
+  direct_f64_item<FAM, false>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.r_begin, it.r_end, it.q_begin,
+                              it.q_count, dst + it.out_index, out_scale, self_excl, rho, nullptr);
 }
 
 __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
@@ -1161,7 +1255,7 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
 }
 
 // One workgroup of kTile64 threads per aligned tile: the tile's FP64 box and
-// smallest scales (see f64_tile_far).
+// scale and mass summary (see f64_tile_far).
 __global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *__restrict__ x,
                                                                   const double *__restrict__ y,
                                                                   const double4 *__restrict__ f0, long long n,
@@ -1192,9 +1286,17 @@ __global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *_
   const double y1 = reduce(valid ? yi : -big, true);
   const double wz = reduce(a.z, false);
   const double ww = reduce(a.w, false);
+  const double wz_max = reduce(valid ? a.z : 0.0, true);
+  // sum of |a.w| by the same tree (an upper bound of the tile's |mass|)
+  sh[threadIdx.x] = valid ? __builtin_fabs(a.w) : 0.0;
+  __syncthreads();
+  for (int o = kTile64 / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     tb[2 * blockIdx.x] = make_double4(x0, x1, y0, y1);
-    tb[2 * blockIdx.x + 1] = make_double4(wz, ww, 0.0, 0.0);
+    tb[2 * blockIdx.x + 1] = make_double4(wz, ww, wz_max, sh[0]);
   }
 }
 
@@ -1760,6 +1862,15 @@ __global__ void live_count_kernel(const double4 *__restrict__ boxes, const void 
     re = rb + len < r_end ? rb + len : r_end;
     if (rb < r_i) rb = r_i;
   }
+  double t_cull = kFarExp64;
+  if constexpr (KIND == 2 && FAM == FAM_GAUSS) {
+    // the kernel's culling exponent for this box, without the exclusion of
+    // the box's own tiles (an estimate for the launch order only)
+    CullBound cb;
+    for (long long ti = r_i / T; ti < (r_end + T - 1) / T; ++ti)
+      cull_bound_add(cb, (const double4 *)meta, ti, r_i, r_end, false, box.x, box.y, box.z, box.w, rho);
+    t_cull = cull_exponent(cb);
+  }
   unsigned int c = 0;
   for (long long tl = (rb / T) * T; tl < re; tl += T) {
     bool far;
@@ -1770,7 +1881,7 @@ __global__ void live_count_kernel(const double4 *__restrict__ boxes, const void 
       far = epan_tile_far<FAM>((const double4 *)meta, tl / T, (float)box.x, (float)box.y, (float)box.z,
                                (float)box.w);
     else
-      far = f64_tile_far<FAM>((const double4 *)meta, tl / T, box.x, box.y, box.z, box.w, rho);
+      far = f64_tile_far<FAM>((const double4 *)meta, tl / T, box.x, box.y, box.z, box.w, rho, t_cull);
     c += far ? 0u : 1u;
   }
   keys[i] = ~c;

@@ -7,9 +7,10 @@ with and without the spatial order.  For the Gaussian kinds the skipped
 terms of the factored tiles are zero too; those of the difference and Gram
 forms are below 2^-100 of the largest mass, so both orders must meet the
 oracle at the parity bar (relative 1e-5 above the floor of 1e-10 times an
-upper bound of the sum, absolute below it; 1e-12 for FP64 contexts).  The
-cases use kd-ordered references and small bandwidths, where most tiles are
-out of reach of a workgroup of sorted queries.
+upper bound of the sum, absolute below it).  FP64 contexts cull only tiles
+whose terms are exactly zero and are held to 1e-12 (Gaussian kinds relative
+down to 1e-290).  The cases use kd-ordered references and small bandwidths, where
+most tiles are out of reach of a workgroup of sorted queries.
 """
 from __future__ import annotations
 
@@ -128,10 +129,10 @@ def test_epanechnikov_culling_far_from_origin(bk, oracle, kind, h):
 @pytest.mark.parametrize("adaptive", [False, True])
 @pytest.mark.parametrize("kind,h", CASES)
 def test_f64_spatial_order_and_culling(bk, oracle, kind, h, adaptive):
-    # FP64 contexts: the same culling on FP64 boxes (Epanechnikov terms of a
-    # culled tile are exactly zero, Gaussian ones below 2^-100 of their mass),
-    # held to the FP64 bar of 1e-12 in both orders; with adaptive bandwidths
-    # the reach of a tile follows its largest abw
+    # FP64 contexts: the same culling on FP64 boxes (every term of a culled
+    # tile is exactly zero in FP64: Epanechnikov past its support, Gaussian
+    # below 2^-1100), held to the FP64 bar of 1e-12 in both orders; with
+    # adaptive bandwidths the reach of a tile follows its largest abw
     n_ref, n_q = 20_000, 7_000
     pts, mass = kd_mixture(n_ref, 900 + kind)
     q = ds.uniform_box(n_q, -4.0, 4.0, 950 + kind)
@@ -142,8 +143,8 @@ def test_f64_spatial_order_and_culling(bk, oracle, kind, h, adaptive):
         spatial = evaluate(bk, kde, desc, n_q, bk.BK_QUERY_ORDER_SPATIAL)
     want = oracle.direct(kind, h[0], h[1], pts, q, mass=mass, abw=abw, normalize=True)
     bound = oracle.normalization(kind, 2, *h) * mass.sum()
-    assert_parity(spatial, want, bound, rtol=1e-12, what=f"f64 spatial kind {kind} adaptive {adaptive}")
-    assert_parity(given, want, bound, rtol=1e-12, what=f"f64 given kind {kind} adaptive {adaptive}")
+    assert_parity(spatial, want, bound, rtol=1e-12, what=f"f64 spatial kind {kind} adaptive {adaptive}", kind=kind)
+    assert_parity(given, want, bound, rtol=1e-12, what=f"f64 given kind {kind} adaptive {adaptive}", kind=kind)
     if kind in EPAN_KINDS:
         np.testing.assert_array_equal(spatial, given)
     assert (want > 0).mean() > 0.05
@@ -173,8 +174,8 @@ def test_1d_spatial_order_and_culling(bk, oracle, kind, h, f64):
     want = oracle.direct(kind, h[0], h[1], P, Q, mass=M, normalize=True)
     bound = oracle.normalization(kind, 1, *h) * mass.sum()
     rtol = 1e-12 if f64 else 1e-5
-    assert_parity(spatial, want, bound, rtol=rtol, what=f"1-D spatial kind {kind} f64 {f64}")
-    assert_parity(given, want, bound, rtol=rtol, what=f"1-D given kind {kind} f64 {f64}")
+    assert_parity(spatial, want, bound, rtol=rtol, what=f"1-D spatial kind {kind} f64 {f64}", kind=kind)
+    assert_parity(given, want, bound, rtol=rtol, what=f"1-D given kind {kind} f64 {f64}", kind=kind)
     if kind in (EPAN, EPANCONV1):
         np.testing.assert_array_equal(spatial, given)
     assert (want > 0).mean() > 0.05
@@ -187,7 +188,7 @@ def test_f64_self_eval_culls_in_tree_order(bk, oracle, kind, h):
     with bk.DirectKde(pts, None, mass, precision=bk.BK_PREC_F64) as kde:
         got = kde.self_eval(desc, exclude_self=True)
     want = oracle.direct(kind, h[0], h[1], pts, pts, mass=mass, exclude_self=True, normalize=True)
-    assert_parity(got, want, oracle.normalization(kind, 2, *h) * mass.sum(), rtol=1e-12, what=f"f64 loo {kind}")
+    assert_parity(got, want, oracle.normalization(kind, 2, *h) * mass.sum(), rtol=1e-12, what=f"f64 loo {kind}", kind=kind)
 
 
 def test_query_order_option_is_checked(bk):

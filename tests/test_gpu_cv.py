@@ -125,13 +125,13 @@ def test_f64_leaf_pairs_and_self_sums(bk, oracle):
             outs = kde.eval_leaf_pairs(desc, pairs)
         for (r_i, r_j, q_i, q_j), got in zip(pairs, outs):
             want = oracle.direct(kind, *h, pts[r_i:r_j + 1], q[q_i:q_j + 1], mass=mass[r_i:r_j + 1])
-            assert_parity(got, want, mass[r_i:r_j + 1].sum(), rtol=1e-12, what=f"f64 kind {kind} pair {r_i, r_j}")
+            assert_parity(got, want, mass[r_i:r_j + 1].sum(), rtol=1e-12, what=f"f64 kind {kind} pair {r_i, r_j}", kind=kind)
         with bk.DirectKde(pts, None, mass, precision=bk.BK_PREC_F64) as kde:
             full = kde.self_eval(desc, exclude_self=False, normalize=False)
             loo = kde.self_eval(desc, exclude_self=True, normalize=False)
-        assert_parity(full, oracle.direct(kind, *h, pts, pts, mass=mass), mass.sum(), rtol=1e-12, what="f64 full")
+        assert_parity(full, oracle.direct(kind, *h, pts, pts, mass=mass), mass.sum(), rtol=1e-12, what="f64 full", kind=kind)
         assert_parity(loo, oracle.direct(kind, *h, pts, pts, mass=mass, exclude_self=True), mass.sum(),
-                      rtol=1e-12, what="f64 loo")
+                      rtol=1e-12, what="f64 loo", kind=kind)
 
 
 @pytest.mark.parametrize("kind,dim,h", [(GAUSSPROD, 2, (0.05, 0.07)), (EPANPROD, 2, (0.15, 0.2)),
@@ -164,7 +164,7 @@ def test_f64_adapt_matches_oracle(bk, oracle):
     np.testing.assert_allclose(lam, lam_want, rtol=1e-11)
     want_after = oracle.direct(GAUSSPROD, 0.05, 0.05, pts, pts, mass=w * lam ** -2.0, abw=lam, normalize=True)
     assert_parity(after, want_after, oracle.normalization(GAUSSPROD, 2, 0.05, 0.05) * (w * lam ** -2.0).sum(),
-                  rtol=1e-12, what="f64 adapted self density")
+                  rtol=1e-12, what="f64 adapted self density", kind=GAUSSPROD)
 
 
 def test_adapt_matches_oracle(bk, oracle):
