@@ -54,6 +54,7 @@ BK_MLP_NUM_PARAMS = 33537
 
 BK_CVG_VARIANT = 1
 BK_CVG_GRAPH = 2
+BK_CVG_COUNT_UNITS = 3
 BK_CVG_VARIANTS = 2
 
 BK_SET_REF = 0
@@ -191,6 +192,7 @@ def load_library() -> ctypes.CDLL:
         "bk_cv_grid_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "bk_cv_grid_last_timing": (ctypes.c_int, [ctypes.c_void_p, _DBL_P, _DBL_P,
                                                   ctypes.POINTER(ctypes.c_longlong)]),
+        "bk_cv_grid_unit_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]),
         "bk_mlp_create": (ctypes.c_int, [c_ctx_p, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]),
         "bk_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
         "bk_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _FLT_P, ctypes.c_double, ctypes.c_double,
@@ -519,6 +521,13 @@ class CvGrid:
         m, t, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
         _check(self._lib.bk_cv_grid_last_timing(self._plan, ctypes.byref(m), ctypes.byref(t), ctypes.byref(r)))
         return m.value, t.value, r.value
+
+    def unit_count(self) -> int:
+        """(query, reference, bandwidth pair) units the pair kernel evaluated in
+        the last run, which ran with set_option(BK_CVG_COUNT_UNITS, 1)."""
+        u = ctypes.c_ulonglong()
+        _check(self._lib.bk_cv_grid_unit_count(self._plan, ctypes.byref(u)))
+        return int(u.value)
 
 
 class Surrogate:

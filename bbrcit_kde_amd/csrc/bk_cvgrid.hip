@@ -84,6 +84,7 @@ struct CvArgs {
   long long plane_stride;
   long long n_points;
   int n_hx, n_hy;
+  unsigned long long *count;  // when set, the Epanechnikov kernel adds the units it evaluates
 };
 
 // ---- Gaussian product kernel: 8 or 4 x-factors, 8 y-factors per reference ----
@@ -336,6 +337,13 @@ __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
       const double gy = fmax(0.0, fmax(tb[2] + qb[3], qb[2] + tb[3]));
       const double ux = (ADAPT ? gx * tb[4] : gx) * kx_min, uy = (ADAPT ? gy * tb[4] : gy) * ky_min;
       if (!(ux >= 2.0 || uy >= 2.0)) {
+        if (g.count && t == 0) {
+          // (query, reference, bandwidth pair) units of this stage
+          const int la = g.n_hx - w.sub_x < A ? g.n_hx - w.sub_x : A;
+          const int lb = g.n_hy - w.sub_y < B ? g.n_hy - w.sub_y : B;
+          const int nr = w.r_end - r0 < kCvTile ? w.r_end - r0 : kCvTile;
+          atomicAdd(g.count, (unsigned long long)nr * (unsigned long long)w.q_count * (unsigned long long)(la * lb));
+        }
         if (r0 < q_end && w.q_begin < rn)
           epan_stage<A, B, ADAPT, true>(tile[buf], q.x, q.y, qi - r0, kx, ky, accK, accT);
         else
@@ -598,6 +606,9 @@ struct bk_cv_grid {
   bool have_run = false;
   double last_main_ms = std::nan(""), last_total_ms = 0.0;
   long long last_rescued = 0;
+  bool count_units = false;           // option BK_CVG_COUNT_UNITS
+  DevBuf counter;                     // units evaluated by the pair kernel
+  long long last_units = -1;          // of the last run; -1: not counted
   int pad_hx = 0, pad_hy = 0;
 
   // device constant table layout (doubles): kx64[pad_hx] ky64[pad_hy]
@@ -732,8 +743,9 @@ int resident_blocks(PairKernel k) {
   return per_cu * cu_count();
 }
 
-CvArgs make_args(const bk_cv_grid *g, const CvItem *items, int n_items) {
+CvArgs make_args(const bk_cv_grid *g, const CvItem *items, int n_items, bool counting = false) {
   CvArgs a{};
+  a.count = counting ? g->counter.as<unsigned long long>() : nullptr;
   a.pts = g->pts.as<float4>();
   a.pts64 = g->pts64.as<double4>();
   a.items = items;
@@ -820,7 +832,7 @@ void enqueue_pipeline(bk_cv_grid *g, bool time_main) {
   const long long G = (long long)g->n_hx * g->n_hy;
   const int planes = g->fam == CV_GAUSS ? 1 : 2;
   const Variant v = variant_shape(g->fam, g->used_variant);
-  const CvArgs args = make_args(g, g->items.as<CvItem>(), g->items_n);
+  const CvArgs args = make_args(g, g->items.as<CvItem>(), g->items_n, g->count_units && g->fam == CV_EPAN);
   hipLaunchKernelGGL(cv_reset_kernel, dim3(1), dim3(64), 0, s, g->queue.as<int>());
   hip_check(hipGetLastError(), "queue reset launch");
   if (time_main) hip_check(hipEventRecord(g->ev[2], s), "hipEventRecord");
@@ -971,7 +983,12 @@ void run_plan(bk_cv_grid *g, const double *hx, const double *hy) {
   upload_consts(g, hx, hy);
   prepare_shape(g);
   const hipStream_t s = g->stream;
-  if (g->use_graph) {
+  const bool counting = g->count_units && g->fam == CV_EPAN;
+  if (counting) {
+    g->counter.reserve(sizeof(unsigned long long));
+    hip_check(hipMemsetAsync(g->counter.ptr, 0, sizeof(unsigned long long), s), "clear the unit counter");
+  }
+  if (g->use_graph && !g->count_units) {
     if (!g->exec) {
       hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       try {
@@ -997,7 +1014,7 @@ void run_plan(bk_cv_grid *g, const double *hx, const double *hy) {
   float total = 0.f;
   hip_check(hipEventElapsedTime(&total, g->ev[0], g->ev[1]), "hipEventElapsedTime");
   g->last_total_ms = total;
-  if (!g->use_graph) {
+  if (!g->use_graph || g->count_units) {
     float main_ms = 0.f;
     hip_check(hipEventElapsedTime(&main_ms, g->ev[2], g->ev[3]), "hipEventElapsedTime");
     g->last_main_ms = main_ms;
@@ -1008,6 +1025,20 @@ void run_plan(bk_cv_grid *g, const double *hx, const double *hy) {
   hip_check(hipMemcpy(counts, g->queue.ptr, sizeof(counts), hipMemcpyDeviceToHost), "read counters");
   g->last_rescued = counts[1];
   if (g->fam == CV_GAUSS && counts[1] > g->rescue_cap) fail(BK_ERR_STATE, "internal: rescue list overflow");
+  g->last_units = -1;
+  if (counting) {
+    unsigned long long units = 0;
+    hip_check(hipMemcpy(&units, g->counter.ptr, sizeof(units), hipMemcpyDeviceToHost), "read the unit counter");
+    g->last_units = (long long)units;
+  } else if (g->count_units) {
+    // the Gaussian pair kernel evaluates every unit
+    long long units = 0;
+    for (int d = 0; d < g->n_sets; ++d) {
+      const long long n = g->begin[d + 1] - g->begin[d];
+      units += n * n;
+    }
+    g->last_units = units * g->n_hx * g->n_hy;
+  }
   g->have_run = true;
 }
 
@@ -1209,6 +1240,10 @@ int bk_cv_grid_set_option(bk_cv_grid *g, int option, int value) {
         if (value != 0 && value != 1) fail(BK_ERR_INVALID_ARG, "BK_CVG_GRAPH takes 0 or 1");
         g->use_graph = value != 0;
         break;
+      case BK_CVG_COUNT_UNITS:
+        if (value != 0 && value != 1) fail(BK_ERR_INVALID_ARG, "BK_CVG_COUNT_UNITS takes 0 or 1");
+        g->count_units = value != 0;
+        break;
       default:
         fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));
     }
@@ -1226,6 +1261,9 @@ int bk_cv_grid_get_option(const bk_cv_grid *g, int option, int *value) {
       case BK_CVG_GRAPH:
         *value = g->use_graph ? 1 : 0;
         break;
+      case BK_CVG_COUNT_UNITS:
+        *value = g->count_units ? 1 : 0;
+        break;
       default:
         fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));
     }
@@ -1239,6 +1277,15 @@ int bk_cv_grid_last_timing(const bk_cv_grid *g, double *main_ms, double *total_m
     if (main_ms) *main_ms = g->last_main_ms;
     if (total_ms) *total_ms = g->last_total_ms;
     if (rescued) *rescued = g->last_rescued;
+  });
+}
+
+int bk_cv_grid_unit_count(const bk_cv_grid *g, unsigned long long *evaluated) {
+  return guarded([&] {
+    if (!g) fail(BK_ERR_INVALID_ARG, "null plan");
+    if (!evaluated) fail(BK_ERR_INVALID_ARG, "null output pointer");
+    if (!g->have_run || g->last_units < 0) fail(BK_ERR_STATE, "the last run did not count (BK_CVG_COUNT_UNITS)");
+    *evaluated = (unsigned long long)g->last_units;
   });
 }
 

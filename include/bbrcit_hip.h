@@ -366,6 +366,10 @@ typedef struct bk_cv_grid bk_cv_grid;
 #define BK_CVG_VARIANTS   2  /* number of tilings: 0 = 8x8, 1 = 4x8 (Gaussian),
                                 0 = 4x8, 1 = 4x4 (Epanechnikov) bandwidth
                                 pairs per thread */
+#define BK_CVG_COUNT_UNITS 3 /* 0 (default); 1: count the (query, reference,
+                                bandwidth pair) units the pair kernel
+                                evaluates (runs launch eagerly while set;
+                                read with bk_cv_grid_unit_count) */
 
 /* Create a plan over n_sets datasets of 2-D points stored back to back:
  * set d holds sizes[d] >= 1 points; coords is row-major (x, y) over all
@@ -402,6 +406,16 @@ int bk_cv_grid_get_option(const bk_cv_grid *plan, int option, int *value);
  * in FP64.  Any pointer may be NULL. */
 int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
                            double *total_ms, long long *rescued);
+
+/* Units of the last run evaluated by the pair kernel, when it ran with
+ * BK_CVG_COUNT_UNITS set: for BK_EPANECHNIKOV_PRODUCT_2D the units of the
+ * 256-reference stages it did not skip (a skipped stage lies beyond the
+ * support of every bandwidth of its sub-grid, its terms exactly zero), for
+ * BK_GAUSSIAN_PRODUCT_2D every unit (sum of n_d^2 times n_hx * n_hy; the
+ * Gaussian kernel skips none).  Self pairs are counted as evaluated.  No
+ * counterpart in the reference, whose cross-validation evaluates every pair
+ * a tree traversal does not prune. */
+int bk_cv_grid_unit_count(const bk_cv_grid *plan, unsigned long long *evaluated);
 
 /* ---- sampling ------------------------------------------------------------ */
 /* The device part of KernelDensity::simulate (include/KernelDensityImpl.h:

@@ -204,3 +204,46 @@ def test_errors(bk):
             cv.loo(0, 2, 0)
         with pytest.raises(IndexError):
             cv.loo(1, 0, 0)
+
+
+def test_unit_count(bk):
+    """BK_CVG_COUNT_UNITS: the Gaussian kernel evaluates every (query,
+    reference, bandwidth pair) unit; the Epanechnikov kernel skips only
+    stages whose terms are all zero, so its count lies between the units
+    inside the convolution kernel's support (|dx| < 2 hx and |dy| < 2 hy,
+    which it must evaluate) and all units.  Counting does not change the
+    scores."""
+    sets = [mixture(3000, 621), mixture(900, 622)]
+    hx = np.geomspace(0.03, 0.4, 6)
+    hy = np.geomspace(0.02, 0.5, 9)
+    total = sum(len(p) ** 2 for p, _ in sets) * len(hx) * len(hy)
+    inside = 0
+    for p, _ in sets:
+        P = f32(p)
+        dx = np.abs(P[:, None, 0] - P[None, :, 0])
+        dy = np.abs(P[:, None, 1] - P[None, :, 1])
+        for a in hx:
+            mx = dx < 2.0 * a * (1 - 1e-9)
+            for b in hy:
+                inside += int(np.count_nonzero(mx & (dy < 2.0 * b * (1 - 1e-9))))
+    for kind in (GAUSSPROD, EPANPROD):
+        with bk.CvGrid(kind, [p for p, _ in sets], [w for _, w in sets]) as cv:
+            base = cv.run(hx, hy)
+            with pytest.raises(bk.BkError):
+                cv.unit_count()
+            cv.set_option(bk.BK_CVG_COUNT_UNITS, 1)
+            counted = cv.run(hx, hy)
+            units = cv.unit_count()
+            again = cv.run(hx, hy)
+            assert cv.unit_count() == units
+            cv.set_option(bk.BK_CVG_COUNT_UNITS, 0)
+            after = cv.run(hx, hy)
+        for r in (counted, again, after):
+            np.testing.assert_array_equal(r[0], base[0])
+            np.testing.assert_array_equal(r[1], base[1])
+        print(f"[cvgrid] kind {kind}: {units} of {total} units evaluated ({units / total:.1%}), "
+              f"{inside} inside the support")
+        if kind == GAUSSPROD:
+            assert units == total
+        else:
+            assert inside <= units < total
