@@ -27,7 +27,8 @@ __all__ = [
     "BK_EPANECHNIKOV", "BK_GAUSSIAN", "BK_EPANECHNIKOV_PRODUCT_2D", "BK_GAUSSIAN_PRODUCT_2D",
     "BK_EPANECHNIKOV_CONV_1D", "BK_EPANECHNIKOV_PRODUCT_CONV_2D", "BK_GAUSSIAN_CONV_1D",
     "BK_GAUSSIAN_PRODUCT_CONV_2D", "BK_PREC_F32", "BK_PREC_F64", "BkError", "KernelDesc",
-    "DirectKde", "CvGrid", "Surrogate", "load_library", "library_path", "lexico_order",
+    "DirectKde", "CvGrid", "Surrogate", "load_library", "library_path", "lexico_order", "tune_set_cache",
+    "tune_entry_count",
 ]
 
 # ---- constants (mirror include/bbrcit_hip.h) ---------------------------------
@@ -55,6 +56,9 @@ BK_MLP_NUM_PARAMS = 33537
 BK_CVG_VARIANT = 1
 BK_CVG_GRAPH = 2
 BK_CVG_COUNT_UNITS = 3
+BK_CVG_TUNE_SOURCE = 4
+BK_CVG_UNROLL = 5
+BK_CVG_WORKGROUPS = 6
 BK_CVG_VARIANTS = 2
 
 BK_SET_REF = 0
@@ -193,6 +197,10 @@ def load_library() -> ctypes.CDLL:
         "bk_cv_grid_last_timing": (ctypes.c_int, [ctypes.c_void_p, _DBL_P, _DBL_P,
                                                   ctypes.POINTER(ctypes.c_longlong)]),
         "bk_cv_grid_unit_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]),
+        "bk_tune_set_cache": (ctypes.c_int, [ctypes.c_char_p]),
+        "bk_cv_grid_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _DBL_P, ctypes.c_size_t, _DBL_P,
+                                           _DBL_P, ctypes.POINTER(ctypes.c_int)]),
+        "bk_tune_entry_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
         "bk_mlp_create": (ctypes.c_int, [c_ctx_p, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]),
         "bk_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
         "bk_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _FLT_P, ctypes.c_double, ctypes.c_double,
@@ -228,6 +236,20 @@ def _check(status: int) -> None:
     if status == BK_ERR_INVALID_ARG:
         raise ValueError(msg)
     raise BkError(status, msg)
+
+
+def tune_set_cache(path) -> None:
+    """Name the autotuner cache file (loaded now, saved after every new
+    choice); None keeps the cache in memory only.  Drops the entries in memory."""
+    lib = load_library()
+    _check(lib.bk_tune_set_cache(None if path is None else str(path).encode()))
+
+
+def tune_entry_count() -> int:
+    """Entries in the in-memory autotuner cache."""
+    n = ctypes.c_size_t()
+    _check(load_library().bk_tune_entry_count(ctypes.byref(n)))
+    return int(n.value)
 
 
 def _dptr(a: Optional[np.ndarray]):
@@ -521,6 +543,19 @@ class CvGrid:
         m, t, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
         _check(self._lib.bk_cv_grid_last_timing(self._plan, ctypes.byref(m), ctypes.byref(t), ctypes.byref(r)))
         return m.value, t.value, r.value
+
+    def tune(self, hx, hy) -> list:
+        """Time every schedule of the pair kernel on whole runs of this grid and
+        record the fastest in the autotuner cache: [(tiling variant, unroll
+        index, pair-kernel ms), ...] in candidate order."""
+        hx = np.ascontiguousarray(hx, dtype=np.float64)
+        hy = np.ascontiguousarray(hy, dtype=np.float64)
+        ms = np.zeros(8)
+        n = ctypes.c_int()
+        _check(self._lib.bk_cv_grid_tune(self._plan, len(hx), _dptr(hx), len(hy), _dptr(hy), _dptr(ms),
+                                         ctypes.byref(n)))
+        per = n.value // BK_CVG_VARIANTS
+        return [(i // per, i % per, float(ms[i])) for i in range(n.value)]
 
     def unit_count(self) -> int:
         """(query, reference, bandwidth pair) units the pair kernel evaluated in

@@ -20,7 +20,7 @@ CSRC = PKG_DIR / "csrc"
 LIB_DIR = PKG_DIR / "lib"
 LIB_PATH = LIB_DIR / "libbbrcit_hip.so"
 
-HIP_SOURCES = ["bk_kernels.hip", "bk_capi.hip", "bk_cvgrid.hip", "bk_mlp.hip", "bk_order.hip", "bk_sample.hip", "bk_tree.hip"]
+HIP_SOURCES = ["bk_kernels.hip", "bk_capi.hip", "bk_cvgrid.hip", "bk_mlp.hip", "bk_order.hip", "bk_sample.hip", "bk_tree.hip", "bk_tune.hip"]
 
 # -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32.
 HIPCC_FLAGS = [

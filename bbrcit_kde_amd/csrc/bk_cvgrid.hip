@@ -23,8 +23,12 @@
 //     rescue, score reductions) is captured once per grid shape in a
 //     hipGraph and replayed; bandwidth constants live in a device table
 //     uploaded before each replay.
-// The register tiling (A x B) is chosen by timing the candidates on a probe
-// of the plan's own data on the first run of a grid shape.
+// The schedule (register tiling A x B, unroll of the reference loop) comes
+// from the autotuner cache (bk_tune.hip, which can be saved to a file and
+// reloaded) when it holds the run's (device, kind, grid shape, dataset
+// sizes), else from a default; bk_cv_grid_tune fills the cache by timing
+// every candidate on whole runs of the plan's own grid.  Every candidate sums
+// in the same order, so the choice changes no result bit.
 #include "../../include/bbrcit_hip.h"
 #include "bk_common.h"
 #include "bk_internal.h"
@@ -55,7 +59,6 @@ constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kRescueFraction = 9.094947017729282e-13;  // 2^-40
 constexpr int kScoreChunk = 4096;                            // points per score block
-constexpr int kProbeRefs = 8192;                             // references per probe item
 
 enum CvFamily : int { CV_GAUSS = 0, CV_EPAN = 1 };
 
@@ -88,15 +91,12 @@ struct CvArgs {
 };
 
 // ---- Gaussian product kernel: 8 or 4 x-factors, 8 y-factors per reference ----
-// Unroll of the Gaussian stage loop over references (measured at the 4-set
-// 8 x 8 workload: 2 -> 4 took the pipeline from 3.58 to 3.42 s; 8 was slower).
-#ifndef BK_CV_GAUSS_UNROLL
-#define BK_CV_GAUSS_UNROLL 4
-#endif
-template <int A, int B, bool ADAPT, bool DIAG>
+// U: unroll of the stage loop over references (a tuned parameter; it does
+// not change the order of any sum).
+template <int A, int B, bool ADAPT, bool DIAG, int U>
 __device__ __forceinline__ void gauss_stage(const float4 *__restrict__ tl, float qx, float qy, int self,
                                             const float (&kx)[A], const float (&ky)[B], float (&acc)[A][B]) {
-#pragma unroll BK_CV_GAUSS_UNROLL
+#pragma unroll U
   for (int k = 0; k < kCvTile; ++k) {
     const float4 r = tl[k];
     const float dx = qx - r.x, dy = qy - r.y;
@@ -126,11 +126,11 @@ __device__ __forceinline__ void gauss_stage(const float4 *__restrict__ tl, float
 // instruction costs gfx950 about 1.8 FP32 ones).  The convolution factor
 // (2-u)^3 (u^2 + 6u + 4) for u < 2 carries 32 times the reference's k(u);
 // the 1/1024 of both axes is applied in the score reduction.
-template <int A, int B, bool ADAPT, bool DIAG>
+template <int A, int B, bool ADAPT, bool DIAG, int U>
 __device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, double qx, double qy, int self,
                                            const double (&kx)[A], const double (&ky)[B], double (&accK)[A][B],
                                            double (&accT)[A][B]) {
-#pragma unroll 2
+#pragma unroll U
   for (int k = 0; k < kCvTile; ++k) {
     const double4 r = tl[k];
     double ax = fabs(qx - r.x), ay = fabs(qy - r.y);
@@ -170,7 +170,7 @@ __device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, doubl
 // Persistent pair kernels.  Every workgroup pulls items until the queue is
 // exhausted; all branches on the item are uniform (the item index is read
 // from LDS and made scalar).
-template <int A, int B, bool ADAPT>
+template <int A, int B, bool ADAPT, int U>
 __global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
   __shared__ float4 tile[2][kCvTile];
   __shared__ int s_item;
@@ -211,9 +211,9 @@ __global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
 #pragma unroll
         for (int b = 0; b < B; ++b) acc[a][b] = 0.0f;
       if (diag)
-        gauss_stage<A, B, ADAPT, true>(tile[buf], q.x, q.y, qi - r0, kx, ky, acc);
+        gauss_stage<A, B, ADAPT, true, U>(tile[buf], q.x, q.y, qi - r0, kx, ky, acc);
       else
-        gauss_stage<A, B, ADAPT, false>(tile[buf], q.x, q.y, 0, kx, ky, acc);
+        gauss_stage<A, B, ADAPT, false, U>(tile[buf], q.x, q.y, 0, kx, ky, acc);
 #pragma unroll
       for (int a = 0; a < A; ++a)
 #pragma unroll
@@ -265,7 +265,7 @@ __device__ __forceinline__ void box_read(const double (&box)[5][kCvBlock / 64], 
 // least 2 (the convolution kernel's support, wider than the kernel's) on one
 // axis.  The test repeats the pair term's own operations on smaller inputs,
 // so every skipped term is exactly zero as computed and no result changes.
-template <int A, int B, bool ADAPT>
+template <int A, int B, bool ADAPT, int U>
 __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
   __shared__ double4 tile[2][kCvTile];
   __shared__ double qbox[5][kCvBlock / 64];
@@ -345,9 +345,9 @@ __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
           atomicAdd(g.count, (unsigned long long)nr * (unsigned long long)w.q_count * (unsigned long long)(la * lb));
         }
         if (r0 < q_end && w.q_begin < rn)
-          epan_stage<A, B, ADAPT, true>(tile[buf], q.x, q.y, qi - r0, kx, ky, accK, accT);
+          epan_stage<A, B, ADAPT, true, U>(tile[buf], q.x, q.y, qi - r0, kx, ky, accK, accT);
         else
-          epan_stage<A, B, ADAPT, false>(tile[buf], q.x, q.y, 0, kx, ky, accK, accT);
+          epan_stage<A, B, ADAPT, false, U>(tile[buf], q.x, q.y, 0, kx, ky, accK, accT);
       }
       if (rn < w.r_end) {
         tile[buf ^ 1][t] = nxt;
@@ -556,18 +556,44 @@ struct Variant {
 constexpr Variant kGaussVariants[2] = {{8, 8}, {4, 8}};
 constexpr Variant kEpanVariants[2] = {{4, 8}, {4, 4}};
 
+// Unroll candidates of the stage loops (index u of a schedule).
+constexpr int kGaussUnrolls[3] = {2, 4, 8};
+constexpr int kEpanUnrolls[2] = {1, 2};
+constexpr int kGaussUnrollDefault = 1;  // 4
+constexpr int kEpanUnrollDefault = 1;   // 2
+
+int unroll_count(int fam) { return fam == CV_GAUSS ? 3 : 2; }
+
 using PairKernel = void (*)(CvArgs);
 
-PairKernel pair_kernel(int fam, int variant, bool adapt) {
-  if (fam == CV_GAUSS) {
-    if (variant == 0) return adapt ? cv_gauss_kernel<8, 8, true> : cv_gauss_kernel<8, 8, false>;
-    return adapt ? cv_gauss_kernel<4, 8, true> : cv_gauss_kernel<4, 8, false>;
+template <int A, int B>
+PairKernel gauss_kernel(bool adapt, int u) {
+  switch (u) {
+    case 0: return adapt ? cv_gauss_kernel<A, B, true, 2> : cv_gauss_kernel<A, B, false, 2>;
+    case 1: return adapt ? cv_gauss_kernel<A, B, true, 4> : cv_gauss_kernel<A, B, false, 4>;
+    default: return adapt ? cv_gauss_kernel<A, B, true, 8> : cv_gauss_kernel<A, B, false, 8>;
   }
-  if (variant == 0) return adapt ? cv_epan_kernel<4, 8, true> : cv_epan_kernel<4, 8, false>;
-  return adapt ? cv_epan_kernel<4, 4, true> : cv_epan_kernel<4, 4, false>;
+}
+
+template <int A, int B>
+PairKernel epan_kernel(bool adapt, int u) {
+  if (u == 0) return adapt ? cv_epan_kernel<A, B, true, 1> : cv_epan_kernel<A, B, false, 1>;
+  return adapt ? cv_epan_kernel<A, B, true, 2> : cv_epan_kernel<A, B, false, 2>;
+}
+
+PairKernel pair_kernel(int fam, int variant, bool adapt, int u) {
+  if (fam == CV_GAUSS) return variant == 0 ? gauss_kernel<8, 8>(adapt, u) : gauss_kernel<4, 8>(adapt, u);
+  return variant == 0 ? epan_kernel<4, 8>(adapt, u) : epan_kernel<4, 4>(adapt, u);
 }
 
 Variant variant_shape(int fam, int v) { return fam == CV_GAUSS ? kGaussVariants[v] : kEpanVariants[v]; }
+
+// A schedule of the pair kernel: register tiling and unroll index.  Every
+// schedule sums in the same order.
+struct Schedule {
+  int variant = -1, unroll = 0;
+  bool operator==(const Schedule &o) const { return variant == o.variant && unroll == o.unroll; }
+};
 
 }  // namespace
 
@@ -595,11 +621,13 @@ struct bk_cv_grid {
   // per grid shape
   int n_hx = 0, n_hy = 0;
   int variant = -1;          // option: -1 autotune
-  int used_variant = -1;
+  Schedule used;             // schedule of the last run
+  Schedule forced;           // set by bk_cv_grid_tune while it times a candidate
+  int tune_source = 0;       // BK_CVG_TUNE_SOURCE of the last run
   bool use_graph = true;
-  int shape_hx = -1, shape_hy = -1, shape_variant = -1;
+  int shape_hx = -1, shape_hy = -1;
+  Schedule shape_sched;      // schedule the items, buffers and graph were made for
   int items_n = 0, nsx = 0, nsy = 0, grid_blocks = 0, rescue_cap = 0;
-  std::vector<int> tuned;    // cache: (n_hx * 65 + n_hy) -> variant, stored as triples
   DevBuf items, queue, consts, P, list, part, scores;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -831,12 +859,12 @@ void enqueue_pipeline(bk_cv_grid *g, bool time_main) {
   const hipStream_t s = g->stream;
   const long long G = (long long)g->n_hx * g->n_hy;
   const int planes = g->fam == CV_GAUSS ? 1 : 2;
-  const Variant v = variant_shape(g->fam, g->used_variant);
+  const Variant v = variant_shape(g->fam, g->used.variant);
   const CvArgs args = make_args(g, g->items.as<CvItem>(), g->items_n, g->count_units && g->fam == CV_EPAN);
   hipLaunchKernelGGL(cv_reset_kernel, dim3(1), dim3(64), 0, s, g->queue.as<int>());
   hip_check(hipGetLastError(), "queue reset launch");
   if (time_main) hip_check(hipEventRecord(g->ev[2], s), "hipEventRecord");
-  const PairKernel k = pair_kernel(g->fam, g->used_variant, g->adapt);
+  const PairKernel k = pair_kernel(g->fam, g->used.variant, g->adapt, g->used.unroll);
   hipLaunchKernelGGL(k, dim3(g->grid_blocks), dim3(kCvBlock), 0, s, args);
   hip_check(hipGetLastError(), "pair kernel launch");
   if (time_main) hip_check(hipEventRecord(g->ev[3], s), "hipEventRecord");
@@ -891,72 +919,71 @@ void enqueue_pipeline(bk_cv_grid *g, bool time_main) {
   }
 }
 
-// Time one tiling on a probe: the longest items with their reference ranges
-// cut to kProbeRefs, enough of them for two rounds of resident workgroups.
-double probe_variant(bk_cv_grid *g, int v) {
-  const Variant sh = variant_shape(g->fam, v);
-  const int nsx = (g->n_hx + sh.A - 1) / sh.A, nsy = (g->n_hy + sh.B - 1) / sh.B;
-  std::vector<CvItem> all = make_items(g, sh.A, sh.B, nsx, nsy);
-  const PairKernel k = pair_kernel(g->fam, v, g->adapt);
-  const int blocks = resident_blocks(k);
-  std::vector<CvItem> probe;
-  double useful = 0.0;
-  for (size_t i = 0; i < all.size() && (int)probe.size() < 2 * blocks; ++i) {
-    CvItem it = all[i];
-    it.r_end = std::min(it.r_end, it.r_begin + kProbeRefs);
-    const int ux = std::min(sh.A, g->n_hx - it.sub_x), uy = std::min(sh.B, g->n_hy - it.sub_y);
-    useful += (double)it.q_count * (it.r_end - it.r_begin) * ux * uy;
-    probe.push_back(it);
+// Workgroups a schedule launches: the kernel's occupancy limit on every CU.
+// (Half of it was a candidate once: slower on every full run at the 4-set
+// 8 x 8 workload, by 7% to 36%, so it is not searched.)
+int schedule_blocks(const bk_cv_grid *g, const Schedule &sc) {
+  return resident_blocks(pair_kernel(g->fam, sc.variant, g->adapt, sc.unroll));
+}
+
+// Autotuner key of a plan's current run: device, kind, bandwidth model, grid
+// shape and dataset sizes.
+std::string tune_key(const bk_cv_grid *g) {
+  std::string k = "cvgrid/" + bk::device_tune_tag() + "/kind" + std::to_string(g->kind) +
+                  (g->adapt ? "/adaptive" : "/fixed") + "/grid" + std::to_string(g->n_hx) + "x" +
+                  std::to_string(g->n_hy) + "/sets";
+  for (int d = 0; d < g->n_sets; ++d) k += (d ? "," : "") + std::to_string(g->begin[d + 1] - g->begin[d]);
+  return k;
+}
+
+bool valid_schedule(const bk_cv_grid *g, const std::vector<int> &v) {
+  return v.size() == 2 && v[0] >= 0 && v[0] < BK_CVG_VARIANTS && v[1] >= 0 && v[1] < unroll_count(g->fam);
+}
+
+// Default schedule of a kind (the fastest at the 4-set 8 x 8 workload).
+Schedule default_schedule(int fam) {
+  Schedule sc;
+  sc.variant = fam == CV_GAUSS ? 0 : 1;
+  sc.unroll = fam == CV_GAUSS ? kGaussUnrollDefault : kEpanUnrollDefault;
+  return sc;
+}
+
+// The schedule of this run: forced (BK_CVG_VARIANT with the default unroll,
+// or a candidate under bk_cv_grid_tune), else the autotuner cache's entry for
+// the run, else the default.
+Schedule choose_schedule(bk_cv_grid *g) {
+  if (g->forced.variant >= 0) {
+    g->tune_source = 0;
+    return g->forced;
   }
-  DevBuf dev;
-  dev.reserve(probe.size() * sizeof(CvItem));
-  hip_check(hipMemcpyAsync(dev.ptr, probe.data(), probe.size() * sizeof(CvItem), hipMemcpyHostToDevice, g->stream),
-            "upload probe items");
-  const CvArgs args = make_args(g, dev.as<CvItem>(), (int)probe.size());
-  float best = std::numeric_limits<float>::max();
-  for (int rep = 0; rep < 3; ++rep) {
-    hipLaunchKernelGGL(cv_reset_kernel, dim3(1), dim3(64), 0, g->stream, g->queue.as<int>());
-    hip_check(hipEventRecord(g->ev[0], g->stream), "hipEventRecord");
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(kCvBlock), 0, g->stream, args);
-    hip_check(hipGetLastError(), "probe launch");
-    hip_check(hipEventRecord(g->ev[1], g->stream), "hipEventRecord");
-    hip_check(hipEventSynchronize(g->ev[1]), "hipEventSynchronize");
-    float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]), "hipEventElapsedTime");
-    if (rep > 0) best = std::min(best, ms);
+  if (g->variant >= 0) {
+    g->tune_source = 0;
+    Schedule sc = default_schedule(g->fam);
+    sc.variant = g->variant;
+    return sc;
   }
-  hip_check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
-  dev.release();
-  return (double)best / useful;  // ms per useful pair-bandwidth evaluation
+  std::vector<int> v;
+  if (bk::tune_lookup(tune_key(g), v) && valid_schedule(g, v)) {
+    g->tune_source = 2;
+    Schedule sc;
+    sc.variant = v[0];
+    sc.unroll = v[1];
+    return sc;
+  }
+  g->tune_source = 1;
+  return default_schedule(g->fam);
 }
 
 // Prepare items, buffers and (optionally) the graph for the current shape.
 void prepare_shape(bk_cv_grid *g) {
-  int v = g->variant;
-  if (v < 0) {
-    const int key = g->n_hx * 65 + g->n_hy;
-    for (size_t i = 0; i + 1 < g->tuned.size(); i += 2)
-      if (g->tuned[i] == key) v = g->tuned[i + 1];
-    if (v < 0) {
-      double best = std::numeric_limits<double>::max();
-      for (int c = 0; c < BK_CVG_VARIANTS; ++c) {
-        const double t = probe_variant(g, c);
-        if (t < best) {
-          best = t;
-          v = c;
-        }
-      }
-      g->tuned.push_back(key);
-      g->tuned.push_back(v);
-    }
-  }
-  if (g->shape_hx == g->n_hx && g->shape_hy == g->n_hy && g->shape_variant == v && g->items_n > 0) {
-    g->used_variant = v;
+  const Schedule sc = choose_schedule(g);
+  if (g->shape_hx == g->n_hx && g->shape_hy == g->n_hy && g->shape_sched == sc && g->items_n > 0) {
+    g->used = sc;
     return;
   }
   drop_graph(g);
-  g->used_variant = v;
-  const Variant sh = variant_shape(g->fam, v);
+  g->used = sc;
+  const Variant sh = variant_shape(g->fam, sc.variant);
   g->nsx = (g->n_hx + sh.A - 1) / sh.A;
   g->nsy = (g->n_hy + sh.B - 1) / sh.B;
   std::vector<CvItem> items = make_items(g, sh.A, sh.B, g->nsx, g->nsy);
@@ -967,14 +994,14 @@ void prepare_shape(bk_cv_grid *g) {
             "upload work items");
   hip_check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
   g->items_n = (int)items.size();
-  g->grid_blocks = resident_blocks(pair_kernel(g->fam, v, g->adapt));
+  g->grid_blocks = schedule_blocks(g, sc);
   const long long cap = (long long)g->n_points * g->nsx * g->nsy;
   if (cap > std::numeric_limits<int>::max()) fail(BK_ERR_INVALID_ARG, "grid too large for the rescue list");
   g->rescue_cap = g->fam == CV_GAUSS ? (int)cap : 0;
   if (g->fam == CV_GAUSS) g->list.reserve((size_t)cap * sizeof(int2));
   g->shape_hx = g->n_hx;
   g->shape_hy = g->n_hy;
-  g->shape_variant = v;
+  g->shape_sched = sc;
 }
 
 void run_plan(bk_cv_grid *g, const double *hx, const double *hy) {
@@ -1208,6 +1235,49 @@ int bk_cv_grid_run(bk_cv_grid *g, size_t n_hx, const double *hx, size_t n_hy, co
   });
 }
 
+int bk_cv_grid_tune(bk_cv_grid *g, size_t n_hx, const double *hx, size_t n_hy, const double *hy, double *ms_out,
+                    int *n_candidates_out) {
+  return guarded([&] {
+    require_plan(g);
+    if (g->variant >= 0) fail(BK_ERR_STATE, "BK_CVG_VARIANT forces a tiling; set it to -1 to tune");
+    std::vector<Schedule> cands;
+    for (int t = 0; t < BK_CVG_VARIANTS; ++t)
+      for (int u = 0; u < unroll_count(g->fam); ++u) {
+        Schedule sc;
+        sc.variant = t;
+        sc.unroll = u;
+        cands.push_back(sc);
+      }
+    if (n_candidates_out) *n_candidates_out = (int)cands.size();
+    const bool graph = g->use_graph, counting = g->count_units;
+    g->use_graph = false;  // eager runs: the pair kernel is timed alone
+    g->count_units = false;
+    Schedule best;
+    double best_ms = std::numeric_limits<double>::max();
+    try {
+      for (size_t i = 0; i < cands.size(); ++i) {
+        g->forced = cands[i];
+        const int st = bk_cv_grid_run(g, n_hx, hx, n_hy, hy, nullptr, nullptr);
+        if (st != BK_OK) fail(st, bk_last_error());
+        if (ms_out) ms_out[i] = g->last_main_ms;
+        if (g->last_main_ms < best_ms) {
+          best_ms = g->last_main_ms;
+          best = cands[i];
+        }
+      }
+    } catch (...) {
+      g->forced = Schedule();
+      g->use_graph = graph;
+      g->count_units = counting;
+      throw;
+    }
+    g->forced = Schedule();
+    g->use_graph = graph;
+    g->count_units = counting;
+    bk::tune_record(tune_key(g), {best.variant, best.unroll});
+  });
+}
+
 int bk_cv_grid_loo(bk_cv_grid *g, size_t set, size_t ix, size_t iy, double *out) {
   return guarded([&] {
     require_plan(g);
@@ -1244,6 +1314,10 @@ int bk_cv_grid_set_option(bk_cv_grid *g, int option, int value) {
         if (value != 0 && value != 1) fail(BK_ERR_INVALID_ARG, "BK_CVG_COUNT_UNITS takes 0 or 1");
         g->count_units = value != 0;
         break;
+      case BK_CVG_TUNE_SOURCE:
+      case BK_CVG_UNROLL:
+      case BK_CVG_WORKGROUPS:
+        fail(BK_ERR_INVALID_ARG, "option " + std::to_string(option) + " is read-only");
       default:
         fail(BK_ERR_INVALID_ARG, "unknown option " + std::to_string(option));
     }
@@ -1256,7 +1330,19 @@ int bk_cv_grid_get_option(const bk_cv_grid *g, int option, int *value) {
     if (!value) fail(BK_ERR_INVALID_ARG, "null output pointer");
     switch (option) {
       case BK_CVG_VARIANT:
-        *value = g->used_variant >= 0 ? g->used_variant : g->variant;
+        *value = g->used.variant >= 0 ? g->used.variant : g->variant;
+        break;
+      case BK_CVG_TUNE_SOURCE:
+        if (!g->have_run) fail(BK_ERR_STATE, "no completed run");
+        *value = g->tune_source;
+        break;
+      case BK_CVG_UNROLL:
+        if (!g->have_run) fail(BK_ERR_STATE, "no completed run");
+        *value = g->fam == CV_GAUSS ? kGaussUnrolls[g->used.unroll] : kEpanUnrolls[g->used.unroll];
+        break;
+      case BK_CVG_WORKGROUPS:
+        if (!g->have_run) fail(BK_ERR_STATE, "no completed run");
+        *value = g->grid_blocks;
         break;
       case BK_CVG_GRAPH:
         *value = g->use_graph ? 1 : 0;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace bk {
 
@@ -274,5 +276,15 @@ hipError_t launch_scatter_f64(const long long *list, long long n, const double *
 int device_cu_count();
 // Workgroups of the FP32 Gaussian range kernel resident per CU.
 int gauss_resident_blocks();
+
+// Autotuner cache (bk_tune.hip).  kTuneVersion names the meaning of the
+// tuned parameters; a cache file of another version is ignored.
+constexpr int kTuneVersion = 1;
+// "<architecture>/<compute units>cu" of the current device
+std::string device_tune_tag();
+// the configuration recorded for key, if any
+bool tune_lookup(const std::string &key, std::vector<int> &value);
+// record a configuration (and save the cache file when one is named)
+void tune_record(const std::string &key, const std::vector<int> &value);
 
 }  // namespace bk

@@ -132,6 +132,18 @@ class CvGrid {
 
   void set_option(int option, int value) { hip_detail::check(bk_cv_grid_set_option(plan_, option, value)); }
 
+  // Time every schedule of the pair kernel on whole runs of the blocks of
+  // at most 64 x 64 bandwidths that run() evaluates for hx x hy, and record
+  // the fastest per block shape in the autotuner cache (bk_cv_grid_tune).
+  void tune(const std::vector<double> &hx, const std::vector<double> &hy) {
+    if (hx.empty() || hy.empty()) throw std::invalid_argument("CvGrid::tune: empty bandwidth list");
+    for (size_t x0 = 0; x0 < hx.size(); x0 += kMaxAxis)
+      for (size_t y0 = 0; y0 < hy.size(); y0 += kMaxAxis) {
+        const size_t nx = std::min<size_t>(kMaxAxis, hx.size() - x0), ny = std::min<size_t>(kMaxAxis, hy.size() - y0);
+        hip_detail::check(bk_cv_grid_tune(plan_, nx, hx.data() + x0, ny, hy.data() + y0, nullptr, nullptr));
+      }
+  }
+
  private:
   bk_ctx *ctx_ = nullptr;
   bk_cv_grid *plan_ = nullptr;

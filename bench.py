@@ -80,6 +80,8 @@ def parse_args():
                     help="skip the timing of the same evaluations in device-sorted query order")
     ap.add_argument("--gauss-form", choices=["auto", "direct"], default="auto",
                     help="Gaussian exponent form (auto: tile-centred on compact tiles)")
+    ap.add_argument("--no-cv-tune", action="store_true",
+                    help="skip the CV grid autotuner (runs use the cached or default schedule)")
     return ap.parse_args()
 
 
@@ -234,19 +236,39 @@ def cv_instr_per_ref(A: int, B: int) -> tuple[int, int]:
     return 4 + 2 * A + 2 * B + A * B, A + B
 
 
-def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
+def cv_schedule(bk, cv) -> dict:
+    """The pair kernel's schedule of a plan's last run and where it came from."""
+    src = cv.get_option(bk.BK_CVG_TUNE_SOURCE)
+    return {"tiling_variant": cv.get_option(bk.BK_CVG_VARIANT), "unroll": cv.get_option(bk.BK_CVG_UNROLL),
+            "workgroups": cv.get_option(bk.BK_CVG_WORKGROUPS),
+            "chosen_by": {0: "forced", 1: "default", 2: "autotuner cache"}[src]}
+
+
+def tune_table(timings):
+    """bk_cv_grid_tune's candidates as JSON: pair-kernel ms per (tiling, unroll index)."""
+    if timings is None:
+        return None
+    return {"pair_kernel_ms": [{"tiling_variant": t, "unroll_index": u, "ms": ms} for t, u, ms in timings],
+            "note": "every schedule timed on one eager whole run of the grid; the fastest is recorded in the "
+                    "autotuner cache and used by the runs below"}
+
+
+def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float, tune: bool = True):
     from bbrcit_kde_amd import datasets as ds
 
     sets = cv_datasets()
     units = float(sum(n * n for n in CV_SIZES)) * len(CV_GRID) ** 2
     res = {}
     with bk.CvGrid(bk.BK_GAUSSIAN_PRODUCT_2D, [p for p, _ in sets], [w for _, w in sets]) as cv:
+        # the autotuner: every schedule timed on a whole run, the fastest recorded
+        tuned = cv.tune(CV_GRID, CV_GRID) if tune else None
         # eager pass: the persistent pair kernel timed alone with HIP events on
-        # the plan's stream (it is also the autotuning run)
+        # the plan's stream
         cv.set_option(bk.BK_CVG_GRAPH, 0)
         lcv, lscv = cv.run(CV_GRID, CV_GRID)
         main_ms, _, rescued = cv.last_timing()
         variant = cv.get_option(bk.BK_CVG_VARIANT)
+        schedule = cv_schedule(bk, cv)
         cv.set_option(bk.BK_CVG_GRAPH, 1)
         for _ in range(warmup):
             cv.run(CV_GRID, CV_GRID)
@@ -266,7 +288,7 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
                     "hx, hy = 0.01*50^(k/7); GaussianProductKernel2d; LCV + convolution LSCV per case",
         "unit": "pair-bw-evals/s", "value": units / (avg * 1e-3), "ms_per_step": avg, "steps": steps,
         "units_per_step": units, "main_kernel_ms": main_ms, "rescued_entries": int(rescued),
-        "tiling": f"{A}x{B}", "graph": True,
+        "tiling": f"{A}x{B}", "schedule": schedule, "autotune": tune_table(tuned), "graph": True,
         "roofline": {"bound": "valu", "pipe": "FP32 VALU + v_exp_f32 issue",
                      "achieved": rate_main * instr, "peak": PEAK_VALU_LANE_INSTR, "unit": "VALU lane-instr/s",
                      "frac": rate_main * instr / PEAK_VALU_LANE_INSTR, "instr_per_pair_bw": instr,
@@ -285,9 +307,11 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
     # kernel (FP64 arithmetic; stages out of every bandwidth's support are
     # skipped, their terms being exactly zero)
     with bk.CvGrid(bk.BK_EPANECHNIKOV_PRODUCT_2D, [p for p, _ in sets], [w for _, w in sets]) as cv:
+        e_tuned = cv.tune(CV_GRID, CV_GRID) if tune else None
         cv.set_option(bk.BK_CVG_GRAPH, 0)
         elcv, elscv = cv.run(CV_GRID, CV_GRID)
         e_main_ms = cv.last_timing()[0]
+        e_schedule = cv_schedule(bk, cv)
         cv.set_option(bk.BK_CVG_GRAPH, 1)
         cv.run(CV_GRID, CV_GRID)
         e_ms = []
@@ -307,7 +331,7 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float):
         "value_kind": "effective: every (ordered pair, bandwidth pair) unit counted",
         "evaluated_units": e_units, "evaluated_frac": e_units / units,
         "evaluated_units_per_s": e_units / (e_avg * 1e-3),
-        "tiling_variant": int(e_variant),
+        "tiling_variant": int(e_variant), "schedule": e_schedule, "autotune": tune_table(e_tuned),
         "note": "the pair kernel skips the 256-reference stages out of the support of every bandwidth of its "
                 "sub-grid (exactly zero terms); evaluated_units counts the units of the stages it runs",
         "best_lscv": [list(map(int, np.unravel_index(np.argmin(elscv[d]), elscv[d].shape)))
@@ -831,7 +855,8 @@ def main() -> int:
     cv_res = None
     if "cvgrid" in kinds:
         log(f"timing cvgrid: warmup {args.warmup}, steps {min(args.steps, 2)}")
-        cv_res = bench_cv_grid(bk, min(args.steps, 2), args.warmup, not args.no_cpu_baseline, args.cpu_seconds)
+        cv_res = bench_cv_grid(bk, min(args.steps, 2), args.warmup, not args.no_cpu_baseline, args.cpu_seconds,
+                               tune=not args.no_cv_tune)
         log(f"cvgrid: {cv_res['ms_per_step']:.1f} ms/step, {cv_res['value']:.4e} pair-bw-evals/s")
 
     sur_res = None

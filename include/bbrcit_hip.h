@@ -370,6 +370,12 @@ typedef struct bk_cv_grid bk_cv_grid;
                                 bandwidth pair) units the pair kernel
                                 evaluates (runs launch eagerly while set;
                                 read with bk_cv_grid_unit_count) */
+/* Read-only options describing the last run's schedule: */
+#define BK_CVG_TUNE_SOURCE 4 /* 0: forced by BK_CVG_VARIANT; 1: the default
+                                schedule (no cache entry for the run); 2:
+                                taken from the autotuner cache */
+#define BK_CVG_UNROLL      5 /* unroll of the pair kernel's reference loop */
+#define BK_CVG_WORKGROUPS  6 /* workgroups the persistent pair kernel ran */
 
 /* Create a plan over n_sets datasets of 2-D points stored back to back:
  * set d holds sizes[d] >= 1 points; coords is row-major (x, y) over all
@@ -416,6 +422,32 @@ int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
  * counterpart in the reference, whose cross-validation evaluates every pair
  * a tree traversal does not prune. */
 int bk_cv_grid_unit_count(const bk_cv_grid *plan, unsigned long long *evaluated);
+
+/* ---- autotuner cache ----------------------------------------------------- */
+/* With BK_CVG_VARIANT at -1 (the default) a plan runs the schedule of its
+ * pair kernel (register tiling x unroll of the reference loop; all sum in
+ * the same order, so results do not depend on the choice) that the
+ * autotuner cache holds for the run's (device, kind, bandwidth model, grid
+ * shape, dataset sizes), else a default.  bk_cv_grid_tune fills the cache:
+ * it runs the plan's whole grid once per candidate schedule (eagerly, the
+ * pair kernel timed with HIP events), writes each candidate's pair-kernel
+ * milliseconds to ms_out (BK_CVG_VARIANTS x unroll candidates, tiling-major;
+ * the count goes to n_candidates_out; either may be NULL; at most 8) and
+ * records the fastest.  It fails when BK_CVG_VARIANT forces a tiling.
+ * bk_tune_set_cache names a file
+ * that the cache is loaded from now and saved to after every new choice
+ * (path NULL: memory only, and the entries in memory are dropped); without
+ * a call the file named by the environment variable BBRCIT_HIP_TUNE_CACHE,
+ * if any, is used.  The file is text with a versioned header; a file of
+ * another version is ignored and replaced on the next save.  Saves go
+ * through a temporary file renamed over the cache.  No counterpart in the
+ * reference, which has no autotuner. */
+int bk_tune_set_cache(const char *path);
+/* Number of entries in the in-memory cache. */
+int bk_tune_entry_count(size_t *count_out);
+int bk_cv_grid_tune(bk_cv_grid *plan, size_t n_hx, const double *hx,
+                    size_t n_hy, const double *hy, double *ms_out,
+                    int *n_candidates_out);
 
 /* ---- sampling ------------------------------------------------------------ */
 /* The device part of KernelDensity::simulate (include/KernelDensityImpl.h:

@@ -101,3 +101,22 @@ def test_kd_order_is_a_compact_permutation():
     rnd = pts[: (len(pts) // 256) * 256].reshape(-1, 256, 2)
     spread_rnd = np.median(rnd.max(1) - rnd.min(1))
     assert spread_kd < 0.25 * spread_rnd
+
+
+def test_tune_cache_file_is_versioned(bk, tmp_path):
+    """bk_tune_set_cache loads entries from a file of this version only (no GPU call)."""
+    good = tmp_path / "tune.txt"
+    good.write_text("bbrcit-hip-tune 1 1\ncvgrid/gfx950/256cu/kind4/fixed/grid8x8/sets100\t0 1\n"
+                    "cvgrid/gfx950/256cu/kind3/fixed/grid4x4/sets100,200\t1 0\nmalformed line without tab\n")
+    try:
+        bk.tune_set_cache(good)
+        assert bk.tune_entry_count() == 2
+        old = tmp_path / "old.txt"
+        old.write_text("bbrcit-hip-tune 1 0\ncvgrid/x\t0 0 0\n")
+        bk.tune_set_cache(old)
+        assert bk.tune_entry_count() == 0
+        bk.tune_set_cache(tmp_path / "missing.txt")
+        assert bk.tune_entry_count() == 0
+    finally:
+        bk.tune_set_cache(None)
+    assert bk.tune_entry_count() == 0

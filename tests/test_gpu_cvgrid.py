@@ -247,3 +247,61 @@ def test_unit_count(bk):
             assert units == total
         else:
             assert inside <= units < total
+
+
+@pytest.mark.parametrize("kind", [GAUSSPROD, EPANPROD])
+def test_autotuner_cache(bk, tmp_path, kind):
+    """Without a cache entry a run uses the default schedule; bk_cv_grid_tune
+    times every schedule on whole runs and records the fastest, which later
+    runs and plans of the same shape take from the cache in memory or from
+    the saved file; a file of another version is ignored and rewritten.
+    Every schedule gives the same bits."""
+    sets = [mixture(2500, 631), mixture(700, 632)]
+    hx = np.geomspace(0.03, 0.4, 5)
+    hy = np.geomspace(0.02, 0.5, 7)
+    path = tmp_path / "tune.txt"
+    path.write_text("bbrcit-hip-tune 1 0\nstale\t0 0\n")
+    try:
+        bk.tune_set_cache(path)
+        assert bk.tune_entry_count() == 0
+        runs = []
+        with bk.CvGrid(kind, [p for p, _ in sets], [w for _, w in sets]) as cv:
+            runs.append(cv.run(hx, hy))
+            assert cv.get_option(bk.BK_CVG_TUNE_SOURCE) == 1
+            timings = cv.tune(hx, hy)
+            assert len(timings) == bk.BK_CVG_VARIANTS * (3 if kind == GAUSSPROD else 2)
+            assert all(ms > 0 for _, _, ms in timings)
+            best = min(timings, key=lambda t: t[2])
+            runs.append(cv.run(hx, hy))
+            assert cv.get_option(bk.BK_CVG_TUNE_SOURCE) == 2
+            assert cv.get_option(bk.BK_CVG_VARIANT) == best[0]
+            sched = (cv.get_option(bk.BK_CVG_VARIANT), cv.get_option(bk.BK_CVG_UNROLL))
+            cv.set_option(bk.BK_CVG_VARIANT, 0)
+            with pytest.raises(bk.BkError):
+                cv.tune(hx, hy)
+            cv.set_option(bk.BK_CVG_VARIANT, -1)
+        lines = path.read_text().splitlines()
+        assert lines[0] == "bbrcit-hip-tune 1 1" and len(lines) == 2
+        key, vals = lines[1].split("\t")
+        assert key.startswith("cvgrid/") and f"/kind{kind}/" in key and key.endswith("/grid5x7/sets2500,700")
+        assert [int(v) for v in vals.split()] == [best[0], best[1]]
+        print(f"[tune] kind {kind}: {key} -> {vals}; timings {timings}")
+        # a new plan of the same shape, the cache reloaded from the file only
+        bk.tune_set_cache(path)
+        assert bk.tune_entry_count() == 1
+        with bk.CvGrid(kind, [p for p, _ in sets], [w for _, w in sets]) as cv:
+            runs.append(cv.run(hx, hy))
+            assert cv.get_option(bk.BK_CVG_TUNE_SOURCE) == 2
+            assert (cv.get_option(bk.BK_CVG_VARIANT), cv.get_option(bk.BK_CVG_UNROLL)) == sched
+            # forced tilings with their default unroll: same bits
+            for v in range(bk.BK_CVG_VARIANTS):
+                cv.set_option(bk.BK_CVG_VARIANT, v)
+                runs.append(cv.run(hx, hy))
+                assert cv.get_option(bk.BK_CVG_TUNE_SOURCE) == 0
+            with pytest.raises(ValueError):
+                cv.set_option(bk.BK_CVG_TUNE_SOURCE, 1)
+        for lcv, lscv in runs[1:]:
+            np.testing.assert_array_equal(lcv, runs[0][0])
+            np.testing.assert_array_equal(lscv, runs[0][1])
+    finally:
+        bk.tune_set_cache(None)
