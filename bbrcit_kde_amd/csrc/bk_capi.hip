@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -222,9 +223,9 @@ const PointSet &query_set(const bk_ctx *ctx) { return ctx->alias ? ctx->refs : c
 // body(lo, hi) over disjoint slices of [0, n), on up to 16 host threads for
 // large n; body must not throw.
 template <class F>
-void host_parallel(size_t n, F &&body) {
+void host_parallel(size_t n, F &&body, size_t min_n = size_t(1) << 18) {
   const unsigned hw = std::thread::hardware_concurrency();
-  const size_t t = n < (size_t(1) << 18) || hw < 2 ? 1 : std::min<size_t>(hw, 16);
+  const size_t t = n < min_n || hw < 2 ? 1 : std::min<size_t>({(size_t)hw, (size_t)16, n});
   if (t == 1) {
     body(size_t(0), n);
     return;
@@ -250,29 +251,50 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
   if (n == 0) return;
   const hipStream_t s = ctx->stream;
   if (ctx->precision == BK_PREC_F32) {
-    std::vector<float> m(n), a(n);
-    host_parallel(n, [&](size_t lo, size_t hi) {
-      for (size_t i = lo; i < hi; ++i) {
-        m[i] = mass ? (float)mass[i] : 1.0f;
-        a[i] = abw ? (float)abw[i] : 1.0f;
+    // host buffers left uninitialised (the threads write every element, so
+    // their pages are first touched in parallel)
+    std::unique_ptr<float[]> m(new float[n]), a(new float[n]);
+    // conversion and validation on threads; the mass sum over fixed chunks
+    // added in chunk order, so it does not depend on the thread count
+    constexpr size_t kChunk = size_t(1) << 16;
+    const size_t n_chunks = (n + kChunk - 1) / kChunk;
+    std::vector<double> chunk_sum(n_chunks, 0.0), chunk_max(n_chunks, 0.0);
+    std::vector<char> chunk_uniform(n_chunks, 1), chunk_bad_abw(n_chunks, 0), chunk_bad_m(n_chunks, 0);
+    host_parallel(n_chunks, [&](size_t c0, size_t c1) {
+      for (size_t c = c0; c < c1; ++c) {
+        double sum = 0.0, mx = 0.0;
+        char uniform = 1, bad_a = 0, bad_m = 0;
+        for (size_t i = c * kChunk, e = std::min(n, (c + 1) * kChunk); i < e; ++i) {
+          const float mi = mass ? (float)mass[i] : 1.0f;
+          const float ai = abw ? (float)abw[i] : 1.0f;
+          m[i] = mi;
+          a[i] = ai;
+          uniform &= ai == 1.0f;
+          bad_a |= !(ai > 0.0f) || !std::isfinite(ai);
+          bad_m |= !(mi >= 0.0f) || !std::isfinite(mi);
+          mx = std::max(mx, (double)mi);
+          sum += (double)mi;
+        }
+        chunk_sum[c] = sum;
+        chunk_max[c] = mx;
+        chunk_uniform[c] = uniform;
+        chunk_bad_abw[c] = bad_a;
+        chunk_bad_m[c] = bad_m;
       }
-    });
-    // validation and the mass sum in index order (deterministic)
-    for (size_t i = 0; i < n; ++i) {
-      if (a[i] != 1.0f) ps.uniform_abw = false;
-      if (!(a[i] > 0.0f) || !std::isfinite(a[i]))
-        fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
-      if (!(m[i] >= 0.0f) || !std::isfinite(m[i]))
-        fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
-      ps.m_max = std::max(ps.m_max, (double)m[i]);
-      ps.m_sum += (double)m[i];
+    }, size_t(4));
+    for (size_t c = 0; c < n_chunks; ++c) {
+      if (chunk_bad_abw[c]) fail(BK_ERR_INVALID_ARG, "abw must be positive and finite");
+      if (chunk_bad_m[c]) fail(BK_ERR_INVALID_ARG, "masses must be non-negative and finite");
+      if (!chunk_uniform[c]) ps.uniform_abw = false;
+      ps.m_max = std::max(ps.m_max, chunk_max[c]);
+      ps.m_sum += chunk_sum[c];
     }
     ps.m.reserve(n * sizeof(float));
     ps.abw.reserve(n * sizeof(float));
-    hip_check(hipMemcpyAsync(ps.m.ptr, m.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload masses");
-    hip_check(hipMemcpyAsync(ps.abw.ptr, a.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload abw");
+    hip_check(hipMemcpyAsync(ps.m.ptr, m.get(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload masses");
+    hip_check(hipMemcpyAsync(ps.abw.ptr, a.get(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload abw");
     if (dim <= 2) {
-      std::vector<float> x(n), y(n);
+      std::unique_ptr<float[]> x(new float[n]), y(new float[n]);
       std::mutex part_mu;
       float lo = (float)coords[0], hi = lo;
       host_parallel(n, [&](size_t b, size_t e) {
@@ -291,8 +313,8 @@ void upload_set(bk_ctx *ctx, PointSet &ps, int dim, size_t n, const double *coor
       ps.box_hi = std::isfinite(hi) && hi > ps.box_lo ? hi : ps.box_lo + 1.0f;
       ps.x.reserve(n * sizeof(float));
       ps.y.reserve(n * sizeof(float));
-      hip_check(hipMemcpyAsync(ps.x.ptr, x.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload x");
-      hip_check(hipMemcpyAsync(ps.y.ptr, y.data(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload y");
+      hip_check(hipMemcpyAsync(ps.x.ptr, x.get(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload x");
+      hip_check(hipMemcpyAsync(ps.y.ptr, y.get(), n * sizeof(float), hipMemcpyHostToDevice, s), "upload y");
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     } else {
       std::vector<float> c(n * dim);

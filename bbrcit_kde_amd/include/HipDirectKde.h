@@ -27,6 +27,7 @@
 #include <utility>
 #include <vector>
 
+#include "HostVector.h"
 #include "Kernels/EpanechnikovKernel.h"
 
 namespace bbrcit {
@@ -41,11 +42,13 @@ inline void check(int status) {
   throw std::runtime_error("bbrcit_hip: " + msg);
 }
 
+// A host buffer of doubles that resize() does not zero-fill.
+using HostBuffer = HostVector<double>;
+
 // Host points -> row-major coordinates, masses, abw (as double); large sets
 // on several host threads (disjoint slices, so the result is the same).
-template <int D, typename HostPointT>
-void flatten(const std::vector<HostPointT> &pts, std::vector<double> &xy, std::vector<double> &m,
-             std::vector<double> &a) {
+template <int D, typename HostPointT, typename Vec>
+void flatten(const std::vector<HostPointT> &pts, Vec &xy, Vec &m, Vec &a) {
   const size_t n = pts.size();
   xy.resize(n * D);
   m.resize(n);
@@ -175,8 +178,8 @@ class HipDirectKde {
   // Bounds of the unnormalised sums over every reference for queries
   // q_i..q_j: upper - lower <= max(abs_err, rel_err * lower)
   // (bk_eval_bounded).
-  void unnormalized_eval_bounded(size_t q_i, size_t q_j, double rel_err, double abs_err, std::vector<double> &lower,
-                                 std::vector<double> &upper) {
+  template <typename Vec>
+  void unnormalized_eval_bounded(size_t q_i, size_t q_j, double rel_err, double abs_err, Vec &lower, Vec &upper) {
     require_ctx();
     if (q_j >= query_size() || reference_size() == 0)
       throw std::out_of_range("HipDirectKde<>: unnormalized_eval_bounded(): j_q should be less than the number of "
@@ -196,7 +199,7 @@ class HipDirectKde {
 
   // host copies kept only to implement copy construction
   struct HostSet {
-    std::vector<double> xy, m, a;
+    hip_detail::HostBuffer xy, m, a;
   };
   struct HostSets {
     HostSet ref, query;

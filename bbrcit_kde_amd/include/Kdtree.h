@@ -39,6 +39,7 @@
 #include "Attributes/PointWeights.h"
 #include "DecoratedPoint.h"
 #include "FloatUtils.h"
+#include "HostVector.h"
 #include "Rectangle.h"
 
 namespace bbrcit {
@@ -300,6 +301,19 @@ class Kdtree {
            points_.size() <= 0x7fffffffULL && device_sort_available();
   }
 
+  // body(c, a, b) for the slices c of [0, n), one per build_threads() thread
+  // (an empty slice when n is smaller)
+  template <typename F>
+  static void on_threads_indexed(size_t n, F &&body) {
+    const int t = build_threads();
+    std::vector<std::future<void>> jobs;
+    for (int c = 0; c < t; ++c) {
+      const size_t a = n * (size_t)c / (size_t)t, b = n * (size_t)(c + 1) / (size_t)t;
+      jobs.push_back(std::async(std::launch::async, [&body, c, a, b] { body(c, a, b); }));
+    }
+    for (auto &j : jobs) j.get();
+  }
+
   // body(a, b) over [0, n) on up to build_threads() host threads
   template <typename F>
   static void on_threads(size_t n, F &&body) {
@@ -324,8 +338,9 @@ class Kdtree {
   // Merged attributes of the runs of duplicates (run u: positions
   // start[u] .. end - 1 of order), on the device for the attribute types it
   // merges (bk_run_merge), else with the host merge; dup lists the runs.
-  void merge_runs(const std::vector<long long> &order, const std::vector<long long> &start, size_t n,
-                  const std::vector<size_t> &dup, std::vector<AttrT> &merged) const {
+  template <typename IdxVec, typename DupVec>
+  void merge_runs(const IdxVec &order, const IdxVec &start, size_t n, const DupVec &dup,
+                  std::vector<AttrT> &merged) const {
     merged.resize(dup.size());
     if (dup.empty()) return;
     auto run_end = [&](size_t u) { return u + 1 < start.size() ? (size_t)start[u + 1] : n; };
@@ -338,14 +353,14 @@ class Kdtree {
     }
   }
 
-  template <typename RunEnd>
-  bool merge_runs_device(const std::vector<long long> &, const std::vector<long long> &, RunEnd &&,
-                         const std::vector<size_t> &, std::vector<AttrT> &, std::false_type) const {
+  template <typename IdxVec, typename DupVec, typename RunEnd>
+  bool merge_runs_device(const IdxVec &, const IdxVec &, RunEnd &&, const DupVec &, std::vector<AttrT> &,
+                         std::false_type) const {
     return false;
   }
-  template <typename RunEnd>
-  bool merge_runs_device(const std::vector<long long> &order, const std::vector<long long> &start, RunEnd &&run_end,
-                         const std::vector<size_t> &dup, std::vector<AttrT> &merged, std::true_type) const {
+  template <typename IdxVec, typename DupVec, typename RunEnd>
+  bool merge_runs_device(const IdxVec &order, const IdxVec &start, RunEnd &&run_end, const DupVec &dup,
+                         std::vector<AttrT> &merged, std::true_type) const {
     constexpr int ns = Merge::n_sum, nmn = Merge::n_min, nmx = Merge::n_max;
     std::vector<long long> off(dup.size() + 1, 0);
     for (size_t k = 0; k < dup.size(); ++k) off[k + 1] = off[k] + (long long)(run_end(dup[k]) - (size_t)start[dup[k]]);
@@ -376,28 +391,45 @@ class Kdtree {
   // compact records, the points moved into tree order once, node attributes
   // on host threads.  false: nothing done.
   bool device_build() {
+    // host scratch arrays are not zero-filled (HostVector): the threads that
+    // fill them touch their pages
     const size_t n = points_.size();
-    std::vector<double> xy(n * D);
+    HostVector<double> xy(n * D);
     on_threads(n, [&](size_t a, size_t b) {
       for (size_t i = a; i < b; ++i)
         for (int d = 0; d < D; ++d) xy[i * D + d] = (double)points_[i][d];
     });
     size_t nu = 0;
-    std::vector<long long> order(n), start(n);
+    HostVector<long long> order(n), start(n);
     std::vector<double> box(2 * D);
     if (bk_lexico_runs(0, D, n, xy.data(), order.data(), &nu, start.data(), box.data()) != BK_OK) return false;
     start.resize(nu);
-    std::vector<size_t> dup, dup_of(nu, (size_t)-1);
-    for (size_t u = 0; u < nu; ++u) {
-      const size_t e = u + 1 < nu ? (size_t)start[u + 1] : n;
-      if (e - (size_t)start[u] > 1) {
-        dup_of[u] = dup.size();
-        dup.push_back(u);
+    // runs of more than one point (duplicates), in run order: counted per
+    // slice on threads, then listed at each slice's offset
+    auto run_len = [&](size_t u) { return (u + 1 < nu ? (size_t)start[u + 1] : n) - (size_t)start[u]; };
+    const int t = build_threads();
+    std::vector<size_t> slice_dups(t + 1, 0);
+    on_threads_indexed(nu, [&](int c, size_t a, size_t b) {
+      size_t k = 0;
+      for (size_t u = a; u < b; ++u) k += run_len(u) > 1;
+      slice_dups[c + 1] = k;
+    });
+    for (int c = 0; c < t; ++c) slice_dups[c + 1] += slice_dups[c];
+    HostVector<size_t> dup(slice_dups[t]), dup_of(nu);
+    on_threads_indexed(nu, [&](int c, size_t a, size_t b) {
+      size_t k = slice_dups[c];
+      for (size_t u = a; u < b; ++u) {
+        if (run_len(u) > 1) {
+          dup_of[u] = k;
+          dup[k++] = u;
+        } else {
+          dup_of[u] = (size_t)-1;
+        }
       }
-    }
+    });
     std::vector<AttrT> merged;
     merge_runs(order, start, n, dup, merged);
-    std::vector<Record> rec(nu);
+    HostVector<Record> rec(nu);
     on_threads(nu, [&](size_t a, size_t b) {
       for (size_t u = a; u < b; ++u) {
         const size_t i = (size_t)order[(size_t)start[u]];
@@ -410,15 +442,26 @@ class Kdtree {
     int spawn_depth = 0;
     while ((1 << spawn_depth) < build_threads()) ++spawn_depth;
     Node *root = make_node_rec(rec, 0, nu - 1, 0, cell, spawn_depth);
-    std::vector<DataPointType> out(nu);
+    // the points into tree order: moved into raw storage on threads (k-th
+    // slot from its source), then moved back over the first nu points
+    using Slot = typename std::aligned_storage<sizeof(DataPointType), alignof(DataPointType)>::type;
+    std::unique_ptr<Slot[]> raw(new Slot[nu]);
+    DataPointType *tmp = reinterpret_cast<DataPointType *>(raw.get());
     on_threads(nu, [&](size_t a, size_t b) {
       for (size_t k = a; k < b; ++k) {
         const size_t u = rec[k].i;
-        out[k] = std::move(points_[(size_t)order[(size_t)start[u]]]);
-        if (dup_of[u] != (size_t)-1) out[k].set_attributes(merged[dup_of[u]]);
+        DataPointType *p = ::new (static_cast<void *>(tmp + k)) DataPointType(
+            std::move(points_[(size_t)order[(size_t)start[u]]]));
+        if (dup_of[u] != (size_t)-1) p->set_attributes(merged[dup_of[u]]);
       }
     });
-    points_.swap(out);
+    on_threads(nu, [&](size_t a, size_t b) {
+      for (size_t k = a; k < b; ++k) {
+        points_[k] = std::move(tmp[k]);
+        tmp[k].~DataPointType();
+      }
+    });
+    points_.resize(nu);
     root_ = root;
     refresh_parallel(root_, spawn_depth);
     return true;
@@ -426,7 +469,7 @@ class Kdtree {
 
   // the partition of make_node over records; node attributes are set after
   // the points move into tree order
-  Node *make_node_rec(std::vector<Record> &rec, IndexType i, IndexType j, int d, const RectangleType &cell,
+  Node *make_node_rec(HostVector<Record> &rec, IndexType i, IndexType j, int d, const RectangleType &cell,
                       int spawn_depth) {
     Node *n = new Node();
     n->start_idx_ = i;

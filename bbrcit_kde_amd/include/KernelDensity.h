@@ -379,7 +379,7 @@ class KernelDensity {
     const FloatType norm = (FloatType)kernel.normalization();
     Device<K> dev(points(), qtree.points());
     dev.kernel() = kernel;
-    std::vector<double> lo, up;
+    hip_detail::HostBuffer lo, up;
     dev.unnormalized_eval_bounded(0, qtree.size() - 1, (double)rel_err, (double)(abs_err / norm), lo, up);
     // bounds into the query points (large sets on host threads), then the
     // reference's precision report for the queries that miss the tolerance,
