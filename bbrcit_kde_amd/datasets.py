@@ -37,6 +37,24 @@ def uniform_box(n: int, lo: float, hi: float, seed: int, dim: int = 2) -> np.nda
     return rng.uniform(lo, hi, size=(n, dim))
 
 
+def _mixture_components(rng, n_components, mean_box, eig_range):
+    means = rng.uniform(-mean_box, mean_box, size=(n_components, 2))
+    lam = rng.uniform(eig_range[0], eig_range[1], size=(n_components, 2))
+    theta = rng.uniform(0.0, np.pi, size=n_components)
+    covs = []
+    for k in range(n_components):
+        c, s = np.cos(theta[k]), np.sin(theta[k])
+        r = np.array([[c, -s], [s, c]])
+        covs.append(r @ np.diag(lam[k]) @ r.T)
+    return means, covs
+
+
+def gaussian_mixture_components(seed: int, n_components: int = 3, mean_box: float = 2.0, eig_range=(0.05, 0.5)):
+    """(means (k, 2), covariances [k x (2, 2)]) of gaussian_mixture(n, seed)'s
+    components (equal proportions), for any n."""
+    return _mixture_components(np.random.default_rng(seed), n_components, mean_box, eig_range)
+
+
 def gaussian_mixture(n: int, seed: int, n_components: int = 3, mean_box: float = 2.0,
                      eig_range=(0.05, 0.5)):
     """Points of an equal-proportion 2-D Gaussian mixture and their weights.
@@ -47,15 +65,8 @@ def gaussian_mixture(n: int, seed: int, n_components: int = 3, mean_box: float =
     Returns (points (n, 2), weights (n,)).
     """
     rng = np.random.default_rng(seed)
-    means = rng.uniform(-mean_box, mean_box, size=(n_components, 2))
-    lam = rng.uniform(eig_range[0], eig_range[1], size=(n_components, 2))
-    theta = rng.uniform(0.0, np.pi, size=n_components)
-    chol = []
-    for k in range(n_components):
-        c, s = np.cos(theta[k]), np.sin(theta[k])
-        r = np.array([[c, -s], [s, c]])
-        cov = r @ np.diag(lam[k]) @ r.T
-        chol.append(np.linalg.cholesky(cov))
+    means, covs = _mixture_components(rng, n_components, mean_box, eig_range)
+    chol = [np.linalg.cholesky(c) for c in covs]
     comp = rng.integers(0, n_components, size=n)
     z = rng.standard_normal((n, 2))
     pts = np.empty((n, 2))
