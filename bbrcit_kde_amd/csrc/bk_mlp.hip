@@ -62,7 +62,7 @@ static_assert(kNumParams == BK_MLP_NUM_PARAMS, "parameter count");
 
 constexpr int kGemmTile = 64;
 constexpr int kGemmK = 16;
-constexpr int kSplitK = 16;   // batch splits of the weight-gradient GEMMs
+constexpr int kSplitK = 64;   // batch splits of the weight-gradient GEMMs (4 output tiles x 64 = 256 workgroups)
 constexpr int kMaxBatch = 65536;
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -185,6 +185,7 @@ __global__ void sum_splits_kernel(const float *__restrict__ part, int splits, in
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   float s = 0.0f;
+#pragma unroll 8
   for (int z = 0; z < splits; ++z) s += part[(long long)z * n + k];
   out[k] = s;
 }
@@ -215,22 +216,104 @@ __global__ __launch_bounds__(256) void mlp_head_kernel(const float *__restrict__
   d[64 + lane] = g * wq[kOffW4 + 64 + lane] * dsilu(z[64 + lane]);
 }
 
-// out[c * stride] = sum_r scale[r] A[r*lda + c] (scale may be null); one
-// workgroup per column, FP64 accumulation, fixed reduction tree.
-__global__ __launch_bounds__(256) void colsum_kernel(const float *__restrict__ A, int lda, int rows,
-                                                     const float *__restrict__ scale, int scale_stride,
-                                                     float *__restrict__ out, int stride) {
-  __shared__ double sh[4];
-  const int c = blockIdx.x;
-  double s = 0.0;
-  for (int r = threadIdx.x; r < rows; r += 256) {
-    const double v = (double)A[(long long)r * lda + c];
-    s += scale ? v * (double)scale[(long long)r * scale_stride] : v;
+// The backward pass's column sums in two passes, each sum in a fixed order.
+// Family f of output column c (kGradFamilies of them):
+//   0: sum_b dy[b] H3[b][c] (dW4)    1: sum_b dZ3[b][c] (db3)
+//   2: sum_b dZ2[b][c] (db2)         3: sum_b dZ1[b][c] x_b (dW1[c][0])
+//   4: sum_b dZ1[b][c] y_b (dW1[c][1])  5: sum_b dZ1[b][c] (db1)
+//   6: sum_b dy[b] (db4; column 0 only)
+// Pass 1: workgroup (chunk, matrix) reads rows [chunk * kGradRows, ...) of
+// one matrix (H3 with dy: families 0 and 6; dZ3: 1; dZ2: 2; dZ1: 3, 4, 5)
+// and sums its families in FP64, thread t on column t % 128 and the rows of
+// parity t / 128 (each row a coalesced 512-byte line, four rows of a thread
+// in flight), the two parities added in order.  Pass 2: per (f, c), the
+// chunks added in order.
+constexpr int kGradFamilies = 7;
+constexpr int kGradRows = 256;
+
+struct GradCols {
+  const float *H3, *dZ3, *dZ2, *dZ1, *dy, *X;
+};
+
+__global__ __launch_bounds__(256) void grad_colsum_partial_kernel(GradCols a, int B, double *__restrict__ part) {
+  __shared__ double sh[3][kWidth];
+  const int chunk = blockIdx.x, mat = blockIdx.y;
+  const int c = threadIdx.x % kWidth, par = threadIdx.x / kWidth;
+  const int r0 = chunk * kGradRows, r1 = min(B, r0 + kGradRows);
+  const float *A = mat == 0 ? a.H3 : mat == 1 ? a.dZ3 : mat == 2 ? a.dZ2 : a.dZ1;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  int r = r0 + par;
+  for (; r + 6 < r1; r += 8) {
+    float v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = A[(long long)(r + 2 * u) * kWidth + c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int ru = r + 2 * u;
+      if (mat == 0) {
+        s0 += (double)v[u] * (double)a.dy[ru];
+        s1 += c == 0 ? (double)a.dy[ru] : 0.0;
+      } else if (mat == 3) {
+        s0 += (double)v[u] * (double)a.X[2 * ru];
+        s1 += (double)v[u] * (double)a.X[2 * ru + 1];
+        s2 += (double)v[u];
+      } else {
+        s0 += (double)v[u];
+      }
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+  for (; r < r1; r += 2) {
+    const float v = A[(long long)r * kWidth + c];
+    if (mat == 0) {
+      s0 += (double)v * (double)a.dy[r];
+      s1 += c == 0 ? (double)a.dy[r] : 0.0;
+    } else if (mat == 3) {
+      s0 += (double)v * (double)a.X[2 * r];
+      s1 += (double)v * (double)a.X[2 * r + 1];
+      s2 += (double)v;
+    } else {
+      s0 += (double)v;
+    }
+  }
+  if (par == 1) {
+    sh[0][c] = s0;
+    sh[1][c] = s1;
+    sh[2][c] = s2;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) out[(long long)c * stride] = (float)((sh[0] + sh[1]) + (sh[2] + sh[3]));
+  if (par == 0) {
+    double *o = part + (long long)chunk * kGradFamilies * kWidth;
+    if (mat == 0) {
+      o[0 * kWidth + c] = s0 + sh[0][c];
+      o[6 * kWidth + c] = s1 + sh[1][c];
+    } else if (mat == 3) {
+      o[3 * kWidth + c] = s0 + sh[0][c];
+      o[4 * kWidth + c] = s1 + sh[1][c];
+      o[5 * kWidth + c] = s2 + sh[2][c];
+    } else {
+      o[mat * kWidth + c] = s0 + sh[0][c];
+    }
+  }
+}
+
+__global__ void grad_colsum_final_kernel(const double *__restrict__ part, int chunks, float *__restrict__ g) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kGradFamilies * kWidth) return;
+  const int f = k / kWidth, c = k % kWidth;
+  if (f == 6 && c != 0) return;
+  double s = 0.0;
+#pragma unroll 8
+  for (int ch = 0; ch < chunks; ++ch) s += part[((long long)ch * kGradFamilies + f) * kWidth + c];
+  const float v = (float)s;
+  switch (f) {
+    case 0: g[kOffW4 + c] = v; break;
+    case 1: g[kOffB3 + c] = v; break;
+    case 2: g[kOffB2 + c] = v; break;
+    case 3: g[kOffW1 + 2 * c] = v; break;
+    case 4: g[kOffW1 + 2 * c + 1] = v; break;
+    case 5: g[kOffB1 + c] = v; break;
+    default: g[kOffB4] = v; break;
+  }
 }
 
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
@@ -479,7 +562,7 @@ struct bk_mlp {
   bool have_params = false;
   // training workspace
   int ws_batch = 0;
-  DevBuf X, T, Z1, H1, Z2, H2, Z3, H3, dZ3, dZ2, dZ1, dH, sq, dy, part;
+  DevBuf X, T, Z1, H1, Z2, H2, Z3, H3, dZ3, dZ2, dZ1, dH, sq, dy, part, gpart;
   // serving workspace
   DevBuf slist, flist, cnt, cell_err, gx, gy, exact, fx, fy, fexact, keys_a, keys_b, sort_tmp;
   long long serve_cap = 0, fb_cap = 0;
@@ -503,6 +586,7 @@ void ensure_workspace(bk_mlp *p, int B) {
   p->sq.reserve((size_t)B * sizeof(float));
   p->dy.reserve((size_t)B * sizeof(float));
   p->part.reserve((size_t)kSplitK * kWidth * kWidth * sizeof(float));
+  p->gpart.reserve((size_t)((B + kGradRows - 1) / kGradRows) * kGradFamilies * kWidth * sizeof(double));
   p->ws_batch = B;
 }
 
@@ -515,9 +599,17 @@ void gemm(hipStream_t s, int M, int N, int K, int splits, const float *A, int ld
   hip_check(hipGetLastError(), "GEMM launch");
 }
 
-void colsum(hipStream_t s, const float *A, int lda, int rows, int cols, const float *scale, int scale_stride,
-            float *out, int stride) {
-  hipLaunchKernelGGL(colsum_kernel, dim3(cols), dim3(256), 0, s, A, lda, rows, scale, scale_stride, out, stride);
+// The backward pass's column sums (grad_colsum_*_kernel) into g.
+void grad_colsums(bk_mlp *p, int B, float *g) {
+  const hipStream_t s = p->stream;
+  const int chunks = (B + kGradRows - 1) / kGradRows;  // gpart sized by ensure_workspace
+  GradCols a{p->H3.as<float>(), p->dZ3.as<float>(), p->dZ2.as<float>(), p->dZ1.as<float>(), p->dy.as<float>(),
+             p->X.as<float>()};
+  hipLaunchKernelGGL(grad_colsum_partial_kernel, dim3(chunks, 4), dim3(256), 0, s, a, B,
+                     p->gpart.as<double>());
+  hip_check(hipGetLastError(), "column-sum launch");
+  hipLaunchKernelGGL(grad_colsum_final_kernel, dim3(grid_for(kGradFamilies * kWidth, 256)), dim3(256), 0, s,
+                     p->gpart.as<double>(), chunks, g);
   hip_check(hipGetLastError(), "column-sum launch");
 }
 
@@ -527,7 +619,6 @@ void forward_backward(bk_mlp *p, int B) {
   const hipStream_t s = p->stream;
   float *wq = p->wq.as<float>();
   float *g = p->grad.as<float>();
-  const float *X = p->X.as<float>();
   hipLaunchKernelGGL(mlp_layer1_kernel, dim3(grid_for((long long)B * kWidth, 256)), dim3(256), 0, s,
                      p->X.as<float2>(), wq, B, p->Z1.as<float>(), p->H1.as<float>());
   hip_check(hipGetLastError(), "layer-1 launch");
@@ -538,9 +629,6 @@ void forward_backward(bk_mlp *p, int B) {
   hipLaunchKernelGGL(mlp_head_kernel, dim3(grid_for(B, 4)), dim3(256), 0, s, p->H3.as<float>(), p->Z3.as<float>(),
                      wq, p->T.as<float>(), B, p->sq.as<float>(), p->dy.as<float>(), p->dZ3.as<float>());
   hip_check(hipGetLastError(), "head launch");
-  // layer 4
-  colsum(s, p->H3.as<float>(), kWidth, B, kWidth, p->dy.as<float>(), 1, g + kOffW4, 1);
-  colsum(s, p->dy.as<float>(), 1, B, 1, nullptr, 0, g + kOffB4, 1);
   // layer 3: dW3 = dZ3^T H2, db3, dZ2 = (dZ3 W3) * SiLU'(Z2)
   const int splits = B % (kGemmK * kSplitK) == 0 ? kSplitK : 1;
   gemm<true, false, EPI_STORE>(s, kWidth, kWidth, B, splits, p->dZ3.as<float>(), kWidth, p->H2.as<float>(), kWidth,
@@ -550,7 +638,6 @@ void forward_backward(bk_mlp *p, int B) {
                        splits, kWidth * kWidth, g + kOffW3);
     hip_check(hipGetLastError(), "split-sum launch");
   }
-  colsum(s, p->dZ3.as<float>(), kWidth, B, kWidth, nullptr, 0, g + kOffB3, 1);
   gemm<false, false, EPI_DSILU>(s, B, kWidth, kWidth, 1, p->dZ3.as<float>(), kWidth, wq + kOffW3, kWidth,
                                 p->dZ2.as<float>(), kWidth, nullptr, nullptr, p->Z2.as<float>());
   // layer 2
@@ -561,13 +648,11 @@ void forward_backward(bk_mlp *p, int B) {
                        splits, kWidth * kWidth, g + kOffW2);
     hip_check(hipGetLastError(), "split-sum launch");
   }
-  colsum(s, p->dZ2.as<float>(), kWidth, B, kWidth, nullptr, 0, g + kOffB2, 1);
   gemm<false, false, EPI_DSILU>(s, B, kWidth, kWidth, 1, p->dZ2.as<float>(), kWidth, wq + kOffW2, kWidth,
                                 p->dZ1.as<float>(), kWidth, nullptr, nullptr, p->Z1.as<float>());
-  // layer 1: dW1[o][0] = sum_b dZ1[b][o] x_b, dW1[o][1] = sum_b dZ1[b][o] y_b
-  colsum(s, p->dZ1.as<float>(), kWidth, B, kWidth, X, 2, g + kOffW1, 2);
-  colsum(s, p->dZ1.as<float>(), kWidth, B, kWidth, X + 1, 2, g + kOffW1 + 1, 2);
-  colsum(s, p->dZ1.as<float>(), kWidth, B, kWidth, nullptr, 0, g + kOffB1, 1);
+  // dW4, db4, db3, db2 and layer 1 (dW1[o][0] = sum_b dZ1[b][o] x_b,
+  // dW1[o][1] = sum_b dZ1[b][o] y_b, db1) as column sums
+  grad_colsums(p, B, g);
 }
 
 void upload_theta(bk_mlp *p, const float *theta) {
@@ -667,7 +752,7 @@ int bk_mlp_destroy(bk_mlp *p) {
       if (p->stream) (void)hipStreamSynchronize(p->stream);
       for (DevBuf *b : {&p->theta, &p->m, &p->v, &p->grad, &p->wq, &p->pk, &p->state, &p->lr, &p->xin, &p->tin,
                         &p->X, &p->T, &p->Z1, &p->H1, &p->Z2, &p->H2, &p->Z3, &p->H3, &p->dZ3, &p->dZ2, &p->dZ1,
-                        &p->dH, &p->sq, &p->dy, &p->part, &p->slist, &p->flist, &p->cnt, &p->cell_err, &p->gx,
+                        &p->dH, &p->sq, &p->dy, &p->part, &p->gpart, &p->slist, &p->flist, &p->cnt, &p->cell_err, &p->gx,
                         &p->gy, &p->exact, &p->fx, &p->fy, &p->fexact, &p->keys_a, &p->keys_b, &p->sort_tmp})
         b->release();
       for (hipEvent_t e : p->ev)
