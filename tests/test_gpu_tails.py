@@ -3,10 +3,11 @@ magnitude below an upper bound of the sum.
 
   * FP64 contexts: a point 15 bandwidths from every other one, evaluated as
     a query in the given and the spatial query order and as a leave-one-out
-    point of a kd-ordered set, held to 1e-12 relative with no floor.  Tile
-    culling in FP64 skips only tiles whose terms underflow to exactly zero,
-    so such sums are the oracle's to FP64 rounding and the likelihood
-    cross-validation score stays finite.
+    point of a kd-ordered set, held to 1e-12 relative with no floor.  FP64
+    Gaussian culling drops at most 2^-50 of each workgroup's sums (its
+    threshold follows a lower bound of those sums), so such sums are the
+    oracle's to FP64 rounding and the likelihood cross-validation score stays
+    finite.
   * FP32 Gram tiles (per-reference scales, tile radius rho^2 in (3, 4] in
     base-2 exponent units): queries whose sums sit just above the parity
     floor (1e-10 of C * sum m) are held to 1e-5 relative.  The worst
