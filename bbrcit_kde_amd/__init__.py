@@ -206,6 +206,8 @@ def load_library() -> ctypes.CDLL:
         "bk_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _FLT_P, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "bk_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _FLT_P, _DBL_P, _DBL_P, _DBL_P, _DBL_P, _DBL_P]),
+        "bk_mlp_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+        "bk_mlp_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
         "bk_mlp_set_training_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _DBL_P, _DBL_P,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "bk_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double,
@@ -608,6 +610,14 @@ class Surrogate:
         vals = [ctypes.c_double() for _ in range(5)]
         _check(self._lib.bk_mlp_get_params(self._mlp, th.ctypes.data_as(_FLT_P), *[ctypes.byref(v) for v in vals]))
         return (th,) + tuple(v.value for v in vals)
+
+    def save(self, path) -> None:
+        """Checkpoint: weights, working copy, Adam state and normalisation (bk_mlp_save)."""
+        _check(self._lib.bk_mlp_save(self._mlp, str(path).encode()))
+
+    def load(self, path) -> None:
+        """Restore a checkpoint written by save (bk_mlp_load)."""
+        _check(self._lib.bk_mlp_load(self._mlp, str(path).encode()))
 
     def set_params(self, theta, lo, hi, mu, sigma, eps) -> None:
         th = np.ascontiguousarray(np.asarray(theta, dtype=np.float32))

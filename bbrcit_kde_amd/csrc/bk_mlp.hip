@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <memory>
 #include <string>
@@ -795,6 +796,144 @@ int bk_mlp_get_params(const bk_mlp *p, float *theta, double *lo, double *hi, dou
     if (mu) *mu = p->mu;
     if (sigma) *sigma = p->sigma;
     if (eps) *eps = p->eps;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Checkpoint file (little-endian):
+//   char[8] magic "BKMLPCKP", uint32 format version, uint32 parameter count,
+//   uint32 width, uint32 layers, int64 Adam steps taken,
+//   double lo, hi, mu, sigma, eps,
+//   float theta[P] (FP32 master), float wq[P] (BF16-rounded working copy),
+//   float m[P], float v[P] (Adam moments),
+//   uint64 FNV-1a 64 of every byte before it.
+constexpr char kCkptMagic[8] = {'B', 'K', 'M', 'L', 'P', 'C', 'K', 'P'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr uint32_t kLayers = 4;
+
+uint64_t fnv1a(const std::vector<char> &b) {
+  uint64_t h = 0xcbf29ce484222325ULL;
+  for (char c : b) {
+    h ^= (uint64_t)(unsigned char)c;
+    h *= 0x100000001b3ULL;
+  }
+  return h;
+}
+
+template <typename T>
+void put(std::vector<char> &b, const T &v) {
+  const char *c = reinterpret_cast<const char *>(&v);
+  b.insert(b.end(), c, c + sizeof(T));
+}
+
+template <typename T>
+T take(const std::vector<char> &b, size_t &off) {
+  if (off + sizeof(T) > b.size()) fail(BK_ERR_INVALID_ARG, "checkpoint truncated");
+  T v;
+  std::memcpy(&v, b.data() + off, sizeof(T));
+  off += sizeof(T);
+  return v;
+}
+
+void copy_params(DevBuf &d, std::vector<float> &h) {
+  h.resize(kNumParams);
+  hip_check(hipMemcpy(h.data(), d.ptr, kNumParams * sizeof(float), hipMemcpyDeviceToHost), "copy state");
+}
+
+}  // namespace
+
+extern "C" {
+
+int bk_mlp_save(const bk_mlp *cp, const char *path) {
+  return guarded([&] {
+    if (!cp) fail(BK_ERR_INVALID_ARG, "null surrogate");
+    if (!path || !*path) fail(BK_ERR_INVALID_ARG, "null or empty path");
+    bk_mlp *p = const_cast<bk_mlp *>(cp);
+    DeviceGuard dg(p->device);
+    hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    std::vector<float> theta, wq, m, v;
+    copy_params(p->theta, theta);
+    copy_params(p->wq, wq);
+    copy_params(p->m, m);
+    copy_params(p->v, v);
+    std::vector<char> b;
+    b.insert(b.end(), kCkptMagic, kCkptMagic + 8);
+    put(b, kCkptVersion);
+    put(b, (uint32_t)kNumParams);
+    put(b, (uint32_t)kWidth);
+    put(b, kLayers);
+    put(b, (int64_t)p->steps_done);
+    for (double x : {p->lo, p->hi, p->mu, p->sigma, p->eps}) put(b, x);
+    for (const std::vector<float> *a : {&theta, &wq, &m, &v})
+      b.insert(b.end(), reinterpret_cast<const char *>(a->data()),
+               reinterpret_cast<const char *>(a->data() + kNumParams));
+    put(b, fnv1a(b));
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+      std::ofstream out(tmp, std::ios::binary | std::ios::trunc);
+      if (!out) fail(BK_ERR_INVALID_ARG, std::string("cannot write ") + tmp);
+      out.write(b.data(), (std::streamsize)b.size());
+      if (!out) fail(BK_ERR_INVALID_ARG, std::string("cannot write ") + tmp);
+    }
+    if (std::rename(tmp.c_str(), path) != 0) fail(BK_ERR_INVALID_ARG, std::string("cannot replace ") + path);
+  });
+}
+
+int bk_mlp_load(bk_mlp *p, const char *path) {
+  return guarded([&] {
+    require_mlp(p);
+    if (!path || !*path) fail(BK_ERR_INVALID_ARG, "null or empty path");
+    std::ifstream in(path, std::ios::binary);
+    if (!in) fail(BK_ERR_INVALID_ARG, std::string("cannot read ") + path);
+    std::vector<char> b((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    if (b.size() < 8 + sizeof(uint64_t) || std::memcmp(b.data(), kCkptMagic, 8) != 0)
+      fail(BK_ERR_INVALID_ARG, "not a surrogate checkpoint");
+    size_t off = b.size() - sizeof(uint64_t);
+    const uint64_t sum = take<uint64_t>(b, off);
+    b.resize(b.size() - sizeof(uint64_t));
+    if (fnv1a(b) != sum) fail(BK_ERR_INVALID_ARG, "checkpoint checksum mismatch (corrupted file)");
+    off = 8;
+    if (take<uint32_t>(b, off) != kCkptVersion) fail(BK_ERR_INVALID_ARG, "checkpoint format version not supported");
+    if (take<uint32_t>(b, off) != (uint32_t)kNumParams || take<uint32_t>(b, off) != (uint32_t)kWidth ||
+        take<uint32_t>(b, off) != kLayers)
+      fail(BK_ERR_INVALID_ARG, "checkpoint of another network shape");
+    const int64_t steps = take<int64_t>(b, off);
+    double nv[5];
+    for (double &x : nv) x = take<double>(b, off);
+    if (b.size() - off != 4 * (size_t)kNumParams * sizeof(float)) fail(BK_ERR_INVALID_ARG, "checkpoint size mismatch");
+    std::vector<float> a[4];
+    for (auto &x : a) {
+      x.resize(kNumParams);
+      std::memcpy(x.data(), b.data() + off, kNumParams * sizeof(float));
+      off += kNumParams * sizeof(float);
+    }
+    if (steps < 0) fail(BK_ERR_INVALID_ARG, "bad step count in checkpoint");
+    if (!(nv[1] > nv[0]) || !(nv[3] > 0.0)) fail(BK_ERR_INVALID_ARG, "bad normalisation in checkpoint");
+    DeviceGuard dg(p->device);
+    hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    std::vector<float> before;
+    copy_params(p->theta, before);
+    upload_theta(p, a[0].data());
+    // the working copy follows from the master weights; a file whose stored
+    // copy differs was written under another rounding rule (the surrogate is
+    // left as it was)
+    std::vector<float> wq;
+    copy_params(p->wq, wq);
+    if (std::memcmp(wq.data(), a[1].data(), kNumParams * sizeof(float)) != 0) {
+      upload_theta(p, before.data());
+      fail(BK_ERR_INVALID_ARG, "checkpoint working copy does not match its master weights");
+    }
+    hip_check(hipMemcpy(p->m.ptr, a[2].data(), kNumParams * sizeof(float), hipMemcpyHostToDevice), "restore m");
+    hip_check(hipMemcpy(p->v.ptr, a[3].data(), kNumParams * sizeof(float), hipMemcpyHostToDevice), "restore v");
+    p->steps_done = (long long)steps;
+    p->lo = nv[0];
+    p->hi = nv[1];
+    p->mu = nv[2];
+    p->sigma = nv[3];
+    p->eps = nv[4];
   });
 }
 

@@ -514,6 +514,19 @@ int bk_mlp_set_params(bk_mlp *mlp, const float *theta, double lo, double hi,
 int bk_mlp_get_params(const bk_mlp *mlp, float *theta, double *lo,
                       double *hi, double *mu, double *sigma, double *eps);
 
+/* Checkpoint and resume: bk_mlp_save writes a versioned binary file (magic
+ * "BKMLPCKP", format version 1, the network shape, the Adam step count, the
+ * input box and output normalisation, the FP32 master parameters, their
+ * BF16-rounded working copy, the Adam moments m and v, and an FNV-1a 64
+ * checksum), through a temporary file renamed over `path`.  bk_mlp_load
+ * restores all of it, so bk_mlp_train continues exactly as it would have
+ * without the save; it fails (leaving the surrogate unchanged) on another
+ * magic, version or shape, a checksum mismatch, or a working copy that does
+ * not follow from the master weights.  The training set is not part of a
+ * checkpoint.  No counterpart in the reference, which has no learned part. */
+int bk_mlp_save(const bk_mlp *mlp, const char *path);
+int bk_mlp_load(bk_mlp *mlp, const char *path);
+
 /* Training set: n queries q (row-major x, y) with exact densities f >= 0;
  * sets lo, hi, eps and mu, sigma (mean and deviation of ln(f + eps)). */
 int bk_mlp_set_training_data(bk_mlp *mlp, size_t n, const double *q,

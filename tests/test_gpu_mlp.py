@@ -82,6 +82,42 @@ def test_training_lowers_loss_and_is_deterministic(bk):
     assert np.array_equal(out[0][0], out[1][0])
 
 
+def test_checkpoint_resumes_training_exactly(bk, tmp_path):
+    """Training 300 + 300 steps with a save and a load in between (in a new
+    surrogate) gives the bits of training without the break: the checkpoint
+    carries the master weights, the Adam moments and step count and the
+    normalisation.  Corrupted, truncated, foreign and other-version files are
+    refused and leave the surrogate unchanged."""
+    q, f = toy_set(65536, 4)
+    path = tmp_path / "mlp.ckpt"
+    with bk.Surrogate(seed=2) as a:
+        a.set_training_data(q, f, -2.0, 2.0, 1e-6)
+        a.train(300, 1024, 3e-3, 1e-4, seed=11)
+        a.save(path)
+        a.train(300, 1024, 1e-3, 1e-5, seed=12)
+        want = a.params()
+    with bk.Surrogate(seed=99) as b:
+        b.set_training_data(q, f, -2.0, 2.0, 1e-6)
+        b.load(path)
+        b.train(300, 1024, 1e-3, 1e-5, seed=12)
+        got = b.params()
+    np.testing.assert_array_equal(got[0], want[0])
+    assert got[1:] == want[1:]
+    raw = path.read_bytes()
+    bad = {"flipped": raw[:200] + bytes([raw[200] ^ 1]) + raw[201:], "truncated": raw[:-9],
+           "foreign": b"NOTACKPT" + raw[8:], "version": raw[:8] + (2).to_bytes(4, "little") + raw[12:]}
+    with bk.Surrogate(seed=5) as c:
+        before = c.params()
+        for name, data in bad.items():
+            p = tmp_path / f"{name}.ckpt"
+            p.write_bytes(data)
+            with pytest.raises(ValueError):
+                c.load(p)
+        after = c.params()
+    np.testing.assert_array_equal(after[0], before[0])
+    assert after[1:] == before[1:]
+
+
 def test_serve_replaces_failing_cells_with_exact_sums(bk, oracle):
     torch = pytest.importorskip("torch")
     pts, w = ds.gaussian_mixture(20000, seed=77)
