@@ -460,7 +460,10 @@ FeatBytes feature_bytes(Path path, size_t n) {
       b.tiles = (n + bk::kTile - 1) / bk::kTile * 2 * sizeof(double4);
       break;
     case Path::GAUSS32:
-      b.f0 = b.f1 = n * sizeof(float4);
+      // f1: whole tiles (a factored tile's packed features fill the first
+      // three quarters of its block, masked past n)
+      b.f0 = n * sizeof(float4);
+      b.f1 = (n + bk::kGaussTile - 1) / bk::kGaussTile * bk::kGaussTile * sizeof(float4);
       b.tiles = (n + bk::kGaussTile - 1) / bk::kGaussTile * bk::kGaussTileMeta * sizeof(float4);
       break;
     case Path::EPAN64:

@@ -56,6 +56,10 @@ constexpr float kFormFactored = 2.0f;   // tile-centred, one W for the tile (til
 constexpr float kFormFactored4 = 3.0f;  // factored about the centres of the tile's four sub-tiles
 constexpr int kTileMeta = kGaussTileMeta;  // float4 per tile: (cx, cy, form, W), 4 sub-tile centres, box
 constexpr int kSubTile = kGaussTile / 4;
+// float4 of a factored tile's packed features: (A0, A1, C) of its kGaussTile
+// references, held in the first slots of the tile's block of the centred features
+constexpr int kGaussPackedF4 = kGaussTile * 3 / 4;
+static_assert(kGaussTile % 4 == 0 && kGaussPackedF4 <= kGaussTile, "packed features fit the tile's block");
 
 __device__ __forceinline__ float clamp01(float v) {
   return __builtin_fminf(__builtin_fmaxf(v, 0.0f), 1.0f);
@@ -338,6 +342,77 @@ __device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cn
     gauss_tile_loop<FORM, ISO, false>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
 }
 
+// Sum of one factored tile staged in the packed layout: reference j's
+// (A0, A1, C) are floats 3j .. 3j+2 of the stage, four references in three
+// float4, so a wavefront reads 12 bytes of LDS per reference instead of 16
+// (the LDS reads, not the VALU, set the pace of the one-float4-per-pair
+// loop).  The terms are those of gauss_factored_term, added in the same
+// order.  MASK: only references j0 <= j < j1 contribute (a tile that the
+// range covers in part); DIAG: the self pair is left out.
+template <bool MASK, bool DIAG>
+__device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf, int j0, int j1, long long tile,
+                                                  long long q_self0, const float *ax, const float *ay, float *s) {
+  const int g0 = MASK ? (j0 >> 2) : 0;
+  const int g1 = MASK ? ((j1 + 3) >> 2) : kGaussTile / 4;
+  auto term = [&](float a0, float a1, float c, int j) {
+#pragma unroll
+    for (int k = 0; k < kGaussQpt; ++k) {
+      const float v = __builtin_amdgcn_exp2f(__builtin_fmaf(a0, ax[k], __builtin_fmaf(a1, ay[k], c)));
+      if constexpr (MASK || DIAG) {
+        bool keep = true;
+        if constexpr (MASK) keep = j >= j0 && j < j1;
+        if constexpr (DIAG) keep = keep && (tile + j != q_self0 + k * kBlock);
+        s[k] += keep ? v : 0.0f;
+      } else {
+        s[k] += v;
+      }
+    }
+  };
+  if constexpr (!MASK && !DIAG) {
+    // the common case, software pipelined: the next group's three float4 are
+    // read before the current group's terms (the scheduling barrier keeps
+    // the reads there)
+    float4 a = sf[3 * g0], b = sf[3 * g0 + 1], c = sf[3 * g0 + 2];
+    for (int g = g0; g < g1; ++g) {
+      const int gn = g + 1 < g1 ? g + 1 : g;
+      const float4 na = sf[3 * gn], nb = sf[3 * gn + 1], nc = sf[3 * gn + 2];
+      __builtin_amdgcn_sched_barrier(0);
+      term(a.x, a.y, a.z, 4 * g);
+      term(a.w, b.x, b.y, 4 * g + 1);
+      term(b.z, b.w, c.x, 4 * g + 2);
+      term(c.y, c.z, c.w, 4 * g + 3);
+      a = na;
+      b = nb;
+      c = nc;
+    }
+  } else {
+    for (int g = g0; g < g1; ++g) {
+      const float4 a = sf[3 * g], b = sf[3 * g + 1], c = sf[3 * g + 2];
+      term(a.x, a.y, a.z, 4 * g);
+      term(a.w, b.x, b.y, 4 * g + 1);
+      term(b.z, b.w, c.x, 4 * g + 2);
+      term(c.y, c.z, c.w, 4 * g + 3);
+    }
+  }
+}
+
+__device__ __forceinline__ void gauss_packed_tile(const float4 *__restrict__ sf, int j0, int j1, long long tile,
+                                                  bool diag, long long q_self0, const float *ax, const float *ay,
+                                                  float *s) {
+  const bool full = j0 == 0 && j1 == kGaussTile;
+  if (full) {
+    if (diag)
+      gauss_packed_loop<false, true>(sf, j0, j1, tile, q_self0, ax, ay, s);
+    else
+      gauss_packed_loop<false, false>(sf, j0, j1, tile, q_self0, ax, ay, s);
+  } else {
+    if (diag)
+      gauss_packed_loop<true, true>(sf, j0, j1, tile, q_self0, ax, ay, s);
+    else
+      gauss_packed_loop<true, false>(sf, j0, j1, tile, q_self0, ax, ay, s);
+  }
+}
+
 // A Gaussian tile is out of reach of a query box when W_min Q2 > kFarExp
 // between the boxes (its terms are below 2^-kFarExp of their masses).
 __device__ __forceinline__ bool gauss_tile_far(const float4 *__restrict__ tiles, long long tile_idx, float bx0,
@@ -347,6 +422,23 @@ __device__ __forceinline__ bool gauss_tile_far(const float4 *__restrict__ tiles,
   const double dx = fmax(0.0, fmax((double)c.x - (double)b.x - (double)bx1, (double)bx0 - (double)c.x - (double)b.x));
   const double dy = fmax(0.0, fmax((double)c.y - (double)b.y - (double)by1, (double)by0 - (double)c.y - (double)b.y));
   return (double)b.z * __builtin_fma(gy * dy, dy, gx * dx * dx) > (double)kFarExp;
+}
+
+// Thread t's float4 of the LDS stage of the tile starting at tl: a factored
+// tile comes packed (kGaussPackedF4 float4; the loop leaves out references
+// outside the range), the others one float4 per reference (references
+// outside [r_begin, r_end) masked).
+__device__ __forceinline__ float4 gauss_stage_load(const float4 *__restrict__ fd, const float4 *__restrict__ fg,
+                                                   long long tl, int t, bool centred, bool factored,
+                                                   long long r_begin, long long r_end) {
+  float4 v = make_float4(0.f, 0.f, 0.f, kMaskedLog);
+  const long long r = tl + t;
+  if (centred && factored) {
+    if (t < kGaussPackedF4) v = fg[r];
+  } else if (t < kGaussTile && r >= r_begin && r < r_end) {
+    v = centred ? fg[r] : fd[r];
+  }
+  return v;
 }
 
 // BOUNDED: only the tiles listed in bmask (the block's tile mask) are summed;
@@ -387,6 +479,11 @@ __device__ __forceinline__ void gauss_item(
 
   float bx0, bx1, by0, by1;
   block_query_box<kGaussQpt>(px, py, sbox, bx0, bx1, by0, by1);
+  // the box is the same for every thread: keep it in scalar registers
+  bx0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(bx0)));
+  bx1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(bx1)));
+  by0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(by0)));
+  by1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(by1)));
   // first tile at or after `tl` that is not culled (or listed), or >= r_end
   auto live_from = [&](long long tl) {
     if constexpr (BOUNDED) return next_listed_tile<T>(bmask, tl, r_end);
@@ -402,8 +499,7 @@ __device__ __forceinline__ void gauss_item(
   if (tile < r_end) {
     meta = tiles[(tile / T) * kTileMeta];
     centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
-    const long long r = tile + t;
-    if (t < T && r >= r_begin && r < r_end) nf = centred ? fg[r] : fd[r];
+    nf = gauss_stage_load(fd, fg, tile, t, centred, meta.z == kFormFactored, r_begin, r_end);
   }
   int buf = 0;
   long long next_tile = r_end;
@@ -419,8 +515,7 @@ __device__ __forceinline__ void gauss_item(
       if (nt < r_end) {
         meta = tiles[(nt / T) * kTileMeta];
         centred = (mode == GAUSS_AUTO) && meta.z != kFormDirect;
-        const long long r = nt + t;
-        if (t < T && r < r_end) nf = centred ? fg[r] : fd[r];
+        nf = gauss_stage_load(fd, fg, nt, t, centred, meta.z == kFormFactored, r_begin, r_end);
       }
     }
     const int cnt = (int)((r_end - tile) < T ? (r_end - tile) : T);
@@ -488,7 +583,8 @@ __device__ __forceinline__ void gauss_item(
     if (form == 2) {
       // a factor of zero for every query of the wavefront: the tile adds nothing
       if (__any(any_near)) {
-        gauss_tile<2, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+        const int j0 = tile < r_begin ? (int)(r_begin - tile) : 0;
+        gauss_packed_tile(sf[buf], j0, cnt, tile, diag, q_self0, ax, ay, s);
         if constexpr (COUNT) evals += (unsigned long long)cnt;
       }
     } else if (form == 1) {
@@ -652,7 +748,23 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
         make_float4(half(xmin, xmax, cx), half(ymin, ymax, cy), wmin, 0.0f);
   }
   __syncthreads();
-  if (valid) {
+  if (s_form == kFormFactored) {
+    // packed: (A0, A1, C) at floats 3j .. 3j+2 of the tile's block; the
+    // slots of references past n are masked (a zero term)
+    float *pk = reinterpret_cast<float *>(fg + (long long)blockIdx.x * T) + 3 * threadIdx.x;
+    float a0 = 0.0f, a1 = 0.0f, c = kMaskedLog;
+    if (valid) {
+      const double rx = (double)xi - (double)cx;
+      const double ry = (double)yi - (double)cy;
+      const double q = p.gx * rx * rx + p.gy * ry * ry;
+      a0 = (float)(2.0 * p.gx * W * rx);
+      a1 = (float)(2.0 * p.gy * W * ry);
+      c = mass > 0.0 ? (float)(lm - W * q) : kMaskedLog;
+    }
+    pk[0] = a0;
+    pk[1] = a1;
+    pk[2] = c;
+  } else if (valid) {
     // centred features about the tile's or the sub-tile's centre
     const bool sub = s_form == kFormFactored4;
     const double rx = (double)xi - (double)(sub ? scx : cx);
