@@ -304,8 +304,8 @@ __device__ __forceinline__ double exp2_split(double e) {
 // Sum of one tile's terms for kGaussQpt queries (self pair excluded when diag).
 // FORM: 0 direct, 1 Gram, 2 factored.
 template <int FORM, bool ISO, bool DIAG>
-__device__ __forceinline__ void gauss_tile_loop(const float4 *__restrict__ sf, int cnt, long long tile,
-                                                long long q_self0, const float *px, const float *py,
+__device__ __forceinline__ void gauss_tile_loop(const float4 *__restrict__ sf, int cnt, int self_j,
+                                                const float *px, const float *py,
                                                 const float *ax, const float *ay, const float *aq, float rho,
                                                 float *s) {
 #pragma unroll BK_GAUSS_UNROLL
@@ -324,7 +324,7 @@ __device__ __forceinline__ void gauss_tile_loop(const float4 *__restrict__ sf, i
         v = gauss_direct_term(px[k], py[k], f, rho);
       }
       if constexpr (DIAG) {
-        s[k] += (tile + j == q_self0 + k * kBlock) ? 0.0f : v;
+        s[k] += (j == self_j + k * kBlock) ? 0.0f : v;
       } else {
         s[k] += v;
       }
@@ -333,13 +333,13 @@ __device__ __forceinline__ void gauss_tile_loop(const float4 *__restrict__ sf, i
 }
 
 template <int FORM, bool ISO>
-__device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cnt, long long tile, bool diag,
-                                           long long q_self0, const float *px, const float *py, const float *ax,
+__device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cnt, bool diag, int self_j,
+                                           const float *px, const float *py, const float *ax,
                                            const float *ay, const float *aq, float rho, float *s) {
   if (diag)
-    gauss_tile_loop<FORM, ISO, true>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
+    gauss_tile_loop<FORM, ISO, true>(sf, cnt, self_j, px, py, ax, ay, aq, rho, s);
   else
-    gauss_tile_loop<FORM, ISO, false>(sf, cnt, tile, q_self0, px, py, ax, ay, aq, rho, s);
+    gauss_tile_loop<FORM, ISO, false>(sf, cnt, self_j, px, py, ax, ay, aq, rho, s);
 }
 
 // Sum of one factored tile staged in the packed layout: reference j's
@@ -350,8 +350,8 @@ __device__ __forceinline__ void gauss_tile(const float4 *__restrict__ sf, int cn
 // order.  MASK: only references j0 <= j < j1 contribute (a tile that the
 // range covers in part); DIAG: the self pair is left out.
 template <bool MASK, bool DIAG>
-__device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf, int j0, int j1, long long tile,
-                                                  long long q_self0, const float *ax, const float *ay, float *s) {
+__device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf, int j0, int j1, int self_j,
+                                                  const float *ax, const float *ay, float *s) {
   const int g0 = MASK ? (j0 >> 2) : 0;
   const int g1 = MASK ? ((j1 + 3) >> 2) : kGaussTile / 4;
   auto term = [&](float a0, float a1, float c, int j) {
@@ -361,7 +361,7 @@ __device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf,
       if constexpr (MASK || DIAG) {
         bool keep = true;
         if constexpr (MASK) keep = j >= j0 && j < j1;
-        if constexpr (DIAG) keep = keep && (tile + j != q_self0 + k * kBlock);
+        if constexpr (DIAG) keep = keep && (j != self_j + k * kBlock);
         s[k] += keep ? v : 0.0f;
       } else {
         s[k] += v;
@@ -369,21 +369,16 @@ __device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf,
     }
   };
   if constexpr (!MASK && !DIAG) {
-    // the common case, software pipelined: the next group's three float4 are
-    // read before the current group's terms (the scheduling barrier keeps
-    // the reads there)
-    float4 a = sf[3 * g0], b = sf[3 * g0 + 1], c = sf[3 * g0 + 2];
-    for (int g = g0; g < g1; ++g) {
-      const int gn = g + 1 < g1 ? g + 1 : g;
-      const float4 na = sf[3 * gn], nb = sf[3 * gn + 1], nc = sf[3 * gn + 2];
-      __builtin_amdgcn_sched_barrier(0);
-      term(a.x, a.y, a.z, 4 * g);
-      term(a.w, b.x, b.y, 4 * g + 1);
-      term(b.z, b.w, c.x, 4 * g + 2);
-      term(c.y, c.z, c.w, 4 * g + 3);
-      a = na;
-      b = nb;
-      c = nc;
+    // the common case: all kGaussTile references, a fixed trip count (the
+    // compiler hoists the LDS reads of later groups across the unrolled body)
+    constexpr int G = kGaussTile / 4;
+#pragma unroll 4
+    for (int g = 0; g < G; ++g) {
+      const float4 a = sf[3 * g], b = sf[3 * g + 1], c = sf[3 * g + 2];
+      term(a.x, a.y, a.z, 0);
+      term(a.w, b.x, b.y, 0);
+      term(b.z, b.w, c.x, 0);
+      term(c.y, c.z, c.w, 0);
     }
   } else {
     for (int g = g0; g < g1; ++g) {
@@ -396,20 +391,19 @@ __device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf,
   }
 }
 
-__device__ __forceinline__ void gauss_packed_tile(const float4 *__restrict__ sf, int j0, int j1, long long tile,
-                                                  bool diag, long long q_self0, const float *ax, const float *ay,
-                                                  float *s) {
+__device__ __forceinline__ void gauss_packed_tile(const float4 *__restrict__ sf, int j0, int j1, bool diag,
+                                                  int self_j, const float *ax, const float *ay, float *s) {
   const bool full = j0 == 0 && j1 == kGaussTile;
   if (full) {
     if (diag)
-      gauss_packed_loop<false, true>(sf, j0, j1, tile, q_self0, ax, ay, s);
+      gauss_packed_loop<false, true>(sf, j0, j1, self_j, ax, ay, s);
     else
-      gauss_packed_loop<false, false>(sf, j0, j1, tile, q_self0, ax, ay, s);
+      gauss_packed_loop<false, false>(sf, j0, j1, self_j, ax, ay, s);
   } else {
     if (diag)
-      gauss_packed_loop<true, true>(sf, j0, j1, tile, q_self0, ax, ay, s);
+      gauss_packed_loop<true, true>(sf, j0, j1, self_j, ax, ay, s);
     else
-      gauss_packed_loop<true, false>(sf, j0, j1, tile, q_self0, ax, ay, s);
+      gauss_packed_loop<true, false>(sf, j0, j1, self_j, ax, ay, s);
   }
 }
 
@@ -520,6 +514,9 @@ __device__ __forceinline__ void gauss_item(
     }
     const int cnt = (int)((r_end - tile) < T ? (r_end - tile) : T);
     const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
+    // position of this thread's first query in the tile (the self pair of a
+    // diagonal tile; |q_self0 - tile| < kGaussTile + kGaussQueriesPerBlock there)
+    const int self_j = diag ? (int)(q_self0 - tile) : 0;
     float s[kGaussQpt], ax[kGaussQpt], ay[kGaussQpt], aq[kGaussQpt];
     double fac[kGaussQpt];
 #pragma unroll
@@ -551,7 +548,7 @@ __device__ __forceinline__ void gauss_item(
         }
         // a factor of zero for every query of the wavefront: the sub-tile adds nothing
         if (__any(any_near)) {
-          gauss_tile<2, ISO>(sf[buf] + c0, cn, tile + c0, diag, q_self0, px, py, ax, ay, aq, rho, s);
+          gauss_tile<2, ISO>(sf[buf] + c0, cn, diag, self_j - c0, px, py, ax, ay, aq, rho, s);
           if constexpr (COUNT) evals += (unsigned long long)cn;
         }
 #pragma unroll
@@ -584,14 +581,14 @@ __device__ __forceinline__ void gauss_item(
       // a factor of zero for every query of the wavefront: the tile adds nothing
       if (__any(any_near)) {
         const int j0 = tile < r_begin ? (int)(r_begin - tile) : 0;
-        gauss_packed_tile(sf[buf], j0, cnt, tile, diag, q_self0, ax, ay, s);
+        gauss_packed_tile(sf[buf], j0, cnt, diag, self_j, ax, ay, s);
         if constexpr (COUNT) evals += (unsigned long long)cnt;
       }
     } else if (form == 1) {
-      gauss_tile<1, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+      gauss_tile<1, ISO>(sf[buf], cnt, diag, self_j, px, py, ax, ay, aq, rho, s);
       if constexpr (COUNT) evals += (unsigned long long)cnt;
     } else {
-      gauss_tile<0, ISO>(sf[buf], cnt, tile, diag, q_self0, px, py, ax, ay, aq, rho, s);
+      gauss_tile<0, ISO>(sf[buf], cnt, diag, self_j, px, py, ax, ay, aq, rho, s);
       if constexpr (COUNT) evals += (unsigned long long)cnt;
     }
 #pragma unroll
