@@ -144,7 +144,7 @@ __device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, doubl
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       const double u = ax * kx[a];
-      const double e = fmax(fma(-u, u, 1.0), 0.0);
+      const double e = bk::clamp_unit(fma(-u, u, 1.0));
       const double v = fmax(2.0 - u, 0.0);
       const double c = v * v * v * fma(u, u + 6.0, 4.0);
       ex[a] = m * e;
@@ -153,7 +153,7 @@ __device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, doubl
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const double u = ay * ky[b];
-      ey[b] = fmax(fma(-u, u, 1.0), 0.0);
+      ey[b] = bk::clamp_unit(fma(-u, u, 1.0));
       const double v = fmax(2.0 - u, 0.0);
       py[b] = v * v * v * fma(u, u + 6.0, 4.0);
     }

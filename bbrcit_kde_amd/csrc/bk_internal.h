@@ -19,6 +19,12 @@ namespace bk {
 //                 k(u) = (2-|u|)^3 (u^2 + 6|u| + 4) / 32 for |u| < 2
 enum Family : int { FAM_GAUSS = 0, FAM_EPAN = 1, FAM_EPANPROD = 2, FAM_EPANCONV = 3 };
 
+// max(v, 0) of an Epanechnikov factor v = 1 - u^2 formed by an FMA (so
+// v <= 1, and v is never -0): written as a clamp to [0, 1], which equals
+// max(v, 0) bit for bit there and which the compiler folds into the FMA's
+// clamp output modifier (no separate v_max_f64).
+__device__ __forceinline__ double clamp_unit(double v) { return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0); }
+
 // Host-derived constants of one kernel descriptor.
 struct EvalParams {
   int family;

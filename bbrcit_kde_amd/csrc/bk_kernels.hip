@@ -792,13 +792,12 @@ __device__ __forceinline__ double epan_term(double qx, double qy, double4 a, dou
   if constexpr (FAM == FAM_EPAN) {
     const double ux = __builtin_fma(a.x, qx, a.y);
     const double uy = __builtin_fma(a.x, qy, a.z);
-    const double t = __builtin_fma(-ux, ux, __builtin_fma(-uy, uy, 1.0));
-    return a.w * __builtin_fmax(t, 0.0);
+    return a.w * clamp_unit(__builtin_fma(-ux, ux, __builtin_fma(-uy, uy, 1.0)));
   } else if constexpr (FAM == FAM_EPANPROD) {
     const double ux = __builtin_fma(a.x, qx, a.y);
     const double uy = __builtin_fma(a.z, qy, a.w);
-    const double tx = __builtin_fmax(__builtin_fma(-ux, ux, 1.0), 0.0);
-    const double ty = __builtin_fmax(__builtin_fma(-uy, uy, 1.0), 0.0);
+    const double tx = clamp_unit(__builtin_fma(-ux, ux, 1.0));
+    const double ty = clamp_unit(__builtin_fma(-uy, uy, 1.0));
     return b.x * (tx * ty);
   } else {
     const double ux = __builtin_fabs(__builtin_fma(a.x, qx, a.y));
@@ -1090,10 +1089,10 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
     return a.w * exp2_f64(-a.z * __builtin_fma(rho * dy, dy, dx * dx));
   } else if constexpr (FAM == FAM_EPAN) {
     const double d2 = __builtin_fma(dy, dy, dx * dx);
-    return a.w * __builtin_fmax(__builtin_fma(-a.z, d2, 1.0), 0.0);
+    return a.w * clamp_unit(__builtin_fma(-a.z, d2, 1.0));
   } else if constexpr (FAM == FAM_EPANPROD) {
-    const double tx = __builtin_fmax(__builtin_fma(-a.z * dx, dx, 1.0), 0.0);
-    const double ty = __builtin_fmax(__builtin_fma(-a.w * dy, dy, 1.0), 0.0);
+    const double tx = clamp_unit(__builtin_fma(-a.z * dx, dx, 1.0));
+    const double ty = clamp_unit(__builtin_fma(-a.w * dy, dy, 1.0));
     return b.x * (tx * ty);
   } else {
     const double ux = __builtin_fabs(dx) * a.z;
@@ -1446,7 +1445,7 @@ __device__ __forceinline__ void direct_general_item(
         // w = (gx / a^2 in base-2 exponent units, mass)
         v = w.y * exp2_split(-w.x * d2);
       } else {
-        v = w.y * __builtin_fmax(__builtin_fma(-w.x, d2, 1.0), 0.0);
+        v = w.y * clamp_unit(__builtin_fma(-w.x, d2, 1.0));
       }
       if (!(self_excl && tile + j == q)) acc += v;
     }
