@@ -136,8 +136,10 @@ hipError_t launch_bounds_combine(const double *exact, const double *lo_b, const 
 // FP32 Gaussian path, D <= 2 ---------------------------------------------
 // Per reference: fd = direct-form features, fg = centred-form features; per
 // aligned tile of kGaussTile references: tiles holds kGaussTileMeta float4
-// per tile: (centre x, centre y, form, W), the centres of its 4 sub-tiles,
-// and (half width x, half width y, smallest W, 0) of its bounding box.
+// per tile: (centre x, centre y, form, W), per sub-tile of 64 (centre x,
+// centre y, reach of the sub-tile, reach of the tile) with reach the W Q2
+// beyond which a factored (sub-)tile's terms are all below 2^-56 of their
+// masses, and (half width x, half width y, smallest W, 0) of its bounding box.
 constexpr int kGaussTileMeta = 6;
 hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
                              const float *m, const float *abw, size_t n,

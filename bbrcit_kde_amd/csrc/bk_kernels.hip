@@ -41,7 +41,13 @@ constexpr float kGramMaxRho2 = 4.0f;      // compactness bound of the Gram form
 #define BK_FACTORED_RHO2 9.0f
 #endif
 constexpr float kFactoredMaxRho2 = BK_FACTORED_RHO2;  // compactness bound of the factored form
-constexpr float kFarExp = 100.0f;         // W Q2 beyond which a factored tile's terms are < 2^-49
+constexpr float kFarExp = 100.0f;         // W Q2 of the workgroup-level culling (terms < 2^-100 of the mass)
+// A factored (sub-)tile of radius rho about its centre (sqrt of base-2
+// exponent units, rho <= 3) contributes, to a query with
+// sqrt(W Q2) > kReachUnits + rho, terms below 2^-56 of their own masses: every
+// reference is then more than kReachUnits >= sqrt(56) units away.  The
+// terms a query loses this way add up to less than 2^-56 of the total mass.
+constexpr double kReachUnits = 7.4833148;
 // FP64 contexts: W Q2 beyond which every term of a tile underflows to exactly
 // zero (exp2_f64 of an exponent below -1100 is below 2^-1074 before rounding)
 constexpr double kFarExp64 = 1100.0;
@@ -54,7 +60,7 @@ constexpr float kFormDirect = 0.0f;     // differences q - r
 constexpr float kFormGram = 1.0f;       // tile-centred, per-reference W
 constexpr float kFormFactored = 2.0f;   // tile-centred, one W for the tile (tiles[t].w)
 constexpr float kFormFactored4 = 3.0f;  // factored about the centres of the tile's four sub-tiles
-constexpr int kTileMeta = kGaussTileMeta;  // float4 per tile: (cx, cy, form, W), 4 sub-tile centres, box
+constexpr int kTileMeta = kGaussTileMeta;  // float4 per tile: (cx, cy, form, W), 4 sub-tiles, box
 constexpr int kSubTile = kGaussTile / 4;
 // float4 of a factored tile's packed features: (A0, A1, C) of its kGaussTile
 // references, held in the first slots of the tile's block of the centred features
@@ -245,15 +251,18 @@ __device__ __forceinline__ long long next_listed_tile(const unsigned long long *
 //   factored: one W for the whole tile and rho^2 <= kFactoredMaxRho2: the
 //             factor exp2(-W Q2) leaves the tile's sum, 2 FMA per pair.  It
 //             is computed per (query, tile) in FP64 (exact differences, split
-//             exp2).  C + A.a = exponent + W Q2 <= W Q2, so the partial sums
-//             stay finite for W Q2 <= kFarExp; farther queries take factor 0
-//             (all their terms in the tile are below 2^-49 of the mass scale:
-//             the nearest reference is at least (10 - 3)^2 = 49 exponent units
-//             away); a wavefront whose queries all take factor 0 skips the tile;
+//             exp2).  Queries with W Q2 > (sqrt(56) + rho_t)^2, rho_t the
+//             tile's own radius (stored with the tile, rounded up), take
+//             factor 0: each of their terms in the tile is below 2^-56 of its
+//             mass (every reference is more than sqrt(56) units, 56 exponent
+//             units, away); a wavefront whose queries all take factor 0 skips
+//             the tile.  C + A.a = exponent + W Q2 <= W Q2 < (sqrt(56) + 3)^2
+//             < 110, so the partial sums stay finite;
 //   factored4: the same about the centres of the tile's four aligned
 //             sub-tiles of kSubTile references (one factor per query and
-//             sub-tile), for tiles too wide for one centre whose sub-tiles
-//             each satisfy rho^2 <= kFactoredMaxRho2.
+//             sub-tile, each sub-tile with its own reach), for tiles too wide
+//             for one centre whose sub-tiles each satisfy
+//             rho^2 <= kFactoredMaxRho2.
 // The rounding error of a centred exponent grows like (rho + distance)^2 in
 // bandwidth units, which bounds rho for the centred forms.
 // Workgroup-level culling: a tile whose bounding box lies farther than
@@ -539,7 +548,7 @@ __device__ __forceinline__ void gauss_item(
           const double dx = (double)px[k] - (double)sc.x;
           const double dy = (double)py[k] - (double)sc.y;
           const double e = (double)cur.w * __builtin_fma(gy * dy, dy, gx * dx * dx);
-          const bool near = e <= (double)kFarExp;
+          const bool near = e <= (double)sc.z;  // the sub-tile's reach
           any_near |= near;
           sfac[k] = near ? exp2_split(-e) : 0.0;
           ax[k] = near ? (float)dx : 0.0f;
@@ -558,6 +567,8 @@ __device__ __forceinline__ void gauss_item(
       continue;
     }
     bool any_near = false;
+    // the tile's reach (factored form): kept in the first sub-tile entry
+    const double reach = form == 2 ? (double)tiles[(tile / T) * kTileMeta + 1].w : 0.0;
     if (form != 0) {
 #pragma unroll
       for (int k = 0; k < kGaussQpt; ++k) {
@@ -569,7 +580,7 @@ __device__ __forceinline__ void gauss_item(
         aq[k] = (float)q2;
         if (form == 2) {
           const double e = (double)cur.w * q2;
-          const bool near = e <= (double)kFarExp;
+          const bool near = e <= reach;
           any_near |= near;
           fac[k] = near ? exp2_split(-e) : 0.0;
           ax[k] = near ? ax[k] : 0.0f;
@@ -670,6 +681,17 @@ __device__ float block_reduce_f(float v, float *sh) {
   return r;
 }
 
+// W Q2 beyond which every term of a factored (sub-)tile whose radius^2 about
+// its centre is rho2 lies below 2^-56 of its mass: (kReachUnits + rho)^2,
+// rounded up (rho2 itself may lie up to half an FP32 ulp low).
+__device__ __forceinline__ float gauss_reach_exp(float rho2) {
+  const double r = sqrt((double)rho2) * (1.0 + 0x1p-20) + 0x1p-20;
+  const double t = (kReachUnits + r) * (kReachUnits + r);
+  float f = (float)t;
+  if ((double)f < t) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (t > 0)
+  return f;
+}
+
 // One workgroup of kGaussTile threads per aligned tile; each wavefront is one
 // sub-tile of kSubTile references.
 __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
@@ -717,6 +739,7 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
   }
   const float rmax = block_reduce_f<T, true>(rho2, sh);
   const float rsub = block_reduce_f<T, true>(rho2_sub, sh);
+  const float rsub_own = wave_reduce_f(rho2_sub, true);  // this wavefront's sub-tile
   const float wmin = block_reduce_f<T, false>(w_lo, sh);
   const float wmax = block_reduce_f<T, true>(w_hi, sh);
   if (threadIdx.x == 0) {
@@ -732,7 +755,8 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
     tiles[blockIdx.x * kTileMeta] = make_float4(cx, cy, form, wmax);
   }
   if (threadIdx.x % kSubTile == 0)
-    tiles[blockIdx.x * kTileMeta + 1 + threadIdx.x / kSubTile] = make_float4(scx, scy, 0.0f, 0.0f);
+    tiles[blockIdx.x * kTileMeta + 1 + threadIdx.x / kSubTile] =
+        make_float4(scx, scy, gauss_reach_exp(rsub_own), gauss_reach_exp(rmax));
   if (threadIdx.x == 0) {
     // half widths about (cx, cy), rounded up so the box holds every point
     auto half = [](float lo, float hi, float c) {
