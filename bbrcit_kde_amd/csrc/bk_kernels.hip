@@ -247,7 +247,10 @@ __device__ __forceinline__ long long next_listed_tile(const unsigned long long *
 // Forms (rho^2 = max_r W (gx (x-cx)^2 + gy (y-cy)^2), the tile's radius in
 // bandwidth units):
 //   direct:   5 VALU per pair from the differences q - r (any tile);
-//   Gram:     3 FMA per pair (C + A.a - W Q2), rho^2 <= kGramMaxRho2;
+//   Gram:     3 FMA per pair (C + A.a - W Q2), rho^2 <= kGramMaxRho2 (rho with
+//             each reference's own W); a wavefront whose queries all lie
+//             beyond the tile's reach, W_min Q2 > (sqrt(56) + rho_t)^2, passes
+//             over the tile (each term left out is below 2^-56 of its mass);
 //   factored: one W for the whole tile and rho^2 <= kFactoredMaxRho2: the
 //             factor exp2(-W Q2) leaves the tile's sum, 2 FMA per pair.  It
 //             is computed per (query, tile) in FP64 (exact differences, split
@@ -567,8 +570,10 @@ __device__ __forceinline__ void gauss_item(
       continue;
     }
     bool any_near = false;
-    // the tile's reach (factored form): kept in the first sub-tile entry
-    const double reach = form == 2 ? (double)tiles[(tile / T) * kTileMeta + 1].w : 0.0;
+    // the tile's reach (centred forms): kept in the first sub-tile entry; the
+    // Gram form, whose references have their own W, tests the tile's smallest W
+    const double reach = form != 0 ? (double)tiles[(tile / T) * kTileMeta + 1].w : 0.0;
+    const double w_min = form == 1 ? (double)tiles[(tile / T) * kTileMeta + kTileMeta - 1].z : 0.0;
     if (form != 0) {
 #pragma unroll
       for (int k = 0; k < kGaussQpt; ++k) {
@@ -585,6 +590,10 @@ __device__ __forceinline__ void gauss_item(
           fac[k] = near ? exp2_split(-e) : 0.0;
           ax[k] = near ? ax[k] : 0.0f;
           ay[k] = near ? ay[k] : 0.0f;
+        } else {
+          // Gram form: W_r |q - r|^2 >= (sqrt(W_min Q2) - rho_t)^2 for every
+          // reference, so beyond the reach each term is below 2^-56 of its mass
+          any_near |= w_min * q2 <= reach;
         }
       }
     }
@@ -596,8 +605,11 @@ __device__ __forceinline__ void gauss_item(
         if constexpr (COUNT) evals += (unsigned long long)cnt;
       }
     } else if (form == 1) {
-      gauss_tile<1, ISO>(sf[buf], cnt, diag, self_j, px, py, ax, ay, aq, rho, s);
-      if constexpr (COUNT) evals += (unsigned long long)cnt;
+      // no query of the wavefront within the tile's reach: the tile is passed over
+      if (__any(any_near)) {
+        gauss_tile<1, ISO>(sf[buf], cnt, diag, self_j, px, py, ax, ay, aq, rho, s);
+        if constexpr (COUNT) evals += (unsigned long long)cnt;
+      }
     } else {
       gauss_tile<0, ISO>(sf[buf], cnt, diag, self_j, px, py, ax, ay, aq, rho, s);
       if constexpr (COUNT) evals += (unsigned long long)cnt;

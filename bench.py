@@ -128,8 +128,8 @@ def time_kernel(kde, desc, steps: int, warmup: int):
 def count_terms(bk, kde, desc, timed_vals):
     """One more launch, outside the timed region, of the instrumented variant
     of the same kernel: the pair terms its wavefronts evaluate.  Wavefronts
-    pass over a tile when the tile's terms are below 2^-49 of their mass for
-    all their 64 queries; those terms are not evaluated (not counted)."""
+    pass over a tile when each of the tile's terms is below 2^-56 of its mass
+    for all their 64 queries; those terms are not evaluated (not counted)."""
     import torch
 
     out = torch.empty(N_QUERY, dtype=torch.float64, device="cuda")
@@ -721,9 +721,9 @@ def bench_adaptive(bk, pts, mass, q):
                 "timing": "host wall clock around each synchronous C-ABI call (includes the FP64 reductions and "
                           "the host mass update of the adaptive pass)",
                 "note": "the pilot and leave-one-out self sums run over kd-ordered points, so their workgroups "
-                        "skip the reference tiles beyond 2^-100 of the kernel, and wavefronts pass over tiles whose "
-                        "terms are below 2^-49 of the mass scale for all their queries; value counts every "
-                        "ordered pair, evaluated_pairs the terms the kernels computed"})
+                        "skip the reference tiles beyond 2^-100 of the kernel; in every pass wavefronts pass over "
+                        "tiles each of whose terms is below 2^-56 of its mass for all their queries; value counts "
+                        "every ordered pair, evaluated_pairs the terms the kernels computed"})
     return res
 
 
@@ -793,7 +793,7 @@ def headline_roofline(head):
             "lane_terms": t.get("lane_terms"), "count_results_identical_to_timed": t.get("results_identical_to_timed"),
             "flop_per_pair": FLOP_PER_PAIR_GAUSS,
             "note": "achieved/frac: evaluated pair terms x 9 FLOP / kernel time; a wavefront passes over a "
-                    "256-reference tile whose terms are below 2^-49 of the mass scale for all its 64 queries",
+                    "256-reference tile each of whose terms is below 2^-56 of its mass for all its 64 queries",
             "traffic": GAUSS_HBM_BYTES_PER_LAUNCH,
             "traffic_note": "bytes/launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a copy-kernel calibration",
             "hbm_gb_s": GAUSS_HBM_BYTES_PER_LAUNCH / sec / 1e9}
