@@ -136,11 +136,13 @@ hipError_t launch_bounds_combine(const double *exact, const double *lo_b, const 
 // FP32 Gaussian path, D <= 2 ---------------------------------------------
 // Per reference: fd = direct-form features, fg = centred-form features; per
 // aligned tile of kGaussTile references: tiles holds kGaussTileMeta float4
-// per tile: (centre x, centre y, form, W), per sub-tile of 64 (centre x,
-// centre y, reach of the sub-tile, reach of the tile) with reach the W Q2
-// beyond which a factored (sub-)tile's terms are all below 2^-56 of their
-// masses, and (half width x, half width y, smallest W, 0) of its bounding box.
-constexpr int kGaussTileMeta = 6;
+// per tile: [0] (centre x, centre y, form, W); [1..4] per sub-tile of 64
+// (centre x, centre y, reach of the sub-tile, reach of the tile) with reach
+// the W Q2 beyond which a centred (sub-)tile's terms are all below 2^-56 of
+// their masses; [5] (half width x, half width y, smallest W, 0) of its
+// bounding box about the centre; [6..9] per sub-tile (half width x, half
+// width y, 0, 0) of its bounding box about its centre.
+constexpr int kGaussTileMeta = 10;
 hipError_t launch_prep_gauss(const EvalParams &p, const float *x, const float *y,
                              const float *m, const float *abw, size_t n,
                              double m_scale, float4 *fd, float4 *fg,

@@ -48,6 +48,9 @@ constexpr float kFarExp = 100.0f;         // W Q2 of the workgroup-level culling
 // reference is then more than kReachUnits >= sqrt(56) units away.  The
 // terms a query loses this way add up to less than 2^-56 of the total mass.
 constexpr double kReachUnits = 7.4833148;
+// The same bound for the difference form, on the distance to a sub-tile's
+// bounding box: W_min d^2 > 56, with 2^-16 of margin for FP32 rounding.
+constexpr float kReachDiff = 56.0f * (1.0f + 0x1p-16f);
 // FP64 contexts: W Q2 beyond which every term of a tile underflows to exactly
 // zero (exp2_f64 of an exponent below -1100 is below 2^-1074 before rounding)
 constexpr double kFarExp64 = 1100.0;
@@ -60,7 +63,9 @@ constexpr float kFormDirect = 0.0f;     // differences q - r
 constexpr float kFormGram = 1.0f;       // tile-centred, per-reference W
 constexpr float kFormFactored = 2.0f;   // tile-centred, one W for the tile (tiles[t].w)
 constexpr float kFormFactored4 = 3.0f;  // factored about the centres of the tile's four sub-tiles
-constexpr int kTileMeta = kGaussTileMeta;  // float4 per tile: (cx, cy, form, W), 4 sub-tiles, box
+constexpr int kTileMeta = kGaussTileMeta;  // float4 per tile: (cx, cy, form, W), 4 sub-tiles, box, 4 sub-boxes
+constexpr int kBoxSlot = 5;                 // the tile's bounding box
+constexpr int kSubBoxSlot = 6;              // the sub-tiles' bounding boxes
 constexpr int kSubTile = kGaussTile / 4;
 // float4 of a factored tile's packed features: (A0, A1, C) of its kGaussTile
 // references, held in the first slots of the tile's block of the centred features
@@ -424,7 +429,7 @@ __device__ __forceinline__ void gauss_packed_tile(const float4 *__restrict__ sf,
 __device__ __forceinline__ bool gauss_tile_far(const float4 *__restrict__ tiles, long long tile_idx, float bx0,
                                                float bx1, float by0, float by1, double gx, double gy) {
   const float4 *m = tiles + tile_idx * kTileMeta;
-  const float4 c = m[0], b = m[kTileMeta - 1];
+  const float4 c = m[0], b = m[kBoxSlot];
   const double dx = fmax(0.0, fmax((double)c.x - (double)b.x - (double)bx1, (double)bx0 - (double)c.x - (double)b.x));
   const double dy = fmax(0.0, fmax((double)c.y - (double)b.y - (double)by1, (double)by0 - (double)c.y - (double)b.y));
   return (double)b.z * __builtin_fma(gy * dy, dy, gx * dx * dx) > (double)kFarExp;
@@ -573,7 +578,7 @@ __device__ __forceinline__ void gauss_item(
     // the tile's reach (centred forms): kept in the first sub-tile entry; the
     // Gram form, whose references have their own W, tests the tile's smallest W
     const double reach = form != 0 ? (double)tiles[(tile / T) * kTileMeta + 1].w : 0.0;
-    const double w_min = form == 1 ? (double)tiles[(tile / T) * kTileMeta + kTileMeta - 1].z : 0.0;
+    const double w_min = form == 1 ? (double)tiles[(tile / T) * kTileMeta + kBoxSlot].z : 0.0;
     if (form != 0) {
 #pragma unroll
       for (int k = 0; k < kGaussQpt; ++k) {
@@ -611,8 +616,32 @@ __device__ __forceinline__ void gauss_item(
         if constexpr (COUNT) evals += (unsigned long long)cnt;
       }
     } else {
-      gauss_tile<0, ISO>(sf[buf], cnt, diag, self_j, px, py, ax, ay, aq, rho, s);
-      if constexpr (COUNT) evals += (unsigned long long)cnt;
+      // difference form, per sub-tile of kSubTile references: a wavefront
+      // passes over a sub-tile when all its queries lie beyond the sub-tile's
+      // reach, W_min d^2 > 56 with d the distance to its bounding box (every
+      // reference's own W is at least W_min, so each term left out is below
+      // 2^-56 of its mass; the FP32 distance is compared with a margin of
+      // 2^-16 for its rounding)
+      const float4 *subc = tiles + (tile / T) * kTileMeta + 1;
+      const float4 *subb = tiles + (tile / T) * kTileMeta + kSubBoxSlot;
+      const float w_lo = tiles[(tile / T) * kTileMeta + kBoxSlot].z;
+      const float gxf = (float)gx, gyf = (float)gy;
+      for (int sb = 0; sb * kSubTile < cnt; ++sb) {
+        const int c0 = sb * kSubTile;
+        const int cn = (cnt - c0) < kSubTile ? (cnt - c0) : kSubTile;
+        const float4 sc = subc[sb], hb = subb[sb];
+        bool near_any = false;
+#pragma unroll
+        for (int k = 0; k < kGaussQpt; ++k) {
+          const float dx = __builtin_fmaxf(__builtin_fabsf(px[k] - sc.x) - hb.x, 0.0f);
+          const float dy = __builtin_fmaxf(__builtin_fabsf(py[k] - sc.y) - hb.y, 0.0f);
+          near_any |= w_lo * __builtin_fmaf(gyf * dy, dy, gxf * dx * dx) <= kReachDiff;
+        }
+        if (__any(near_any)) {
+          gauss_tile<0, ISO>(sf[buf] + c0, cn, diag, self_j - c0, px, py, ax, ay, aq, rho, s);
+          if constexpr (COUNT) evals += (unsigned long long)cn;
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < kGaussQpt; ++k) acc[k] = __builtin_fma((double)s[k], fac[k], acc[k]);
@@ -693,6 +722,14 @@ __device__ float block_reduce_f(float v, float *sh) {
   return r;
 }
 
+// Half width of [lo, hi] about c, rounded up so the box holds every point.
+__device__ __forceinline__ float half_width_up(float lo, float hi, float c) {
+  const double hw = fmax((double)hi - (double)c, (double)c - (double)lo);
+  float f = (float)hw;
+  if ((double)f < hw) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f >= 0)
+  return f;
+}
+
 // W Q2 beyond which every term of a factored (sub-)tile whose radius^2 about
 // its centre is rho2 lies below 2^-56 of its mass: (kReachUnits + rho)^2,
 // rounded up (rho2 itself may lie up to half an FP32 ulp low).
@@ -766,19 +803,17 @@ __global__ __launch_bounds__(kGaussTile) void prep_gauss_kernel(
     s_form = form;
     tiles[blockIdx.x * kTileMeta] = make_float4(cx, cy, form, wmax);
   }
-  if (threadIdx.x % kSubTile == 0)
+  if (threadIdx.x % kSubTile == 0) {
     tiles[blockIdx.x * kTileMeta + 1 + threadIdx.x / kSubTile] =
         make_float4(scx, scy, gauss_reach_exp(rsub_own), gauss_reach_exp(rmax));
+    // the sub-tile's box about its centre (a single point for a sub-tile without points)
+    tiles[blockIdx.x * kTileMeta + kSubBoxSlot + threadIdx.x / kSubTile] =
+        sub_empty ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                  : make_float4(half_width_up(sxmin, sxmax, scx), half_width_up(symin, symax, scy), 0.0f, 0.0f);
+  }
   if (threadIdx.x == 0) {
-    // half widths about (cx, cy), rounded up so the box holds every point
-    auto half = [](float lo, float hi, float c) {
-      const double hw = fmax((double)hi - (double)c, (double)c - (double)lo);
-      float f = (float)hw;
-      if ((double)f < hw) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f >= 0)
-      return f;
-    };
-    tiles[blockIdx.x * kTileMeta + kTileMeta - 1] =
-        make_float4(half(xmin, xmax, cx), half(ymin, ymax, cy), wmin, 0.0f);
+    tiles[blockIdx.x * kTileMeta + kBoxSlot] =
+        make_float4(half_width_up(xmin, xmax, cx), half_width_up(ymin, ymax, cy), wmin, 0.0f);
   }
   __syncthreads();
   if (s_form == kFormFactored) {

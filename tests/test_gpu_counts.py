@@ -2,12 +2,15 @@
 pair terms its wavefronts evaluate and returns the same results as the plain
 kernel.
 
-Exact cases: with every tile in the difference form nothing is passed over, so
-the real-query count is n_query x n_ref and the lane count is that of whole
-workgroups (ceil(n_query / 256) x 256 lanes per reference, one query per lane;
-the padding lanes of the last workgroup run too).
-With the default forms, the count falls below the nominal pairs by the tiles
-whose factor is zero for a whole wavefront; the results stay bit-identical.
+Exact cases: with every tile in the difference form and a bandwidth wide
+enough that every query lies within reach of every sub-tile, nothing is passed
+over, so the real-query count is n_query x n_ref and the lane count is that of
+whole workgroups (ceil(n_query / 256) x 256 lanes per reference, one query per
+lane; the padding lanes of the last workgroup run too).
+With a narrow bandwidth the count falls below the nominal pairs by the
+(sub-)tiles beyond the reach of a whole wavefront (each term left out is below
+2^-56 of its mass); the results stay bit-identical to the plain kernel's and
+within the parity bar of the FP64 oracle.
 """
 from __future__ import annotations
 
@@ -29,8 +32,11 @@ def _case(n_ref, n_q, seed):
 
 
 def test_difference_form_counts_every_pair(bk):
+    # h = 2: a term is left out only beyond d = sqrt(56 / g) = 8.8 h = 17.6 with
+    # g = log2(e) / (2 h^2), farther than any query lies from any reference
+    # here (under 11), so nothing is passed over
     pts, mass, q = _case(20_000, 3_001, 61)
-    desc = bk.KernelDesc(GAUSS, 2, 0.05)
+    desc = bk.KernelDesc(GAUSS, 2, 2.0)
     with bk.DirectKde(pts, q, mass) as kde:
         kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
         plain = kde.eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
@@ -42,6 +48,24 @@ def test_difference_form_counts_every_pair(bk):
     assert np.array_equal(plain, counted)
     assert real == len(q) * len(pts)
     assert lanes == -(-len(q) // 256) * 256 * len(pts)
+
+
+def test_difference_form_passes_over_subtiles_out_of_reach(bk, oracle):
+    from tests.test_gpu_direct import assert_parity, f32
+
+    pts, mass, q = _case(20_000, 3_001, 63)
+    desc = bk.KernelDesc(GAUSS, 2, 0.05)
+    with bk.DirectKde(pts, q, mass) as kde:
+        kde.set_option(bk.BK_OPT_GAUSS_FORM, bk.BK_GAUSS_FORM_DIRECT)
+        plain = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        kde.set_option(bk.BK_OPT_COUNT_TERMS, 1)
+        counted = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+        lanes, real = kde.term_count()
+        kde.set_option(bk.BK_OPT_COUNT_TERMS, 0)
+    assert np.array_equal(plain, counted)
+    assert 0 < real < len(q) * len(pts)
+    want = oracle.direct(GAUSS, 0.05, 0.05, f32(pts), f32(q), mass=f32(mass))
+    assert_parity(plain, want, mass.sum(), what="difference form, sub-tiles passed over")
 
 
 def test_default_forms_count_fewer_terms_and_same_results(bk):
