@@ -59,7 +59,8 @@ BK_CVG_COUNT_UNITS = 3
 BK_CVG_TUNE_SOURCE = 4
 BK_CVG_UNROLL = 5
 BK_CVG_WORKGROUPS = 6
-BK_CVG_VARIANTS = 2
+BK_CVG_VARIANTS = 3
+BK_CVG_TUNE_MAX = 9
 
 BK_SET_REF = 0
 BK_SET_QUERY = 1
@@ -552,7 +553,7 @@ class CvGrid:
         index, pair-kernel ms), ...] in candidate order."""
         hx = np.ascontiguousarray(hx, dtype=np.float64)
         hy = np.ascontiguousarray(hy, dtype=np.float64)
-        ms = np.zeros(8)
+        ms = np.zeros(BK_CVG_TUNE_MAX)
         n = ctypes.c_int()
         _check(self._lib.bk_cv_grid_tune(self._plan, len(hx), _dptr(hx), len(hy), _dptr(hy), _dptr(ms),
                                          ctypes.byref(n)))

@@ -58,6 +58,9 @@ constexpr float kMaskedLog2 = -1.0e30f;  // base-2 log mass of a masked referenc
 constexpr double kLog2e = 1.4426950408889634074;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kRescueFraction = 9.094947017729282e-13;  // 2^-40
+// base-2 exponent beyond which a Gaussian stage's products ex ey are all
+// below 2^-152 (with the rounding of their FP32 exponents: below 2^-151)
+constexpr double kCvGaussCullExp = 152.0;
 constexpr int kScoreChunk = 4096;                            // points per score block
 
 enum CvFamily : int { CV_GAUSS = 0, CV_EPAN = 1 };
@@ -87,7 +90,7 @@ struct CvArgs {
   long long plane_stride;
   long long n_points;
   int n_hx, n_hy;
-  unsigned long long *count;  // when set, the Epanechnikov kernel adds the units it evaluates
+  unsigned long long *count;  // when set, the pair kernel adds the units it evaluates
 };
 
 // ---- Gaussian product kernel: 8 or 4 x-factors, 8 y-factors per reference ----
@@ -167,73 +170,6 @@ __device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, doubl
   }
 }
 
-// Persistent pair kernels.  Every workgroup pulls items until the queue is
-// exhausted; all branches on the item are uniform (the item index is read
-// from LDS and made scalar).
-template <int A, int B, bool ADAPT, int U>
-__global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
-  __shared__ float4 tile[2][kCvTile];
-  __shared__ int s_item;
-  const int t = threadIdx.x;
-  for (;;) {
-    __syncthreads();
-    if (t == 0) s_item = atomicAdd(g.queue, 1);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    if (it >= g.n_items) break;
-    const CvItem w = g.items[it];
-    const bool active = t < w.q_count;
-    const int qi = w.q_begin + (active ? t : 0);
-    const float4 q = g.pts[qi];
-    float kx[A], ky[B];
-#pragma unroll
-    for (int a = 0; a < A; ++a) kx[a] = g.kx[w.sub_x + a];
-#pragma unroll
-    for (int b = 0; b < B; ++b) ky[b] = g.ky[w.sub_y + b];
-    double tot[A][B];
-#pragma unroll
-    for (int a = 0; a < A; ++a)
-#pragma unroll
-      for (int b = 0; b < B; ++b) tot[a][b] = 0.0;
-
-    tile[0][t] = w.r_begin + t < w.r_end ? g.pts[w.r_begin + t] : make_float4(0.f, 0.f, kMaskedLog2, 0.f);
-    __syncthreads();
-    int buf = 0;
-    const int q_end = w.q_begin + w.q_count;
-    for (int r0 = w.r_begin; r0 < w.r_end; r0 += kCvTile) {
-      const int rn = r0 + kCvTile;
-      float4 nxt = make_float4(0.f, 0.f, kMaskedLog2, 0.f);
-      if (rn + t < w.r_end) nxt = g.pts[rn + t];
-      const bool diag = r0 < q_end && w.q_begin < rn;
-      float acc[A][B];
-#pragma unroll
-      for (int a = 0; a < A; ++a)
-#pragma unroll
-        for (int b = 0; b < B; ++b) acc[a][b] = 0.0f;
-      if (diag)
-        gauss_stage<A, B, ADAPT, true, U>(tile[buf], q.x, q.y, qi - r0, kx, ky, acc);
-      else
-        gauss_stage<A, B, ADAPT, false, U>(tile[buf], q.x, q.y, 0, kx, ky, acc);
-#pragma unroll
-      for (int a = 0; a < A; ++a)
-#pragma unroll
-        for (int b = 0; b < B; ++b) tot[a][b] += (double)acc[a][b];
-      if (rn < w.r_end) tile[buf ^ 1][t] = nxt;
-      __syncthreads();
-      buf ^= 1;
-    }
-    if (active) {
-      double *dst = g.P + (long long)w.slot * g.slot_stride + qi;
-#pragma unroll
-      for (int a = 0; a < A; ++a)
-#pragma unroll
-        for (int b = 0; b < B; ++b)
-          if (w.sub_x + a < g.n_hx && w.sub_y + b < g.n_hy)
-            dst[(long long)((w.sub_x + a) * g.n_hy + w.sub_y + b) * g.n_points] = tot[a][b];
-    }
-  }
-}
-
 // Wave-wide minimum of a double (every lane gets it).
 __device__ __forceinline__ double wave_min_d(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -255,6 +191,142 @@ __device__ __forceinline__ void box_read(const double (&box)[5][kCvBlock / 64], 
     v[i] = box[i][0];
 #pragma unroll
     for (int w = 1; w < kCvBlock / 64; ++w) v[i] = fmin(v[i], box[i][w]);
+  }
+}
+
+// Persistent pair kernels.  Every workgroup pulls items until the queue is
+// exhausted; all branches on the item are uniform (the item index is read
+// from LDS and made scalar).
+//
+// The Gaussian pair kernel skips a stage when every product ex ey of it is
+// below 2^-152 for every query of the item: the per-axis gaps between the
+// query box and the stage's box give W_min (gx^2 kx_min + gy^2 ky_min) >=
+// kCvGaussCullExp, with kx_min, ky_min the sub-grid's smallest exponent
+// factors, W_min the stage's smallest 1/abw^2 and log2(m / m_scale) <= 0.
+// Such a product is below half of FP32's smallest denormal, so the FMA that
+// would add it leaves its accumulator unchanged: skipping changes no result
+// bit.  Small bandwidth sub-grids skip most stages of Z-ordered data.
+template <int A, int B, bool ADAPT, int U>
+__global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
+  // the 8 x 8 tiling holds every bandwidth of a typical grid (nothing to
+  // skip) and uses every register of 2 waves per SIMD: it tests no stage
+  constexpr bool kCull = A * B < 64;
+  __shared__ float4 tile[2][kCvTile];
+  __shared__ double qbox[5][kCvBlock / 64];
+  __shared__ double tbox[2][5][kCvBlock / 64];
+  __shared__ int s_item;
+  const int t = threadIdx.x;
+  const double inf = __builtin_inf();
+  // (x0, -x1, y0, -y1, smallest W) of one point; a masked slot is neutral
+  auto point_box = [&](const float4 &r, bool valid, double (&v)[5]) {
+    v[0] = valid ? (double)r.x : inf;
+    v[1] = valid ? -(double)r.x : inf;
+    v[2] = valid ? (double)r.y : inf;
+    v[3] = valid ? -(double)r.y : inf;
+    v[4] = valid ? (ADAPT ? (double)r.w : 1.0) : inf;
+  };
+  for (;;) {
+    __syncthreads();
+    if (t == 0) s_item = atomicAdd(g.queue, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= g.n_items) break;
+    const CvItem w = g.items[it];
+    const bool active = t < w.q_count;
+    const int qi = w.q_begin + (active ? t : 0);
+    const float4 q = g.pts[qi];
+    float kx[A], ky[B];
+#pragma unroll
+    for (int a = 0; a < A; ++a) kx[a] = g.kx[w.sub_x + a];
+#pragma unroll
+    for (int b = 0; b < B; ++b) ky[b] = g.ky[w.sub_y + b];
+    // padded sub-grid entries repeat the last bandwidth, so the minimum over
+    // all A (B) entries is the sub-grid's smallest factor
+    float kx_min = kx[0], ky_min = ky[0];
+#pragma unroll
+    for (int a = 1; a < A; ++a) kx_min = __builtin_fminf(kx_min, kx[a]);
+#pragma unroll
+    for (int b = 1; b < B; ++b) ky_min = __builtin_fminf(ky_min, ky[b]);
+    double tot[A][B];
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+      for (int b = 0; b < B; ++b) tot[a][b] = 0.0;
+
+    if constexpr (kCull) {
+      double v[5];
+      point_box(q, true, v);  // inactive lanes hold the item's first query
+      box_write(qbox, v);
+    }
+    const bool first_valid = w.r_begin + t < w.r_end;
+    const float4 first = first_valid ? g.pts[w.r_begin + t] : make_float4(0.f, 0.f, kMaskedLog2, 0.f);
+    tile[0][t] = first;
+    if constexpr (kCull) {
+      double v[5];
+      point_box(first, first_valid, v);
+      box_write(tbox[0], v);
+    }
+    __syncthreads();
+    int buf = 0;
+    const int q_end = w.q_begin + w.q_count;
+    for (int r0 = w.r_begin; r0 < w.r_end; r0 += kCvTile) {
+      const int rn = r0 + kCvTile;
+      const bool nxt_valid = rn + t < w.r_end;
+      float4 nxt = make_float4(0.f, 0.f, kMaskedLog2, 0.f);
+      if (nxt_valid) nxt = g.pts[rn + t];
+      bool live = true;
+      if constexpr (kCull) {
+        // both boxes re-read from LDS at every stage (no registers held across it)
+        double qb[5], tb[5];
+        box_read(qbox, qb);
+        box_read(tbox[buf], tb);
+        const double gx = fmax(0.0, fmax(tb[0] + qb[1], qb[0] + tb[1]));  // x0_t - x1_q, x0_q - x1_t
+        const double gy = fmax(0.0, fmax(tb[2] + qb[3], qb[2] + tb[3]));
+        live = !(tb[4] * (gx * gx * (double)kx_min + gy * gy * (double)ky_min) >= kCvGaussCullExp);
+      }
+      if (live) {
+        if (g.count && t == 0) {
+          // (query, reference, bandwidth pair) units of this stage
+          const int la = g.n_hx - w.sub_x < A ? g.n_hx - w.sub_x : A;
+          const int lb = g.n_hy - w.sub_y < B ? g.n_hy - w.sub_y : B;
+          const int nr = w.r_end - r0 < kCvTile ? w.r_end - r0 : kCvTile;
+          atomicAdd(g.count, (unsigned long long)nr * (unsigned long long)w.q_count * (unsigned long long)(la * lb));
+        }
+        const bool diag = r0 < q_end && w.q_begin < rn;
+        float acc[A][B];
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+#pragma unroll
+          for (int b = 0; b < B; ++b) acc[a][b] = 0.0f;
+        if (diag)
+          gauss_stage<A, B, ADAPT, true, U>(tile[buf], q.x, q.y, qi - r0, kx, ky, acc);
+        else
+          gauss_stage<A, B, ADAPT, false, U>(tile[buf], q.x, q.y, 0, kx, ky, acc);
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+#pragma unroll
+          for (int b = 0; b < B; ++b) tot[a][b] += (double)acc[a][b];
+      }
+      if (rn < w.r_end) {
+        tile[buf ^ 1][t] = nxt;
+        if constexpr (kCull) {
+          double v[5];
+          point_box(nxt, nxt_valid, v);
+          box_write(tbox[buf ^ 1], v);
+        }
+      }
+      __syncthreads();
+      buf ^= 1;
+    }
+    if (active) {
+      double *dst = g.P + (long long)w.slot * g.slot_stride + qi;
+#pragma unroll
+      for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+          if (w.sub_x + a < g.n_hx && w.sub_y + b < g.n_hy)
+            dst[(long long)((w.sub_x + a) * g.n_hy + w.sub_y + b) * g.n_points] = tot[a][b];
+    }
   }
 }
 
@@ -553,16 +625,17 @@ __global__ __launch_bounds__(kCvBlock) void cv_score_final_kernel(const double *
 struct Variant {
   int A, B;
 };
-constexpr Variant kGaussVariants[2] = {{8, 8}, {4, 8}};
-constexpr Variant kEpanVariants[2] = {{4, 8}, {4, 4}};
+constexpr Variant kGaussVariants[BK_CVG_VARIANTS] = {{8, 8}, {4, 8}, {4, 4}};
+constexpr Variant kEpanVariants[BK_CVG_VARIANTS] = {{4, 8}, {4, 4}, {2, 4}};
 
 // Unroll candidates of the stage loops (index u of a schedule).
 constexpr int kGaussUnrolls[3] = {2, 4, 8};
 constexpr int kEpanUnrolls[2] = {1, 2};
-constexpr int kGaussUnrollDefault = 1;  // 4
+constexpr int kGaussUnrollDefault = 0;  // 2
 constexpr int kEpanUnrollDefault = 1;   // 2
 
 int unroll_count(int fam) { return fam == CV_GAUSS ? 3 : 2; }
+static_assert(BK_CVG_VARIANTS * 3 <= BK_CVG_TUNE_MAX, "bk_cv_grid_tune writes at most BK_CVG_TUNE_MAX timings");
 
 using PairKernel = void (*)(CvArgs);
 
@@ -582,8 +655,12 @@ PairKernel epan_kernel(bool adapt, int u) {
 }
 
 PairKernel pair_kernel(int fam, int variant, bool adapt, int u) {
-  if (fam == CV_GAUSS) return variant == 0 ? gauss_kernel<8, 8>(adapt, u) : gauss_kernel<4, 8>(adapt, u);
-  return variant == 0 ? epan_kernel<4, 8>(adapt, u) : epan_kernel<4, 4>(adapt, u);
+  if (fam == CV_GAUSS) {
+    if (variant == 0) return gauss_kernel<8, 8>(adapt, u);
+    return variant == 1 ? gauss_kernel<4, 8>(adapt, u) : gauss_kernel<4, 4>(adapt, u);
+  }
+  if (variant == 0) return epan_kernel<4, 8>(adapt, u);
+  return variant == 1 ? epan_kernel<4, 4>(adapt, u) : epan_kernel<2, 4>(adapt, u);
 }
 
 Variant variant_shape(int fam, int v) { return fam == CV_GAUSS ? kGaussVariants[v] : kEpanVariants[v]; }
@@ -860,7 +937,7 @@ void enqueue_pipeline(bk_cv_grid *g, bool time_main) {
   const long long G = (long long)g->n_hx * g->n_hy;
   const int planes = g->fam == CV_GAUSS ? 1 : 2;
   const Variant v = variant_shape(g->fam, g->used.variant);
-  const CvArgs args = make_args(g, g->items.as<CvItem>(), g->items_n, g->count_units && g->fam == CV_EPAN);
+  const CvArgs args = make_args(g, g->items.as<CvItem>(), g->items_n, g->count_units);
   hipLaunchKernelGGL(cv_reset_kernel, dim3(1), dim3(64), 0, s, g->queue.as<int>());
   hip_check(hipGetLastError(), "queue reset launch");
   if (time_main) hip_check(hipEventRecord(g->ev[2], s), "hipEventRecord");
@@ -897,8 +974,10 @@ void enqueue_pipeline(bk_cv_grid *g, bool time_main) {
                      g->P.as<double>(), g->n_points)
     if (v.A == 8)
       BK_RESCUE(8, 8);
-    else
+    else if (v.B == 8)
       BK_RESCUE(4, 8);
+    else
+      BK_RESCUE(4, 4);
 #undef BK_RESCUE
     hip_check(hipGetLastError(), "rescue launch");
   }
@@ -943,7 +1022,7 @@ bool valid_schedule(const bk_cv_grid *g, const std::vector<int> &v) {
 // Default schedule of a kind (the fastest at the 4-set 8 x 8 workload).
 Schedule default_schedule(int fam) {
   Schedule sc;
-  sc.variant = fam == CV_GAUSS ? 0 : 1;
+  sc.variant = fam == CV_GAUSS ? 2 : 1;
   sc.unroll = fam == CV_GAUSS ? kGaussUnrollDefault : kEpanUnrollDefault;
   return sc;
 }
@@ -1010,7 +1089,7 @@ void run_plan(bk_cv_grid *g, const double *hx, const double *hy) {
   upload_consts(g, hx, hy);
   prepare_shape(g);
   const hipStream_t s = g->stream;
-  const bool counting = g->count_units && g->fam == CV_EPAN;
+  const bool counting = g->count_units;
   if (counting) {
     g->counter.reserve(sizeof(unsigned long long));
     hip_check(hipMemsetAsync(g->counter.ptr, 0, sizeof(unsigned long long), s), "clear the unit counter");
@@ -1057,14 +1136,6 @@ void run_plan(bk_cv_grid *g, const double *hx, const double *hy) {
     unsigned long long units = 0;
     hip_check(hipMemcpy(&units, g->counter.ptr, sizeof(units), hipMemcpyDeviceToHost), "read the unit counter");
     g->last_units = (long long)units;
-  } else if (g->count_units) {
-    // the Gaussian pair kernel evaluates every unit
-    long long units = 0;
-    for (int d = 0; d < g->n_sets; ++d) {
-      const long long n = g->begin[d + 1] - g->begin[d];
-      units += n * n;
-    }
-    g->last_units = units * g->n_hx * g->n_hy;
   }
   g->have_run = true;
 }
@@ -1125,7 +1196,7 @@ int bk_cv_grid_create(bk_ctx *ctx, int kind, size_t n_sets, const size_t *sizes,
       // each dataset goes to the device in Z-curve order (Morton code on a
       // 1024 x 1024 grid over its box, ties by index): consecutive points
       // are close, so a work item's queries and a stage's references span
-      // small boxes (the Epanechnikov kernel skips stages out of reach)
+      // small boxes (the pair kernels skip stages out of reach)
       spatial_order(coords, g->begin[d], g->begin[d + 1], g->perm);
       for (int j = g->begin[d]; j < g->begin[d + 1]; ++j) {
         const int i = g->perm[j];

@@ -363,9 +363,11 @@ typedef struct bk_cv_grid bk_cv_grid;
                                 register tiling of the bandwidth grid */
 #define BK_CVG_GRAPH      2  /* 1 (default): replay a captured hipGraph of the
                                 whole pipeline; 0: launch eagerly */
-#define BK_CVG_VARIANTS   2  /* number of tilings: 0 = 8x8, 1 = 4x8 (Gaussian),
-                                0 = 4x8, 1 = 4x4 (Epanechnikov) bandwidth
-                                pairs per thread */
+#define BK_CVG_VARIANTS   3  /* number of tilings: 0 = 8x8, 1 = 4x8, 2 = 4x4
+                                (Gaussian), 0 = 4x8, 1 = 4x4, 2 = 2x4
+                                (Epanechnikov) bandwidth pairs per thread */
+#define BK_CVG_TUNE_MAX   9  /* most candidate schedules of bk_cv_grid_tune:
+                                BK_CVG_VARIANTS x at most 3 unrolls */
 #define BK_CVG_COUNT_UNITS 3 /* 0 (default); 1: count the (query, reference,
                                 bandwidth pair) units the pair kernel
                                 evaluates (runs launch eagerly while set;
@@ -414,11 +416,12 @@ int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
                            double *total_ms, long long *rescued);
 
 /* Units of the last run evaluated by the pair kernel, when it ran with
- * BK_CVG_COUNT_UNITS set: for BK_EPANECHNIKOV_PRODUCT_2D the units of the
- * 256-reference stages it did not skip (a skipped stage lies beyond the
- * support of every bandwidth of its sub-grid, its terms exactly zero), for
- * BK_GAUSSIAN_PRODUCT_2D every unit (sum of n_d^2 times n_hx * n_hy; the
- * Gaussian kernel skips none).  Self pairs are counted as evaluated.  No
+ * BK_CVG_COUNT_UNITS set: the units of the 256-reference stages it did not
+ * skip.  An Epanechnikov stage is skipped when it lies beyond the support of
+ * every bandwidth of its sub-grid (its terms exactly zero), a Gaussian stage
+ * when every product of its x and y factors is below 2^-152 for every query
+ * of the work item (each would add nothing to its FP32 stage sum).  Self
+ * pairs are counted as evaluated.  No
  * counterpart in the reference, whose cross-validation evaluates every pair
  * a tree traversal does not prune. */
 int bk_cv_grid_unit_count(const bk_cv_grid *plan, unsigned long long *evaluated);
@@ -432,7 +435,8 @@ int bk_cv_grid_unit_count(const bk_cv_grid *plan, unsigned long long *evaluated)
  * it runs the plan's whole grid once per candidate schedule (eagerly, the
  * pair kernel timed with HIP events), writes each candidate's pair-kernel
  * milliseconds to ms_out (BK_CVG_VARIANTS x unroll candidates, tiling-major;
- * the count goes to n_candidates_out; either may be NULL; at most 8) and
+ * the count goes to n_candidates_out; either may be NULL; at most
+ * BK_CVG_TUNE_MAX) and
  * records the fastest.  It fails when BK_CVG_VARIANT forces a tiling.
  * bk_tune_set_cache names a file
  * that the cache is loaded from now and saved to after every new choice

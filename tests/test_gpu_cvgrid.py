@@ -207,12 +207,14 @@ def test_errors(bk):
 
 
 def test_unit_count(bk):
-    """BK_CVG_COUNT_UNITS: the Gaussian kernel evaluates every (query,
-    reference, bandwidth pair) unit; the Epanechnikov kernel skips only
-    stages whose terms are all zero, so its count lies between the units
-    inside the convolution kernel's support (|dx| < 2 hx and |dy| < 2 hy,
-    which it must evaluate) and all units.  Counting does not change the
-    scores."""
+    """BK_CVG_COUNT_UNITS: the Epanechnikov kernel skips only stages whose
+    terms are all zero, so its count lies between the units inside the
+    convolution kernel's support (|dx| < 2 hx and |dy| < 2 hy, which it must
+    evaluate) and all units.  The Gaussian kernel skips only stages whose
+    products are all below 2^-152 (each would leave its FP32 sum unchanged):
+    with small-bandwidth sub-grids (the 4 x 4 tiling) it skips some, and its
+    scores stay bit-identical to every other tiling's.  Counting does not
+    change the scores."""
     sets = [mixture(3000, 621), mixture(900, 622)]
     hx = np.geomspace(0.03, 0.4, 6)
     hy = np.geomspace(0.02, 0.5, 9)
@@ -244,9 +246,39 @@ def test_unit_count(bk):
         print(f"[cvgrid] kind {kind}: {units} of {total} units evaluated ({units / total:.1%}), "
               f"{inside} inside the support")
         if kind == GAUSSPROD:
-            assert units == total
+            assert 0 < units <= total
         else:
             assert inside <= units < total
+
+
+def test_gaussian_stages_skipped_without_changing_a_bit(bk):
+    """The Gaussian kernel's 4 x 4 tiling skips stages whose products are all
+    below 2^-152 (small bandwidths, points far apart in Z order); its LOO
+    sums and scores must equal bit for bit those of the 8 x 8 tiling, which
+    tests no stage, and of the 4 x 8 one."""
+    rng = np.random.default_rng(651)
+    pts = rng.uniform(-4.0, 4.0, (40_000, 2))
+    w = np.full(len(pts), 1.0 / len(pts))
+    hx = np.geomspace(0.004, 0.03, 4)
+    hy = np.geomspace(0.005, 0.025, 4)
+    total = len(pts) ** 2 * len(hx) * len(hy)
+    with bk.CvGrid(GAUSSPROD, [pts], [w]) as cv:
+        cv.set_option(bk.BK_CVG_VARIANT, 0)
+        base = cv.run(hx, hy)
+        loo0 = cv.loo(0, 1, 2)
+        cv.set_option(bk.BK_CVG_VARIANT, 1)
+        mid = cv.run(hx, hy)
+        cv.set_option(bk.BK_CVG_VARIANT, 2)
+        cv.set_option(bk.BK_CVG_COUNT_UNITS, 1)
+        small = cv.run(hx, hy)
+        units = cv.unit_count()
+        loo2 = cv.loo(0, 1, 2)
+    for r in (mid, small):
+        np.testing.assert_array_equal(r[0], base[0])
+        np.testing.assert_array_equal(r[1], base[1])
+    np.testing.assert_array_equal(loo2, loo0)
+    print(f"[cvgrid] Gaussian 4x4 sub-grids: {units} of {total} units evaluated ({units / total:.1%})")
+    assert 0 < units < total // 4
 
 
 @pytest.mark.parametrize("kind", [GAUSSPROD, EPANPROD])
