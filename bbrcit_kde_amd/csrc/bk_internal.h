@@ -161,9 +161,13 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
                         int self_excl, int mode, hipStream_t s, const BoundArgs *ba = nullptr);
 
 // FP64 Epanechnikov-family path, D <= 2, on FP32 point data ----------------
-// tb: 2 double4 per aligned tile of kTile references (box and smallest
-// scales, for workgroup-level culling).
-constexpr int kEpanTileMeta = 2;
+// tb: kEpanTileMeta double4 per aligned tile of kTile references: [0] the
+// FP32 coordinate box (x0, x1, y0, y1), [1] the smallest scales and largest
+// |offsets| (sx, sy, ox, oy) -- both for workgroup-level culling -- then the
+// reach boxes (L_x, H_x, L_y, H_y) of the tile [2] and of its kTile / kEpanSub
+// sub-tiles [3..], for the wavefront-level tests.
+constexpr int kEpanSub = 64;  // references per wavefront stage
+constexpr int kEpanTileMeta = 3 + kTile / kEpanSub;
 hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
                             const float *m, const float *abw, size_t n,
                             double4 *f0, double4 *f1, double4 *tb, hipStream_t s);

@@ -24,6 +24,9 @@
 //     compact support makes 1 - d2/(h a)^2 cancel near the support edge,
 //     which FP32 cannot resolve to 1e-5; each reference carries its scale
 //     s = 1/(h a) and offset -s r per axis, so u = s (q - r) is one FP64 FMA;
+//     each wavefront walks the references on its own in sub-tiles of 64
+//     (its own LDS slot, no workgroup barrier) and passes over every
+//     sub-tile whose terms are exactly zero for all its queries;
 //   * large reference ranges are split over workgroups at tile boundaries,
 //     with a fixed-order FP64 reduction of the partial sums, so results are
 //     deterministic.
@@ -893,18 +896,15 @@ __device__ __forceinline__ double epan_term(double qx, double qy, double4 a, dou
 // G - 2^-50 (1 + o_max), while the computed |u| >= G (1 - 2^-53) -
 // 2^-52 o_max.  So u_lower >= c (c >= 1) gives |u| > c, and u_lower <= 4
 // gives |u| >= u_lower.
-__device__ __forceinline__ double u_lower(double r0, double r1, float b0, float b1, double s_min, double o_max) {
-  const double gap = fmax(0.0, fmax(r0 - (double)b1, (double)b0 - r1));
+__device__ __forceinline__ double u_lower(double r0, double r1, double b0, double b1, double s_min, double o_max) {
+  const double gap = fmax(0.0, fmax(r0 - b1, b0 - r1));
   return fmax(0.0, __builtin_fma(s_min * gap, 1.0 - 0x1p-50, -0x1p-50 * (1.0 + o_max)));
 }
 
+// Every term is exactly zero once the lower bounds lx, ly of |ux|, |uy|
+// reach the support edge of the family.
 template <int FAM>
-__device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, long long tile_idx, float bx0, float bx1,
-                                              float by0, float by1) {
-  const double4 b = tb[2 * tile_idx];
-  const double4 w = tb[2 * tile_idx + 1];
-  const double lx = u_lower(b.x, b.y, bx0, bx1, w.x, w.z);
-  const double ly = u_lower(b.z, b.w, by0, by1, w.y, w.w);
+__device__ __forceinline__ bool epan_beyond_support(double lx, double ly) {
   if constexpr (FAM == FAM_EPAN) {
     // t = fma(-ux, ux, fma(-uy, uy, 1)) <= 0 once ux^2 + uy^2 >= 1 + 2^-52;
     // ux^2 + uy^2 >= lx^2 + ly^2 when both bounds are <= 4
@@ -917,6 +917,59 @@ __device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, lo
   }
 }
 
+template <int FAM>
+__device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, long long tile_idx, float bx0, float bx1,
+                                              float by0, float by1) {
+  const double4 b = tb[kEpanTileMeta * tile_idx];
+  const double4 w = tb[kEpanTileMeta * tile_idx + 1];
+  return epan_beyond_support<FAM>(u_lower(b.x, b.y, bx0, bx1, w.x, w.z), u_lower(b.z, b.w, by0, by1, w.y, w.w));
+}
+
+// Reach box (L_x, H_x, L_y, H_y) of a tile or sub-tile: a query with
+// q_x <= L_x or q_x >= H_x (likewise on y) has, as computed, |u| > c on that
+// axis with every reference of it, c being the support edge of |u| on one
+// axis (2 for the convolution kernel, else 1), so each of its terms there is
+// exactly zero (for the radial kernel too: ux^2 + uy^2 > 1 then).  With s_min
+// the smallest scale and o_max the largest |o|, |s q + o| >= s_min |q - r| -
+// 2^-52 o_max, so a reach r = c (1 + 2^-20 + 2^-50 o_max) / s_min past the
+// box suffices; the bounds carry their own rounding outward.
+__device__ __forceinline__ double4 epan_reach_box(float x0, float x1, float y0, float y1, float sx, float sy,
+                                                  float ox, float oy, double c) {
+  const double rx = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)ox) / (double)sx;
+  const double ry = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)oy) / (double)sy;
+  auto below = [](double lo, double r) { return (lo - r) - 0x1p-50 * (fabs(lo) + r); };
+  auto above = [](double hi, double r) { return (hi + r) + 0x1p-50 * (fabs(hi) + r); };
+  return make_double4(below(x0, rx), above(x1, rx), below(y0, ry), above(y1, ry));
+}
+
+// True when no query of the wavefront is inside the reach box e on both
+// axes: every term of the (sub-)tile is then exactly zero for all its
+// queries, and passing over it leaves the sums bit-identical.  A NaN
+// coordinate compares false and keeps the tile.
+template <int NQ>
+__device__ __forceinline__ bool epan_wave_far(double4 e, const double *px, const double *py) {
+  bool near = false;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) near |= !((px[k] <= e.x) | (px[k] >= e.y) | (py[k] <= e.z) | (py[k] >= e.w));
+  return !__any(near);
+}
+
+// Wavefront-scope barrier: orders this wavefront's LDS writes before its
+// later reads (and the reads before later writes) in the compiled code; the
+// hardware keeps one wavefront's LDS operations in order.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One workgroup, kQpt queries per thread.  Each wavefront walks the
+// reference range on its own in sub-tiles of kEpanSub references (one per
+// lane), staged in its own LDS slot, the next needed sub-tile prefetched
+// into registers: tiles culled for the whole workgroup (or not listed in a
+// bounded evaluation) are passed over first, then tiles and sub-tiles whose
+// reach box holds none of the wavefront's queries.  Terms are summed in
+// reference order, so the sums equal those of every pair in that order.
 template <int FAM, bool BOUNDED>
 __device__ __forceinline__ void epan_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
@@ -924,10 +977,11 @@ __device__ __forceinline__ void epan_item(
     long long r_begin, long long r_end, long long q_begin, int q_count,
     double *__restrict__ dst, double out_scale, int self_excl, const unsigned long long *__restrict__ bmask) {
   constexpr int kNf = EpanNf<FAM>::kNf;
-  __shared__ double4 s0[2][kTile];
-  __shared__ double4 s1[2][kNf == 2 ? kTile : 1];
-  __shared__ float sbox[4][kBlock / 64];
-  const int t = threadIdx.x;
+  constexpr int kWaves = kBlock / 64;
+  __shared__ double4 s0[kWaves][kEpanSub];
+  __shared__ double sm[kWaves][kNf == 2 ? kEpanSub : 1];
+  __shared__ float sbox[4][kWaves];
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
     return;
@@ -949,48 +1003,69 @@ __device__ __forceinline__ void epan_item(
       if (!epan_tile_far<FAM>(tb, tl / kTile, bx0, bx1, by0, by1)) break;
     return tl;
   };
+  // first sub-tile at or after `sub` (a multiple of kEpanSub) that this
+  // wavefront evaluates, or r_end; `whole` = the tile-level tests of sub's
+  // tile are still to be made
+  auto next_needed = [&](long long sub, bool whole) __attribute__((always_inline)) {
+    for (;;) {
+      if (sub >= r_end) return r_end;
+      if (whole) {
+        const long long tl0 = (sub / kTile) * kTile;
+        const long long tl = live_from(tl0);
+        if (tl >= r_end) return r_end;
+        if (tl != tl0) sub = tl;
+        if (epan_wave_far<kQpt>(tb[kEpanTileMeta * (tl / kTile) + 2], px, py)) {
+          sub = tl + kTile;
+          continue;
+        }
+        whole = false;
+      }
+      const long long ti = sub / kTile;
+      const int si = (int)((sub - ti * kTile) / kEpanSub);
+      if (!epan_wave_far<kQpt>(tb[kEpanTileMeta * ti + 3 + si], px, py)) return sub;
+      sub += kEpanSub;
+      whole = (sub % kTile) == 0;
+    }
+  };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
   const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
-  long long tile = live_from((r_begin / kTile) * kTile);
-  double4 n0 = zero, n1 = zero;
-  if (tile < r_end) {
-    const long long r = tile + t;
-    if (r >= r_begin && r < r_end) {
-      n0 = f0[r];
-      if constexpr (kNf == 2) n1 = f1[r];
-    }
+  // this lane's reference of a sub-tile (zero features outside the range
+  // give exact zero terms)
+  auto in_range = [&](long long sub) { return sub + lane >= r_begin && sub + lane < r_end; };
+  long long cur = next_needed((r_begin / kEpanSub) * kEpanSub, true);
+  double4 n0 = zero;
+  double nm = 0.0;
+  if (cur < r_end && in_range(cur)) {
+    n0 = f0[cur + lane];
+    if constexpr (kNf == 2) nm = f1[cur + lane].x;
   }
-  int buf = 0;
-  long long next_tile = r_end;
-  for (; tile < r_end; tile = next_tile) {
-    s0[buf][t] = n0;
-    if constexpr (kNf == 2) s1[buf][t] = n1;
-    __syncthreads();
-    {
-      next_tile = live_from(tile + kTile);
-      const long long r = next_tile + t;
-      n0 = zero;
-      n1 = zero;
-      if (r < r_end) {
-        n0 = f0[r];
-        if constexpr (kNf == 2) n1 = f1[r];
-      }
+  while (cur < r_end) {
+    wave_sync();
+    s0[wv][lane] = n0;
+    if constexpr (kNf == 2) sm[wv][lane] = nm;
+    wave_sync();
+    const long long nxt = next_needed(cur + kEpanSub, ((cur + kEpanSub) % kTile) == 0);
+    n0 = zero;
+    nm = 0.0;
+    if (nxt < r_end && in_range(nxt)) {
+      n0 = f0[nxt + lane];
+      if constexpr (kNf == 2) nm = f1[nxt + lane].x;
     }
-    const int cnt = (int)((r_end - tile) < kTile ? (r_end - tile) : kTile);
-    const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
+    const int cnt = (int)((r_end - cur) < kEpanSub ? (r_end - cur) : kEpanSub);
+    const bool diag = self_excl && cur < q_hi && q_lo < cur + cnt;
     if (!diag) {
 #pragma unroll 2
       for (int j = 0; j < cnt; ++j) {
-        const double4 a = s0[buf][j];
-        const double4 b = (kNf == 2) ? s1[buf][j] : a;
+        const double4 a = s0[wv][j];
+        const double4 b = make_double4(kNf == 2 ? sm[wv][j] : 0.0, 0.0, 0.0, 0.0);
 #pragma unroll
         for (int k = 0; k < kQpt; ++k) acc[k] += epan_term<FAM>(px[k], py[k], a, b);
       }
     } else {
       for (int j = 0; j < cnt; ++j) {
-        const double4 a = s0[buf][j];
-        const double4 b = (kNf == 2) ? s1[buf][j] : a;
-        const long long rj = tile + j;
+        const double4 a = s0[wv][j];
+        const double4 b = make_double4(kNf == 2 ? sm[wv][j] : 0.0, 0.0, 0.0, 0.0);
+        const long long rj = cur + j;
 #pragma unroll
         for (int k = 0; k < kQpt; ++k) {
           const double v = epan_term<FAM>(px[k], py[k], a, b);
@@ -998,7 +1073,7 @@ __device__ __forceinline__ void epan_item(
         }
       }
     }
-    buf ^= 1;
+    cur = nxt;
   }
 #pragma unroll
   for (int k = 0; k < kQpt; ++k) {
@@ -1068,11 +1143,14 @@ __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
 }
 
 // One workgroup of kTile threads per aligned tile: the tile's box, smallest
-// scales and largest offsets (see epan_tile_far).
+// scales and largest offsets (see epan_tile_far), and the reach boxes of the
+// tile and of each wavefront's sub-tile of kEpanSub references (see
+// epan_reach_box); c = the support edge of |u| on one axis.
 __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__restrict__ x,
                                                                  const float *__restrict__ y,
                                                                  const double4 *__restrict__ f0, long long n,
-                                                                 int iso, double4 *__restrict__ tb) {
+                                                                 int iso, double c, double4 *__restrict__ tb) {
+  static_assert(kEpanSub == 64, "one sub-tile per wavefront");
   __shared__ float sh[kTile];
   const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
   const bool valid = i < n;
@@ -1102,10 +1180,20 @@ __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__r
   const float sy = block_reduce_f<kTile, false>(valid ? down(syv) : big, sh);
   const float ox = block_reduce_f<kTile, true>(valid ? up(oxv) : 0.0f, sh);
   const float oy = block_reduce_f<kTile, true>(valid ? up(oyv) : 0.0f, sh);
+  double4 *m = tb + kEpanTileMeta * blockIdx.x;
   if (threadIdx.x == 0) {
-    tb[2 * blockIdx.x] = make_double4(x0, x1, y0, y1);
-    tb[2 * blockIdx.x + 1] = make_double4(sx, sy, ox, oy);
+    m[0] = make_double4(x0, x1, y0, y1);
+    m[1] = make_double4(sx, sy, ox, oy);
+    m[2] = epan_reach_box(x0, x1, y0, y1, sx, sy, ox, oy, c);
   }
+  const float w_x0 = wave_reduce_f(valid ? xi : big, false), w_x1 = wave_reduce_f(valid ? xi : -big, true);
+  const float w_y0 = wave_reduce_f(valid ? yi : big, false), w_y1 = wave_reduce_f(valid ? yi : -big, true);
+  const float w_sx = wave_reduce_f(valid ? down(sxv) : big, false);
+  const float w_sy = wave_reduce_f(valid ? down(syv) : big, false);
+  const float w_ox = wave_reduce_f(valid ? up(oxv) : 0.0f, true);
+  const float w_oy = wave_reduce_f(valid ? up(oyv) : 0.0f, true);
+  if ((threadIdx.x & 63) == 0)
+    m[3 + (threadIdx.x >> 6)] = epan_reach_box(w_x0, w_x1, w_y0, w_y1, w_sx, w_sy, w_ox, w_oy, c);
 }
 
 // ---- FP64 context path ---------------------------------------------------
@@ -2233,8 +2321,8 @@ hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
   prep_epan_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, x, y, m, abw, (long long)n, f0, f1);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  prep_epan_tiles_kernel<<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(x, y, f0, (long long)n,
-                                                                                p.family == FAM_EPAN ? 1 : 0, tb);
+  prep_epan_tiles_kernel<<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(
+      x, y, f0, (long long)n, p.family == FAM_EPAN ? 1 : 0, p.family == FAM_EPANCONV ? 2.0 : 1.0, tb);
   return hipGetLastError();
 }
 
