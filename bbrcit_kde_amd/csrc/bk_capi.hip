@@ -431,7 +431,8 @@ Path path_of(const bk_ctx *ctx, const EvalParams &p) {
 // Queries handled by one workgroup / work item on a path.
 int queries_per_item(Path path) {
   if (path == Path::GAUSS32) return bk::kGaussQueriesPerBlock;
-  return (path == Path::EPAN64 || path == Path::F64) ? bk::kQueriesPerBlock : 256;
+  if (path == Path::EPAN64) return bk::kEpanQueriesPerBlock;
+  return path == Path::F64 ? bk::kQueriesPerBlock : 256;
 }
 
 int tile_of(Path path) {
@@ -468,7 +469,7 @@ FeatBytes feature_bytes(Path path, size_t n) {
       break;
     case Path::EPAN64:
       b.f0 = b.f1 = n * sizeof(double4);
-      b.tiles = (n + bk::kTile - 1) / bk::kTile * bk::kEpanTileMeta * sizeof(double4);
+      b.tiles = (size_t)bk::epan_meta_tiles((long long)n) * bk::kEpanTileMeta * sizeof(double4);
       break;
     case Path::GENERAL:
       b.f0 = n * sizeof(double2);
@@ -613,7 +614,7 @@ hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const FeatPt
     case Path::EPAN64:
       return bk::launch_epan(p, q.x.as<float>(), q.y.as<float>(), static_cast<const double4 *>(fp.f0),
                              static_cast<const double4 *>(fp.f1), static_cast<const double4 *>(fp.tiles), items,
-                             n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s, order);
+                             n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, r.uniform_abw, s, order);
     case Path::GRAM:
       return bk::launch_gram(p, q.coords.as<float>(), r.coords.as<float>(), static_cast<const double4 *>(fp.f0),
                              items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s);
@@ -725,7 +726,8 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                            ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, op);
     else
       e = bk::launch_epan(p, ox, oy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
-                          nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s, op);
+                          nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, ctx->refs.uniform_abw, s,
+                          op);
     hip_check(e, "direct kernel launch");
     if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, res, s), "reduce launch");
     hip_check(bk::launch_scatter_f64(list, nq, res, out, s), "scatter launch");
@@ -802,7 +804,9 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
   const double factor = normalize ? p.norm : 1.0;  // mass units -> output units
   const double abs_u = normalize ? abs_err / p.norm : abs_err;
   const double m_total = ctx->refs.m_sum > 0.0 ? ctx->refs.m_sum : 1.0;
-  const int qpb = queries_per_item(path);
+  // tiles are listed per block of qpb queries (kQueriesPerBlock on the
+  // Epanechnikov path: its launch gives each listed block several workgroups)
+  const int qpb = path == Path::EPAN64 ? bk::kQueriesPerBlock : queries_per_item(path);
   // queries are processed in chunks whose tile masks stay within 2^29 words (4 GiB)
   const long long chunk_blocks = std::max<long long>(1, (1LL << 29) / nwords);
   const long long chunk_q = chunk_blocks * qpb;
@@ -904,7 +908,7 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
                            ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? bk::GAUSS_DIRECT : bk::GAUSS_AUTO, s, &ba);
     else if (path == Path::EPAN64)
       e = bk::launch_epan(p, qx32, qy32, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
-                          nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, s, &ba);
+                          nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, ctx->refs.uniform_abw, s, &ba);
     else
       e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(),
                                 ctx->tiles.as<double4>(), nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, s,
@@ -1040,7 +1044,7 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
                        ctx->gauss_form == BK_GAUSS_FORM_DIRECT ? GAUSS_DIRECT : GAUSS_AUTO, s, op);
     else
       e = launch_epan(p, qx, qy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(), nullptr, 0,
-                      0, nr, 0, (long long)nq, splits, dst, scale, 0, s, op);
+                      0, nr, 0, (long long)nq, splits, dst, scale, 0, ctx->refs.uniform_abw, s, op);
     hip_check(e, "direct kernel launch");
     if (splits > 1) hip_check(launch_reduce_splits(dst, splits, (long long)nq, out, s), "reduce launch");
   } catch (...) {

@@ -59,9 +59,23 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
 #ifndef BK_GAUSS_UNROLL
 #define BK_GAUSS_UNROLL 2
 #endif
-constexpr int kQpt = BK_QPT;       // queries held in registers per thread (FP64 kernels)
+#ifndef BK_EPAN_QPT
+#define BK_EPAN_QPT 1
+#endif
+#ifndef BK_EPAN_UNROLL
+#define BK_EPAN_UNROLL 2
+#endif
+constexpr int kQpt = BK_QPT;       // queries held in registers per thread (FP64-context kernels)
+// queries per thread of the Epanechnikov kernels on FP32 points: one, so a
+// wavefront's 64 queries pass over every sub-tile none of them reaches
+constexpr int kEpanQpt = BK_EPAN_QPT;
+constexpr int kEpanQueriesPerBlock = kBlock * kEpanQpt;
 constexpr int kGaussQpt = BK_GAUSS_QPT;  // queries per thread of the FP32 Gaussian kernel
 constexpr int kQueriesPerBlock = kBlock * kQpt;
+// A bounded evaluation lists tiles per block of kQueriesPerBlock queries on
+// the Epanechnikov path too; that many workgroups share one block's list.
+constexpr int kEpanBoundParts = kQueriesPerBlock / kEpanQueriesPerBlock;
+static_assert(kEpanBoundParts * kEpanQueriesPerBlock == kQueriesPerBlock, "whole workgroups per listed block");
 constexpr int kGaussQueriesPerBlock = kBlock * kGaussQpt;
 constexpr int kTile = 256;         // references per LDS stage (tiles are aligned to kTile)
 #ifndef BK_GAUSS_TILE
@@ -163,11 +177,18 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
 // FP64 Epanechnikov-family path, D <= 2, on FP32 point data ----------------
 // tb: kEpanTileMeta double4 per aligned tile of kTile references: [0] the
 // FP32 coordinate box (x0, x1, y0, y1), [1] the smallest scales and largest
-// |offsets| (sx, sy, ox, oy) -- both for workgroup-level culling -- then the
-// reach boxes (L_x, H_x, L_y, H_y) of the tile [2] and of its kTile / kEpanSub
-// sub-tiles [3..], for the wavefront-level tests.
+// |offsets| (sx, sy, ox, oy) -- both for the culling of a query box -- then,
+// from [2] on, 1 + kTile / kEpanSub float4: the FP32 reach boxes
+// (L_x, H_x, L_y, H_y) of the tile and of its sub-tiles, for the
+// wavefront-level tests.
+// The tiles carry metadata up to a multiple of kEpanScanBatch (tiles past the
+// points hold empty boxes, out of reach of every query).
 constexpr int kEpanSub = 64;  // references per wavefront stage
-constexpr int kEpanTileMeta = 3 + kTile / kEpanSub;
+constexpr int kEpanTileMeta = 2 + (1 + kTile / kEpanSub + 1) / 2;
+constexpr int kEpanScanBatch = 8;  // tile-level reach tests whose loads are issued together
+inline long long epan_meta_tiles(long long n) {
+  return (n + (long long)kTile * kEpanScanBatch - 1) / ((long long)kTile * kEpanScanBatch) * kEpanScanBatch;
+}
 hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
                             const float *m, const float *abw, size_t n,
                             double4 *f0, double4 *f1, double4 *tb, hipStream_t s);
@@ -175,7 +196,7 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
                        const double4 *f0, const double4 *f1, const double4 *tb,
                        const WorkItem *items, int n_items, long long r_i,
                        long long r_end, long long q_i, long long nq, int splits,
-                       double *dst, double out_scale, int self_excl,
+                       double *dst, double out_scale, int self_excl, bool uniform_scales,
                        hipStream_t s, const BoundArgs *ba = nullptr);
 
 // FP64 context path, D <= 2 ------------------------------------------------

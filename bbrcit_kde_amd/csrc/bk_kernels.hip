@@ -932,26 +932,47 @@ __device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, lo
 // exactly zero (for the radial kernel too: ux^2 + uy^2 > 1 then).  With s_min
 // the smallest scale and o_max the largest |o|, |s q + o| >= s_min |q - r| -
 // 2^-52 o_max, so a reach r = c (1 + 2^-20 + 2^-50 o_max) / s_min past the
-// box suffices; the bounds carry their own rounding outward.
-__device__ __forceinline__ double4 epan_reach_box(float x0, float x1, float y0, float y1, float sx, float sy,
-                                                  float ox, float oy, double c) {
+// box suffices.  The bounds carry their own rounding outward and are then
+// rounded outward to FP32, the precision of the query coordinates.
+__device__ __forceinline__ float4 epan_reach_box(float x0, float x1, float y0, float y1, float sx, float sy,
+                                                 float ox, float oy, double c) {
   const double rx = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)ox) / (double)sx;
   const double ry = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)oy) / (double)sy;
-  auto below = [](double lo, double r) { return (lo - r) - 0x1p-50 * (fabs(lo) + r); };
-  auto above = [](double hi, double r) { return (hi + r) + 0x1p-50 * (fabs(hi) + r); };
-  return make_double4(below(x0, rx), above(x1, rx), below(y0, ry), above(y1, ry));
+  // largest FP32 value <= v, smallest >= v (v finite; the float nearest v
+  // is then finite or an infinity of v's sign, so the step stays ordered)
+  auto below = [](double lo, double r) {
+    const double v = (lo - r) - 0x1p-50 * (fabs(lo) + r);
+    const float f = (float)v;
+    if (!((double)f > v)) return f;
+    return f == 0.0f ? -0x1p-149f : __uint_as_float(__float_as_uint(f) + (f > 0.0f ? -1 : 1));
+  };
+  auto above = [](double hi, double r) {
+    const double v = (hi + r) + 0x1p-50 * (fabs(hi) + r);
+    const float f = (float)v;
+    if (!((double)f < v)) return f;
+    return f == 0.0f ? 0x1p-149f : __uint_as_float(__float_as_uint(f) + (f > 0.0f ? 1 : -1));
+  };
+  return make_float4(below(x0, rx), above(x1, rx), below(y0, ry), above(y1, ry));
+}
+
+// Reach boxes of tile ti: [0] the tile's, [1 + s] that of its sub-tile s.
+__device__ __forceinline__ const float4 *epan_reach(const double4 *__restrict__ tb, long long ti) {
+  return reinterpret_cast<const float4 *>(tb + kEpanTileMeta * ti + 2);
 }
 
 // True when no query of the wavefront is inside the reach box e on both
 // axes: every term of the (sub-)tile is then exactly zero for all its
 // queries, and passing over it leaves the sums bit-identical.  A NaN
 // coordinate compares false and keeps the tile.
+// (The comparisons are taken as lane masks, so the test is four compares
+// per query and scalar mask operations.)
 template <int NQ>
-__device__ __forceinline__ bool epan_wave_far(double4 e, const double *px, const double *py) {
-  bool near = false;
+__device__ __forceinline__ bool epan_wave_far(float4 e, const float *fx, const float *fy) {
+  unsigned long long far = ~0ull;
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) near |= !((px[k] <= e.x) | (px[k] >= e.y) | (py[k] <= e.z) | (py[k] >= e.w));
-  return !__any(near);
+  for (int k = 0; k < NQ; ++k)
+    far &= __ballot(fx[k] <= e.x) | __ballot(fx[k] >= e.y) | __ballot(fy[k] <= e.z) | __ballot(fy[k] >= e.w);
+  return (__builtin_amdgcn_read_exec() & ~far) == 0ull;
 }
 
 // Wavefront-scope barrier: orders this wavefront's LDS writes before its
@@ -963,111 +984,164 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One workgroup, kQpt queries per thread.  Each wavefront walks the
-// reference range on its own in sub-tiles of kEpanSub references (one per
-// lane), staged in its own LDS slot, the next needed sub-tile prefetched
-// into registers: tiles culled for the whole workgroup (or not listed in a
-// bounded evaluation) are passed over first, then tiles and sub-tiles whose
-// reach box holds none of the wavefront's queries.  Terms are summed in
-// reference order, so the sums equal those of every pair in that order.
-template <int FAM, bool BOUNDED>
+// One workgroup, kEpanQpt queries per thread.  Each wavefront walks the
+// reference range on its own, with no workgroup barrier: it tests the reach
+// boxes of kEpanScanBatch tiles at a time (and, for a tile in reach, those of its
+// sub-tiles), stages each sub-tile of kEpanSub references in reach in its own
+// LDS slot, one reference per lane, and prefetches the next into registers.
+// Terms are summed in reference order, so the sums equal those over every
+// pair in that order.  UNI: every reference has the scales (usx, usy) (no
+// adaptive bandwidths), so only the offsets and the mass are staged.  In a
+// bounded evaluation only the tiles listed in bmask are visited.
+template <int FAM, bool BOUNDED, bool UNI>
 __device__ __forceinline__ void epan_item(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_begin, long long r_end, long long q_begin, int q_count,
-    double *__restrict__ dst, double out_scale, int self_excl, const unsigned long long *__restrict__ bmask) {
+    double *__restrict__ dst, double out_scale, int self_excl, const unsigned long long *__restrict__ bmask,
+    double usx, double usy) {
   constexpr int kNf = EpanNf<FAM>::kNf;
   constexpr int kWaves = kBlock / 64;
-  __shared__ double4 s0[kWaves][kEpanSub];
-  __shared__ double sm[kWaves][kNf == 2 ? kEpanSub : 1];
-  __shared__ float sbox[4][kWaves];
+  constexpr int kSubs = kTile / kEpanSub;
+  // the staged features: all of f0 (and the mass of f1), or with UNI the
+  // offsets (o_x, o_y) and the mass
+  __shared__ double4 s0[kWaves][UNI ? 1 : kEpanSub];
+  __shared__ double2 so[kWaves][UNI ? kEpanSub : 1];
+  __shared__ double sm[kWaves][(kNf == 2 || UNI) ? kEpanSub : 1];
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
   if (r_begin >= r_end) {
     write_zeros(dst, q_count);
     return;
   }
-  double px[kQpt], py[kQpt], acc[kQpt];
+  double px[kEpanQpt], py[kEpanQpt], acc[kEpanQpt];
+  float fx[kEpanQpt], fy[kEpanQpt];
 #pragma unroll
-  for (int k = 0; k < kQpt; ++k) {
+  for (int k = 0; k < kEpanQpt; ++k) {
     const int qi = t + k * kBlock;
     const long long q = q_begin + (qi < q_count ? qi : 0);
-    px[k] = (double)qx[q];
-    py[k] = (double)qy[q];
+    fx[k] = qx[q];
+    fy[k] = qy[q];
+    px[k] = (double)fx[k];
+    py[k] = (double)fy[k];
     acc[k] = 0.0;
   }
-  float bx0, bx1, by0, by1;
-  block_query_box<kQpt>(px, py, sbox, bx0, bx1, by0, by1);
-  auto live_from = [&](long long tl) {
-    if constexpr (BOUNDED) return next_listed_tile<kTile>(bmask, tl, r_end);
-    for (; tl < r_end; tl += kTile)
-      if (!epan_tile_far<FAM>(tb, tl / kTile, bx0, bx1, by0, by1)) break;
-    return tl;
-  };
+  long long batch = -1, stile = -1;
+  unsigned near_t = 0, near_s = 0;
   // first sub-tile at or after `sub` (a multiple of kEpanSub) that this
-  // wavefront evaluates, or r_end; `whole` = the tile-level tests of sub's
-  // tile are still to be made
-  auto next_needed = [&](long long sub, bool whole) __attribute__((always_inline)) {
+  // wavefront evaluates, or r_end
+  auto next_needed = [&](long long sub) __attribute__((always_inline)) {
     for (;;) {
       if (sub >= r_end) return r_end;
-      if (whole) {
-        const long long tl0 = (sub / kTile) * kTile;
-        const long long tl = live_from(tl0);
-        if (tl >= r_end) return r_end;
-        if (tl != tl0) sub = tl;
-        if (epan_wave_far<kQpt>(tb[kEpanTileMeta * (tl / kTile) + 2], px, py)) {
-          sub = tl + kTile;
-          continue;
+      long long ti = sub / kTile;
+      const long long b = ti - ti % kEpanScanBatch;
+      if (b != batch) {
+        unsigned listed = (1u << kEpanScanBatch) - 1;
+        if constexpr (BOUNDED) {
+          listed &= (unsigned)(bmask[b >> 6] >> (b & 63));
+          if (listed == 0) {  // no listed tile in this batch: on to the next listed one
+            sub = next_listed_tile<kTile>(bmask, (b + kEpanScanBatch) * kTile, r_end);
+            continue;
+          }
         }
-        whole = false;
+        batch = b;
+        float4 e[kEpanScanBatch];
+#pragma unroll
+        for (int i = 0; i < kEpanScanBatch; ++i) e[i] = epan_reach(tb, b + i)[0];
+        unsigned m = 0;
+#pragma unroll
+        for (int i = 0; i < kEpanScanBatch; ++i) m |= epan_wave_far<kEpanQpt>(e[i], fx, fy) ? 0u : 1u << i;
+        near_t = m & listed;
       }
-      const long long ti = sub / kTile;
-      const int si = (int)((sub - ti * kTile) / kEpanSub);
-      if (!epan_wave_far<kQpt>(tb[kEpanTileMeta * ti + 3 + si], px, py)) return sub;
-      sub += kEpanSub;
-      whole = (sub % kTile) == 0;
+      const unsigned rest = near_t >> (int)(ti - b);
+      if (rest == 0) {
+        sub = (b + kEpanScanBatch) * kTile;
+        continue;
+      }
+      const int skip = __builtin_ctz(rest);
+      if (skip) {
+        ti += skip;
+        sub = ti * kTile;
+      }
+      if (ti != stile) {
+        stile = ti;
+        const float4 *er = epan_reach(tb, ti) + 1;
+        float4 e[kSubs];
+#pragma unroll
+        for (int i = 0; i < kSubs; ++i) e[i] = er[i];
+        unsigned m = 0;
+#pragma unroll
+        for (int i = 0; i < kSubs; ++i) m |= epan_wave_far<kEpanQpt>(e[i], fx, fy) ? 0u : 1u << i;
+        near_s = m;
+      }
+      const unsigned srest = near_s >> (int)((sub - ti * kTile) / kEpanSub);
+      if (srest == 0) {
+        sub = (ti + 1) * kTile;
+        continue;
+      }
+      return sub + (long long)__builtin_ctz(srest) * kEpanSub;
+    }
+  };
+  // this lane's reference of a sub-tile (zero features outside the range
+  // give exact zero terms)
+  const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 n0 = zero;
+  double nm = 0.0;
+  auto load_ref = [&](long long sub) __attribute__((always_inline)) {
+    const long long r = sub + lane;
+    n0 = zero;
+    nm = 0.0;
+    if (r >= r_begin && r < r_end) {
+      n0 = f0[r];
+      if constexpr (kNf == 2) nm = f1[r].x;
+    }
+  };
+  auto stage = [&]() __attribute__((always_inline)) {
+    if constexpr (UNI) {
+      so[wv][lane] = (FAM == FAM_EPAN) ? make_double2(n0.y, n0.z) : make_double2(n0.y, n0.w);
+      sm[wv][lane] = (FAM == FAM_EPAN) ? n0.w : nm;
+    } else {
+      s0[wv][lane] = n0;
+      if constexpr (kNf == 2) sm[wv][lane] = nm;
+    }
+  };
+  // features of reference j of the staged sub-tile, as epan_term takes them
+  auto staged = [&](int j, double4 &a, double4 &b) __attribute__((always_inline)) {
+    if constexpr (UNI) {
+      const double2 o = so[wv][j];
+      const double m = sm[wv][j];
+      a = (FAM == FAM_EPAN) ? make_double4(usx, o.x, o.y, m) : make_double4(usx, o.x, usy, o.y);
+      b = make_double4(m, 0.0, 0.0, 0.0);
+    } else {
+      a = s0[wv][j];
+      b = make_double4(kNf == 2 ? sm[wv][j] : 0.0, 0.0, 0.0, 0.0);
     }
   };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
-  const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
-  // this lane's reference of a sub-tile (zero features outside the range
-  // give exact zero terms)
-  auto in_range = [&](long long sub) { return sub + lane >= r_begin && sub + lane < r_end; };
-  long long cur = next_needed((r_begin / kEpanSub) * kEpanSub, true);
-  double4 n0 = zero;
-  double nm = 0.0;
-  if (cur < r_end && in_range(cur)) {
-    n0 = f0[cur + lane];
-    if constexpr (kNf == 2) nm = f1[cur + lane].x;
-  }
+  long long cur = next_needed((r_begin / kEpanSub) * kEpanSub);
+  if (cur < r_end) load_ref(cur);
   while (cur < r_end) {
     wave_sync();
-    s0[wv][lane] = n0;
-    if constexpr (kNf == 2) sm[wv][lane] = nm;
+    stage();
     wave_sync();
-    const long long nxt = next_needed(cur + kEpanSub, ((cur + kEpanSub) % kTile) == 0);
-    n0 = zero;
-    nm = 0.0;
-    if (nxt < r_end && in_range(nxt)) {
-      n0 = f0[nxt + lane];
-      if constexpr (kNf == 2) nm = f1[nxt + lane].x;
-    }
+    const long long nxt = next_needed(cur + kEpanSub);
+    if (nxt < r_end) load_ref(nxt);
     const int cnt = (int)((r_end - cur) < kEpanSub ? (r_end - cur) : kEpanSub);
     const bool diag = self_excl && cur < q_hi && q_lo < cur + cnt;
     if (!diag) {
-#pragma unroll 2
+#pragma unroll BK_EPAN_UNROLL
       for (int j = 0; j < cnt; ++j) {
-        const double4 a = s0[wv][j];
-        const double4 b = make_double4(kNf == 2 ? sm[wv][j] : 0.0, 0.0, 0.0, 0.0);
+        double4 a, b;
+        staged(j, a, b);
 #pragma unroll
-        for (int k = 0; k < kQpt; ++k) acc[k] += epan_term<FAM>(px[k], py[k], a, b);
+        for (int k = 0; k < kEpanQpt; ++k) acc[k] += epan_term<FAM>(px[k], py[k], a, b);
       }
     } else {
       for (int j = 0; j < cnt; ++j) {
-        const double4 a = s0[wv][j];
-        const double4 b = make_double4(kNf == 2 ? sm[wv][j] : 0.0, 0.0, 0.0, 0.0);
+        double4 a, b;
+        staged(j, a, b);
         const long long rj = cur + j;
 #pragma unroll
-        for (int k = 0; k < kQpt; ++k) {
+        for (int k = 0; k < kEpanQpt; ++k) {
           const double v = epan_term<FAM>(px[k], py[k], a, b);
           acc[k] += (rj == q_begin + t + k * kBlock) ? 0.0 : v;
         }
@@ -1076,43 +1150,49 @@ __device__ __forceinline__ void epan_item(
     cur = nxt;
   }
 #pragma unroll
-  for (int k = 0; k < kQpt; ++k) {
+  for (int k = 0; k < kEpanQpt; ++k) {
     const int qi = t + k * kBlock;
     if (qi < q_count) dst[qi] = acc[k] * out_scale;
   }
 }
 
-template <int FAM, bool BOUNDED>
+template <int FAM, bool BOUNDED, bool UNI>
 __global__ __launch_bounds__(kBlock) void epan_range_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_i, long long r_end, long long split_len, long long q_i,
-    long long nq, double *__restrict__ dst, double out_scale, int self_excl, BoundArgs ba) {
-  long long bx = blockIdx.x, by = blockIdx.y;
+    long long nq, double *__restrict__ dst, double out_scale, int self_excl, BoundArgs ba, double usx,
+    double usy) {
+  // bounded: kEpanBoundParts consecutive workgroups take the parts of one
+  // listed block of kQueriesPerBlock queries
+  constexpr int kParts = BOUNDED ? kEpanBoundParts : 1;
+  long long bx = blockIdx.x / kParts, by = blockIdx.y;
+  const int part = (int)(blockIdx.x % kParts);
   if (ba.order) {
-    const int it = ba.order[blockIdx.x];
+    const int it = ba.order[bx];
     bx = it % ba.nblocks;
     by = it / ba.nblocks;
   }
-  const long long qb = bx * kQueriesPerBlock;
+  const long long qb = bx * (kParts * kEpanQueriesPerBlock) + part * kEpanQueriesPerBlock;
+  if (qb >= nq) return;
   long long rb, re;
   split_bounds<kTile>(r_i, r_end, split_len, (int)by, rb, re);
   const long long left = nq - qb;
-  const int q_count = (int)(left < kQueriesPerBlock ? left : kQueriesPerBlock);
+  const int q_count = (int)(left < kEpanQueriesPerBlock ? left : kEpanQueriesPerBlock);
   const long long off = by * nq + qb;
-  epan_item<FAM, BOUNDED>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
-                          BOUNDED ? ba.mask + bx * ba.nwords : nullptr);
+  epan_item<FAM, BOUNDED, UNI>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
+                               BOUNDED ? ba.mask + bx * ba.nwords : nullptr, usx, usy);
 }
 
-template <int FAM>
+template <int FAM, bool UNI>
 __global__ __launch_bounds__(kBlock) void epan_list_kernel(
     const float *__restrict__ qx, const float *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     const WorkItem *__restrict__ items, double *__restrict__ dst,
-    double out_scale, int self_excl) {
+    double out_scale, int self_excl, double usx, double usy) {
   const WorkItem it = items[blockIdx.x];
-  epan_item<FAM, false>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
-                        out_scale, self_excl, nullptr);
+  epan_item<FAM, false, UNI>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
+                             out_scale, self_excl, nullptr, usx, usy);
 }
 
 __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
@@ -1181,10 +1261,11 @@ __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__r
   const float ox = block_reduce_f<kTile, true>(valid ? up(oxv) : 0.0f, sh);
   const float oy = block_reduce_f<kTile, true>(valid ? up(oyv) : 0.0f, sh);
   double4 *m = tb + kEpanTileMeta * blockIdx.x;
+  float4 *reach = reinterpret_cast<float4 *>(m + 2);
   if (threadIdx.x == 0) {
     m[0] = make_double4(x0, x1, y0, y1);
     m[1] = make_double4(sx, sy, ox, oy);
-    m[2] = epan_reach_box(x0, x1, y0, y1, sx, sy, ox, oy, c);
+    reach[0] = epan_reach_box(x0, x1, y0, y1, sx, sy, ox, oy, c);
   }
   const float w_x0 = wave_reduce_f(valid ? xi : big, false), w_x1 = wave_reduce_f(valid ? xi : -big, true);
   const float w_y0 = wave_reduce_f(valid ? yi : big, false), w_y1 = wave_reduce_f(valid ? yi : -big, true);
@@ -1193,7 +1274,7 @@ __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__r
   const float w_ox = wave_reduce_f(valid ? up(oxv) : 0.0f, true);
   const float w_oy = wave_reduce_f(valid ? up(oyv) : 0.0f, true);
   if ((threadIdx.x & 63) == 0)
-    m[3 + (threadIdx.x >> 6)] = epan_reach_box(w_x0, w_x1, w_y0, w_y1, w_sx, w_sy, w_ox, w_oy, c);
+    reach[1 + (threadIdx.x >> 6)] = epan_reach_box(w_x0, w_x1, w_y0, w_y1, w_sx, w_sy, w_ox, w_oy, c);
 }
 
 // ---- FP64 context path ---------------------------------------------------
@@ -2321,7 +2402,7 @@ hipError_t launch_prep_epan(const EvalParams &p, const float *x, const float *y,
   prep_epan_kernel<<<(unsigned)((n + th - 1) / th), th, 0, s>>>(p, x, y, m, abw, (long long)n, f0, f1);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  prep_epan_tiles_kernel<<<(unsigned)((n + kTile - 1) / kTile), kTile, 0, s>>>(
+  prep_epan_tiles_kernel<<<(unsigned)epan_meta_tiles(n), kTile, 0, s>>>(
       x, y, f0, (long long)n, p.family == FAM_EPAN ? 1 : 0, p.family == FAM_EPANCONV ? 2.0 : 1.0, tb);
   return hipGetLastError();
 }
@@ -2330,39 +2411,58 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
                        const double4 *f0, const double4 *f1, const double4 *tb,
                        const WorkItem *items, int n_items, long long r_i,
                        long long r_end, long long q_i, long long nq, int splits,
-                       double *dst, double out_scale, int self_excl,
+                       double *dst, double out_scale, int self_excl, bool uniform_scales,
                        hipStream_t s, const BoundArgs *ba) {
+  // the scales of every reference when uniform_scales: 1/h_d (a = 1)
+  const double usx = p.ix, usy = p.iy;
   if (items) {
     if (n_items <= 0) return hipSuccess;
-#define BK_EL(F) epan_list_kernel<F><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale, self_excl)
+#define BK_EL(F, U) \
+  epan_list_kernel<F, U><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale, self_excl, usx, usy)
+#define BK_EL2(F)     \
+  if (uniform_scales) \
+    BK_EL(F, true);   \
+  else                \
+    BK_EL(F, false)
     switch (p.family) {
-      case FAM_EPAN: BK_EL(FAM_EPAN); break;
-      case FAM_EPANPROD: BK_EL(FAM_EPANPROD); break;
-      default: BK_EL(FAM_EPANCONV); break;
+      case FAM_EPAN: BK_EL2(FAM_EPAN); break;
+      case FAM_EPANPROD: BK_EL2(FAM_EPANPROD); break;
+      default: BK_EL2(FAM_EPANCONV); break;
     }
+#undef BK_EL2
 #undef BK_EL
   } else {
     if (nq <= 0) return hipSuccess;
     const long long len = split_length(r_i, r_end, splits, kTile);
-    dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock), (unsigned)splits);
     const BoundArgs b = ba ? *ba : BoundArgs{};
+    const bool bounded = ba && ba->mask;
+    // a bounded launch covers its listed blocks of kQueriesPerBlock queries
+    // with kEpanBoundParts workgroups each
+    dim3 grid(bounded ? (unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock * kEpanBoundParts)
+                      : (unsigned)((nq + kEpanQueriesPerBlock - 1) / kEpanQueriesPerBlock),
+              (unsigned)splits);
     if (b.order) grid = dim3(grid.x * grid.y, 1);
-#define BK_ER(F, B)                                                                                 \
-  epan_range_kernel<F, B><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, \
-                                                  self_excl, b)
-    if (ba && ba->mask) {
-      switch (p.family) {
-        case FAM_EPAN: BK_ER(FAM_EPAN, true); break;
-        case FAM_EPANPROD: BK_ER(FAM_EPANPROD, true); break;
-        default: BK_ER(FAM_EPANCONV, true); break;
-      }
-    } else {
-      switch (p.family) {
-        case FAM_EPAN: BK_ER(FAM_EPAN, false); break;
-        case FAM_EPANPROD: BK_ER(FAM_EPANPROD, false); break;
-        default: BK_ER(FAM_EPANCONV, false); break;
-      }
+#define BK_ER(F, B, U)                                                                                          \
+  epan_range_kernel<F, B, U><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, \
+                                                     self_excl, b, usx, usy)
+#define BK_ER2(F)                 \
+  if (bounded) {                  \
+    if (uniform_scales)           \
+      BK_ER(F, true, true);       \
+    else                          \
+      BK_ER(F, true, false);      \
+  } else {                        \
+    if (uniform_scales)           \
+      BK_ER(F, false, true);      \
+    else                          \
+      BK_ER(F, false, false);     \
+  }
+    switch (p.family) {
+      case FAM_EPAN: BK_ER2(FAM_EPAN); break;
+      case FAM_EPANPROD: BK_ER2(FAM_EPANPROD); break;
+      default: BK_ER2(FAM_EPANCONV); break;
     }
+#undef BK_ER2
 #undef BK_ER
   }
   return hipGetLastError();
