@@ -411,6 +411,27 @@ __device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf,
   }
 }
 
+// The factored loop over the kSubTile references of sub-tile sb only (a
+// whole tile, no self pair).
+__device__ __forceinline__ void gauss_packed_sub(const float4 *__restrict__ sf, int sb, const float *ax,
+                                                 const float *ay, float *s) {
+  constexpr int G = kSubTile / 4;
+  const float4 *f = sf + 3 * G * sb;
+  auto term = [&](float a0, float a1, float c) {
+#pragma unroll
+    for (int k = 0; k < kGaussQpt; ++k)
+      s[k] += __builtin_amdgcn_exp2f(__builtin_fmaf(a0, ax[k], __builtin_fmaf(a1, ay[k], c)));
+  };
+#pragma unroll 4
+  for (int g = 0; g < G; ++g) {
+    const float4 a = f[3 * g], b = f[3 * g + 1], c = f[3 * g + 2];
+    term(a.x, a.y, a.z);
+    term(a.w, b.x, b.y);
+    term(b.z, b.w, c.x);
+    term(c.y, c.z, c.w);
+  }
+}
+
 __device__ __forceinline__ void gauss_packed_tile(const float4 *__restrict__ sf, int j0, int j1, bool diag,
                                                   int self_j, const float *ax, const float *ay, float *s) {
   const bool full = j0 == 0 && j1 == kGaussTile;
@@ -609,8 +630,37 @@ __device__ __forceinline__ void gauss_item(
       // a factor of zero for every query of the wavefront: the tile adds nothing
       if (__any(any_near)) {
         const int j0 = tile < r_begin ? (int)(r_begin - tile) : 0;
-        gauss_packed_tile(sf[buf], j0, cnt, diag, self_j, ax, ay, s);
-        if constexpr (COUNT) evals += (unsigned long long)cnt;
+        unsigned subs = 0xFu;
+        if (j0 == 0 && cnt == T && !diag) {
+          // the same rule per sub-tile of kSubTile references, about its own
+          // centre and with its own reach: a sub-tile none of whose queries
+          // is within reach is passed over (the FP32 exponent is compared
+          // with a margin of 2^-16 for its rounding)
+          const float4 *subc = tiles + (tile / T) * kTileMeta + 1;
+          const float gxf = (float)gx, gyf = (float)gy;
+          subs = 0u;
+#pragma unroll
+          for (int sb = 0; sb < kGaussTile / kSubTile; ++sb) {
+            const float4 sc = subc[sb];
+            bool near_sub = false;
+#pragma unroll
+            for (int k = 0; k < kGaussQpt; ++k) {
+              const float dx = px[k] - sc.x, dy = py[k] - sc.y;
+              near_sub |= cur.w * __builtin_fmaf(gyf * dy, dy, gxf * dx * dx) <= sc.z * (1.0f + 0x1p-16f);
+            }
+            subs |= __any(near_sub) ? 1u << sb : 0u;
+          }
+        }
+        if (subs == 0xFu) {
+          gauss_packed_tile(sf[buf], j0, cnt, diag, self_j, ax, ay, s);
+          if constexpr (COUNT) evals += (unsigned long long)cnt;
+        } else {
+          for (int sb = 0; sb < kGaussTile / kSubTile; ++sb)
+            if ((subs >> sb) & 1u) {
+              gauss_packed_sub(sf[buf], sb, ax, ay, s);
+              if constexpr (COUNT) evals += (unsigned long long)kSubTile;
+            }
+        }
       }
     } else if (form == 1) {
       // no query of the wavefront within the tile's reach: the tile is passed over
