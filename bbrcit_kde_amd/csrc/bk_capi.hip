@@ -458,7 +458,7 @@ FeatBytes feature_bytes(Path path, size_t n) {
   switch (path) {
     case Path::F64:
       b.f0 = b.f1 = n * sizeof(double4);
-      b.tiles = (n + bk::kTile - 1) / bk::kTile * 2 * sizeof(double4);
+      b.tiles = (size_t)bk::epan_meta_tiles((long long)n) * bk::kF64TileMeta * sizeof(double4);
       break;
     case Path::GAUSS32:
       // f1: whole tiles (a factored tile's packed features fill the first
@@ -605,7 +605,8 @@ hipError_t launch_path(bk_ctx *ctx, Path path, const EvalParams &p, const FeatPt
     case Path::F64:
       return bk::launch_direct_f64(p, q.xd.as<double>(), q.yd.as<double>(), static_cast<const double4 *>(fp.f0),
                                    static_cast<const double4 *>(fp.f1), static_cast<const double4 *>(fp.tiles),
-                                   items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl, s, order);
+                                   items, n_items, r_i, r_end, q_i, nq, splits, dst, scale, self_excl,
+                                   r.uniform_abw, s, order);
     case Path::GAUSS32:
       return bk::launch_gauss(p, q.x.as<float>(), q.y.as<float>(), static_cast<const float4 *>(fp.f0),
                               static_cast<const float4 *>(fp.f1), static_cast<const float4 *>(fp.tiles), items,
@@ -705,8 +706,8 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
       hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
       const bk::BoundArgs ord = live_order(ctx, path, p, nullptr, nullptr, ox, oy, nq, (long long)r_i, r_end, splits);
       e = bk::launch_direct_f64(p, ox, oy, ctx->f0.as<double4>(), ctx->f1.as<double4>(), ctx->tiles.as<double4>(),
-                                nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0, s,
-                                ord.order ? &ord : nullptr);
+                                nullptr, 0, (long long)r_i, r_end, 0, nq, splits, dst, scale, 0,
+                                ctx->refs.uniform_abw, s, ord.order ? &ord : nullptr);
       hip_check(e, "direct kernel launch");
       if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, nq, res, s), "reduce launch");
       hip_check(bk::launch_scatter_f64(list, nq, res, out, s), "scatter launch");
@@ -911,8 +912,8 @@ void eval_bounded_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t q_i, size_
                           nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, ctx->refs.uniform_abw, s, &ba);
     else
       e = bk::launch_direct_f64(p, qx64, qy64, ctx->f0.as<double4>(), ctx->f1.as<double4>(),
-                                ctx->tiles.as<double4>(), nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0, s,
-                                &ba);
+                                ctx->tiles.as<double4>(), nullptr, 0, 0, nr, q_off, cn, splits, dst, scale, 0,
+                                ctx->refs.uniform_abw, s, &ba);
     hip_check(e, "bounded kernel launch");
     if (splits > 1) hip_check(bk::launch_reduce_splits(dst, splits, cn, exact + c0, s), "reduce launch");
     hip_check(bk::launch_bounds_combine(exact + c0, lo_b, gap_b, cn, qpb, factor, lo_all + c0, hi_all + c0, s),

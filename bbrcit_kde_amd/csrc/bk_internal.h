@@ -186,6 +186,9 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
 constexpr int kEpanSub = 64;  // references per wavefront stage
 constexpr int kEpanTileMeta = 2 + (1 + kTile / kEpanSub + 1) / 2;
 constexpr int kEpanScanBatch = 8;  // tile-level reach tests whose loads are issued together
+// FP64 path: the same layout ([1] holds that path's scale summary; the reach
+// boxes are written for the Epanechnikov families).
+constexpr int kF64TileMeta = kEpanTileMeta;
 inline long long epan_meta_tiles(long long n) {
   return (n + (long long)kTile * kEpanScanBatch - 1) / ((long long)kTile * kEpanScanBatch) * kEpanScanBatch;
 }
@@ -204,14 +207,16 @@ hipError_t launch_prep_f64(const EvalParams &p, const double *x,
                            const double *y, const double *m,
                            const double *abw, size_t n, double4 *f0,
                            double4 *f1, double4 *tb, hipStream_t s);
-// tb: per aligned tile of kTile references, the box and smallest scales
-// written by launch_prep_f64 (tile culling).
+// tb: kF64TileMeta double4 per aligned tile of kTile references (up to
+// epan_meta_tiles(n) tiles), written by launch_prep_f64.  The Epanechnikov
+// families run the wavefront-level kernel of the FP32 path on FP64 data
+// (uniform_scales: every abw is 1).
 hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              const double *qy, const double4 *f0,
                              const double4 *f1, const double4 *tb, const WorkItem *items,
                              int n_items, long long r_i, long long r_end,
                              long long q_i, long long nq, int splits,
-                             double *dst, double out_scale, int self_excl,
+                             double *dst, double out_scale, int self_excl, bool uniform_scales,
                              hipStream_t s, const BoundArgs *ba = nullptr);
 
 // General dimension, 3 <= D <= 16 (FAM_GAUSS and FAM_EPAN) ---------------

@@ -34,6 +34,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace bk {
 
 namespace {
@@ -984,10 +986,10 @@ __device__ __forceinline__ bool epan_tile_far(const double4 *__restrict__ tb, lo
 // 2^-52 o_max, so a reach r = c (1 + 2^-20 + 2^-50 o_max) / s_min past the
 // box suffices.  The bounds carry their own rounding outward and are then
 // rounded outward to FP32, the precision of the query coordinates.
-__device__ __forceinline__ float4 epan_reach_box(float x0, float x1, float y0, float y1, float sx, float sy,
-                                                 float ox, float oy, double c) {
-  const double rx = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)ox) / (double)sx;
-  const double ry = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)oy) / (double)sy;
+// The box [x0, x1] x [y0, y1] widened by rx and ry, outward-rounded to
+// FP32: below(lo, r) <= lo - r and above(hi, r) >= hi + r however the FP64
+// operations round.
+__device__ __forceinline__ float4 reach_box_f32(double x0, double x1, double y0, double y1, double rx, double ry) {
   // largest FP32 value <= v, smallest >= v (v finite; the float nearest v
   // is then finite or an infinity of v's sign, so the step stays ordered)
   auto below = [](double lo, double r) {
@@ -1005,6 +1007,13 @@ __device__ __forceinline__ float4 epan_reach_box(float x0, float x1, float y0, f
   return make_float4(below(x0, rx), above(x1, rx), below(y0, ry), above(y1, ry));
 }
 
+__device__ __forceinline__ float4 epan_reach_box(float x0, float x1, float y0, float y1, float sx, float sy,
+                                                 float ox, float oy, double c) {
+  const double rx = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)ox) / (double)sx;
+  const double ry = c * (1.0 + 0x1p-20 + 0x1p-50 * (double)oy) / (double)sy;
+  return reach_box_f32(x0, x1, y0, y1, rx, ry);
+}
+
 // Reach boxes of tile ti: [0] the tile's, [1 + s] that of its sub-tile s.
 __device__ __forceinline__ const float4 *epan_reach(const double4 *__restrict__ tb, long long ti) {
   return reinterpret_cast<const float4 *>(tb + kEpanTileMeta * ti + 2);
@@ -1016,13 +1025,26 @@ __device__ __forceinline__ const float4 *epan_reach(const double4 *__restrict__ 
 // coordinate compares false and keeps the tile.
 // (The comparisons are taken as lane masks, so the test is four compares
 // per query and scalar mask operations.)
+// FP64 queries compare through FP32 values rounded up (xu >= x) and down
+// (xd <= x), so a far verdict holds for the FP64 coordinate.
 template <int NQ>
-__device__ __forceinline__ bool epan_wave_far(float4 e, const float *fx, const float *fy) {
+__device__ __forceinline__ bool epan_wave_far(float4 e, const float *xu, const float *xd, const float *yu,
+                                              const float *yd) {
   unsigned long long far = ~0ull;
 #pragma unroll
   for (int k = 0; k < NQ; ++k)
-    far &= __ballot(fx[k] <= e.x) | __ballot(fx[k] >= e.y) | __ballot(fy[k] <= e.z) | __ballot(fy[k] >= e.w);
+    far &= __ballot(xu[k] <= e.x) | __ballot(xd[k] >= e.y) | __ballot(yu[k] <= e.z) | __ballot(yd[k] >= e.w);
   return (__builtin_amdgcn_read_exec() & ~far) == 0ull;
+}
+
+// FP32 values just above and below an FP64 coordinate.
+__device__ __forceinline__ float f32_up(double v) {
+  const float f = (float)v;
+  return (double)f >= v ? f : (f == 0.0f ? 0x1p-149f : __uint_as_float(__float_as_uint(f) + (f > 0.0f ? 1 : -1)));
+}
+__device__ __forceinline__ float f32_down(double v) {
+  const float f = (float)v;
+  return (double)f <= v ? f : (f == 0.0f ? -0x1p-149f : __uint_as_float(__float_as_uint(f) + (f > 0.0f ? -1 : 1)));
 }
 
 // Wavefront-scope barrier: orders this wavefront's LDS writes before its
@@ -1034,18 +1056,24 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <int FAM>
+__device__ __forceinline__ double pair_term_f64(double qx, double qy, double4 a, double4 b, double rho);
+
 // One workgroup, kEpanQpt queries per thread.  Each wavefront walks the
 // reference range on its own, with no workgroup barrier: it tests the reach
-// boxes of kEpanScanBatch tiles at a time (and, for a tile in reach, those of its
-// sub-tiles), stages each sub-tile of kEpanSub references in reach in its own
-// LDS slot, one reference per lane, and prefetches the next into registers.
-// Terms are summed in reference order, so the sums equal those over every
-// pair in that order.  UNI: every reference has the scales (usx, usy) (no
-// adaptive bandwidths), so only the offsets and the mass are staged.  In a
-// bounded evaluation only the tiles listed in bmask are visited.
-template <int FAM, bool BOUNDED, bool UNI>
+// boxes of kEpanScanBatch tiles at a time (and, for a tile in reach, those of
+// its sub-tiles), stages each sub-tile of kEpanSub references in reach in its
+// own LDS slot, one reference per lane, and prefetches the next into
+// registers.  Terms are summed in reference order, so the sums equal those
+// over every pair in that order.  UNI: every reference has the scales
+// (usx, usy) (no adaptive bandwidths), so only two coordinates (offsets, or
+// FP64 points) and the mass are staged.  In a bounded evaluation only the
+// tiles listed in bmask are visited.  F64: FP64 points and features
+// (pair_term_f64's layout); else FP32 points and epan_term's layout.
+template <int FAM, bool BOUNDED, bool UNI, bool F64>
 __device__ __forceinline__ void epan_item(
-    const float *__restrict__ qx, const float *__restrict__ qy,
+    const std::conditional_t<F64, double, float> *__restrict__ qx,
+    const std::conditional_t<F64, double, float> *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_begin, long long r_end, long long q_begin, int q_count,
     double *__restrict__ dst, double out_scale, int self_excl, const unsigned long long *__restrict__ bmask,
@@ -1053,8 +1081,8 @@ __device__ __forceinline__ void epan_item(
   constexpr int kNf = EpanNf<FAM>::kNf;
   constexpr int kWaves = kBlock / 64;
   constexpr int kSubs = kTile / kEpanSub;
-  // the staged features: all of f0 (and the mass of f1), or with UNI the
-  // offsets (o_x, o_y) and the mass
+  // the staged features: all of f0 (and the mass of f1), or with UNI two
+  // coordinates and the mass
   __shared__ double4 s0[kWaves][UNI ? 1 : kEpanSub];
   __shared__ double2 so[kWaves][UNI ? kEpanSub : 1];
   __shared__ double sm[kWaves][(kNf == 2 || UNI) ? kEpanSub : 1];
@@ -1064,15 +1092,22 @@ __device__ __forceinline__ void epan_item(
     return;
   }
   double px[kEpanQpt], py[kEpanQpt], acc[kEpanQpt];
-  float fx[kEpanQpt], fy[kEpanQpt];
+  float xu[kEpanQpt], xd[kEpanQpt], yu[kEpanQpt], yd[kEpanQpt];
 #pragma unroll
   for (int k = 0; k < kEpanQpt; ++k) {
     const int qi = t + k * kBlock;
     const long long q = q_begin + (qi < q_count ? qi : 0);
-    fx[k] = qx[q];
-    fy[k] = qy[q];
-    px[k] = (double)fx[k];
-    py[k] = (double)fy[k];
+    px[k] = (double)qx[q];
+    py[k] = (double)qy[q];
+    if constexpr (F64) {
+      xu[k] = f32_up(px[k]);
+      xd[k] = f32_down(px[k]);
+      yu[k] = f32_up(py[k]);
+      yd[k] = f32_down(py[k]);
+    } else {
+      xu[k] = xd[k] = qx[q];
+      yu[k] = yd[k] = qy[q];
+    }
     acc[k] = 0.0;
   }
   long long batch = -1, stile = -1;
@@ -1099,7 +1134,7 @@ __device__ __forceinline__ void epan_item(
         for (int i = 0; i < kEpanScanBatch; ++i) e[i] = epan_reach(tb, b + i)[0];
         unsigned m = 0;
 #pragma unroll
-        for (int i = 0; i < kEpanScanBatch; ++i) m |= epan_wave_far<kEpanQpt>(e[i], fx, fy) ? 0u : 1u << i;
+        for (int i = 0; i < kEpanScanBatch; ++i) m |= epan_wave_far<kEpanQpt>(e[i], xu, xd, yu, yd) ? 0u : 1u << i;
         near_t = m & listed;
       }
       const unsigned rest = near_t >> (int)(ti - b);
@@ -1120,7 +1155,7 @@ __device__ __forceinline__ void epan_item(
         for (int i = 0; i < kSubs; ++i) e[i] = er[i];
         unsigned m = 0;
 #pragma unroll
-        for (int i = 0; i < kSubs; ++i) m |= epan_wave_far<kEpanQpt>(e[i], fx, fy) ? 0u : 1u << i;
+        for (int i = 0; i < kSubs; ++i) m |= epan_wave_far<kEpanQpt>(e[i], xu, xd, yu, yd) ? 0u : 1u << i;
         near_s = m;
       }
       const unsigned srest = near_s >> (int)((sub - ti * kTile) / kEpanSub);
@@ -1132,7 +1167,7 @@ __device__ __forceinline__ void epan_item(
     }
   };
   // this lane's reference of a sub-tile (zero features outside the range
-  // give exact zero terms)
+  // give exact zero terms: a zero mass, or with FP64 features zero scales)
   const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
   double4 n0 = zero;
   double nm = 0.0;
@@ -1147,24 +1182,37 @@ __device__ __forceinline__ void epan_item(
   };
   auto stage = [&]() __attribute__((always_inline)) {
     if constexpr (UNI) {
-      so[wv][lane] = (FAM == FAM_EPAN) ? make_double2(n0.y, n0.z) : make_double2(n0.y, n0.w);
+      if constexpr (F64)
+        so[wv][lane] = make_double2(n0.x, n0.y);
+      else
+        so[wv][lane] = (FAM == FAM_EPAN) ? make_double2(n0.y, n0.z) : make_double2(n0.y, n0.w);
       sm[wv][lane] = (FAM == FAM_EPAN) ? n0.w : nm;
     } else {
       s0[wv][lane] = n0;
       if constexpr (kNf == 2) sm[wv][lane] = nm;
     }
   };
-  // features of reference j of the staged sub-tile, as epan_term takes them
+  // features of reference j of the staged sub-tile, as the term takes them
+  // (a masked reference has zero mass; with UNI its scales are usx, usy)
   auto staged = [&](int j, double4 &a, double4 &b) __attribute__((always_inline)) {
     if constexpr (UNI) {
       const double2 o = so[wv][j];
       const double m = sm[wv][j];
-      a = (FAM == FAM_EPAN) ? make_double4(usx, o.x, o.y, m) : make_double4(usx, o.x, usy, o.y);
+      if constexpr (F64)
+        a = (FAM == FAM_EPAN) ? make_double4(o.x, o.y, usx, m) : make_double4(o.x, o.y, usx, usy);
+      else
+        a = (FAM == FAM_EPAN) ? make_double4(usx, o.x, o.y, m) : make_double4(usx, o.x, usy, o.y);
       b = make_double4(m, 0.0, 0.0, 0.0);
     } else {
       a = s0[wv][j];
       b = make_double4(kNf == 2 ? sm[wv][j] : 0.0, 0.0, 0.0, 0.0);
     }
+  };
+  auto term = [&](int k, double4 a, double4 b) __attribute__((always_inline)) {
+    if constexpr (F64)
+      return pair_term_f64<FAM>(px[k], py[k], a, b, 0.0);
+    else
+      return epan_term<FAM>(px[k], py[k], a, b);
   };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
   long long cur = next_needed((r_begin / kEpanSub) * kEpanSub);
@@ -1183,7 +1231,7 @@ __device__ __forceinline__ void epan_item(
         double4 a, b;
         staged(j, a, b);
 #pragma unroll
-        for (int k = 0; k < kEpanQpt; ++k) acc[k] += epan_term<FAM>(px[k], py[k], a, b);
+        for (int k = 0; k < kEpanQpt; ++k) acc[k] += term(k, a, b);
       }
     } else {
       for (int j = 0; j < cnt; ++j) {
@@ -1192,7 +1240,7 @@ __device__ __forceinline__ void epan_item(
         const long long rj = cur + j;
 #pragma unroll
         for (int k = 0; k < kEpanQpt; ++k) {
-          const double v = epan_term<FAM>(px[k], py[k], a, b);
+          const double v = term(k, a, b);
           acc[k] += (rj == q_begin + t + k * kBlock) ? 0.0 : v;
         }
       }
@@ -1206,16 +1254,18 @@ __device__ __forceinline__ void epan_item(
   }
 }
 
-template <int FAM, bool BOUNDED, bool UNI>
+// kParts consecutive workgroups take the parts of one block of
+// kParts * kEpanQueriesPerBlock queries (the listed blocks of a bounded
+// evaluation, and the query blocks of the FP64 path).
+template <int FAM, bool BOUNDED, bool UNI, bool F64>
 __global__ __launch_bounds__(kBlock) void epan_range_kernel(
-    const float *__restrict__ qx, const float *__restrict__ qy,
+    const std::conditional_t<F64, double, float> *__restrict__ qx,
+    const std::conditional_t<F64, double, float> *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_i, long long r_end, long long split_len, long long q_i,
     long long nq, double *__restrict__ dst, double out_scale, int self_excl, BoundArgs ba, double usx,
     double usy) {
-  // bounded: kEpanBoundParts consecutive workgroups take the parts of one
-  // listed block of kQueriesPerBlock queries
-  constexpr int kParts = BOUNDED ? kEpanBoundParts : 1;
+  constexpr int kParts = (BOUNDED || F64) ? kEpanBoundParts : 1;
   long long bx = blockIdx.x / kParts, by = blockIdx.y;
   const int part = (int)(blockIdx.x % kParts);
   if (ba.order) {
@@ -1230,19 +1280,26 @@ __global__ __launch_bounds__(kBlock) void epan_range_kernel(
   const long long left = nq - qb;
   const int q_count = (int)(left < kEpanQueriesPerBlock ? left : kEpanQueriesPerBlock);
   const long long off = by * nq + qb;
-  epan_item<FAM, BOUNDED, UNI>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale, self_excl,
-                               BOUNDED ? ba.mask + bx * ba.nwords : nullptr, usx, usy);
+  epan_item<FAM, BOUNDED, UNI, F64>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale,
+                                    self_excl, BOUNDED ? ba.mask + bx * ba.nwords : nullptr, usx, usy);
 }
 
-template <int FAM, bool UNI>
+// Work items of up to kParts * kEpanQueriesPerBlock queries, kParts
+// workgroups each.
+template <int FAM, bool UNI, bool F64>
 __global__ __launch_bounds__(kBlock) void epan_list_kernel(
-    const float *__restrict__ qx, const float *__restrict__ qy,
+    const std::conditional_t<F64, double, float> *__restrict__ qx,
+    const std::conditional_t<F64, double, float> *__restrict__ qy,
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     const WorkItem *__restrict__ items, double *__restrict__ dst,
     double out_scale, int self_excl, double usx, double usy) {
-  const WorkItem it = items[blockIdx.x];
-  epan_item<FAM, false, UNI>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin, it.q_count, dst + it.out_index,
-                             out_scale, self_excl, nullptr, usx, usy);
+  constexpr int kParts = F64 ? kEpanBoundParts : 1;
+  const WorkItem it = items[blockIdx.x / kParts];
+  const int q0 = (int)(blockIdx.x % kParts) * kEpanQueriesPerBlock;
+  if (q0 >= it.q_count) return;
+  const int qn = it.q_count - q0 < kEpanQueriesPerBlock ? it.q_count - q0 : kEpanQueriesPerBlock;
+  epan_item<FAM, false, UNI, F64>(qx, qy, f0, f1, tb, it.r_begin, it.r_end, it.q_begin + q0, qn,
+                                  dst + it.out_index + q0, out_scale, self_excl, nullptr, usx, usy);
 }
 
 __global__ void prep_epan_kernel(EvalParams p, const float *__restrict__ x,
@@ -1408,8 +1465,8 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
 template <int FAM>
 __device__ __forceinline__ bool f64_tile_far(const double4 *__restrict__ tb, long long tile_idx, double bx0,
                                              double bx1, double by0, double by1, double rho, double t_cull) {
-  const double4 b = tb[2 * tile_idx];
-  const double4 w = tb[2 * tile_idx + 1];
+  const double4 b = tb[kF64TileMeta * tile_idx];
+  const double4 w = tb[kF64TileMeta * tile_idx + 1];
   const double gx = fmax(0.0, fmax(b.x - bx1, bx0 - b.y));
   const double gy = fmax(0.0, fmax(b.z - by1, by0 - b.w));
   if constexpr (FAM == FAM_GAUSS) {
@@ -1440,8 +1497,8 @@ struct CullBound {
 __device__ __forceinline__ void cull_bound_add(CullBound &c, const double4 *__restrict__ tb, long long ti,
                                                long long r_begin, long long r_end, bool own, double bx0, double bx1,
                                                double by0, double by1, double rho) {
-  const double4 b = tb[2 * ti];
-  const double4 w = tb[2 * ti + 1];
+  const double4 b = tb[kF64TileMeta * ti];
+  const double4 w = tb[kF64TileMeta * ti + 1];
   c.mass += w.w;
   c.min_m = fmin(c.min_m, w.y);
   const long long s = ti * kTile64;
@@ -1654,10 +1711,20 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
 
 // One workgroup of kTile64 threads per aligned tile: the tile's FP64 box and
 // scale and mass summary (see f64_tile_far).
+//
+// For the Epanechnikov families also the FP32 reach boxes of the tile and of
+// its sub-tiles (see reach_box_f32): a query whose coordinate on one axis is
+// at least r from every reference there has a zero term with each of them,
+// as computed -- r = (1 + 2^-20) / sqrt(s_min) with s_min the smallest
+// squared scale (a.z, or a.w on y for the product kernel), or
+// 2 (1 + 2^-20) / s_min with s_min the smallest scale (convolution kernel):
+// the FP64 products and sums that form the term lose at most a few 2^-53 of
+// its parts.
 __global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *__restrict__ x,
                                                                   const double *__restrict__ y,
                                                                   const double4 *__restrict__ f0, long long n,
-                                                                  double4 *__restrict__ tb) {
+                                                                  int family, double4 *__restrict__ tb) {
+  static_assert(kTile64 == kTile && kEpanSub == 64, "one sub-tile per wavefront");
   __shared__ double sh[kTile64];
   const long long i = (long long)blockIdx.x * kTile64 + threadIdx.x;
   const bool valid = i < n;
@@ -1692,10 +1759,24 @@ __global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *_
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
+  double4 *mt = tb + kF64TileMeta * blockIdx.x;
   if (threadIdx.x == 0) {
-    tb[2 * blockIdx.x] = make_double4(x0, x1, y0, y1);
-    tb[2 * blockIdx.x + 1] = make_double4(wz, ww, wz_max, sh[0]);
+    mt[0] = make_double4(x0, x1, y0, y1);
+    mt[1] = make_double4(wz, ww, wz_max, sh[0]);
   }
+  if (family == FAM_GAUSS) return;
+  auto reach = [&](double s_min) {
+    return family == FAM_EPANCONV ? 2.0 * (1.0 + 0x1p-20) / s_min : (1.0 + 0x1p-20) / sqrt(s_min);
+  };
+  float4 *rb = reinterpret_cast<float4 *>(mt + 2);
+  const double sy_t = family == FAM_EPAN ? wz : ww;
+  if (threadIdx.x == 0) rb[0] = reach_box_f32(x0, x1, y0, y1, reach(wz), reach(sy_t));
+  const double w_x0 = wave_reduce_d(valid ? xi : big, false), w_x1 = wave_reduce_d(valid ? xi : -big, true);
+  const double w_y0 = wave_reduce_d(valid ? yi : big, false), w_y1 = wave_reduce_d(valid ? yi : -big, true);
+  const double w_sx = wave_reduce_d(a.z, false);
+  const double w_sy = family == FAM_EPAN ? w_sx : wave_reduce_d(a.w, false);
+  if ((threadIdx.x & 63) == 0)
+    rb[1 + (threadIdx.x >> 6)] = reach_box_f32(w_x0, w_x1, w_y0, w_y1, reach(w_sx), reach(w_sy));
 }
 
 // ---- general dimension (3 <= D <= 16), FP64 distances -----------------------
@@ -2468,7 +2549,7 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
   if (items) {
     if (n_items <= 0) return hipSuccess;
 #define BK_EL(F, U) \
-  epan_list_kernel<F, U><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale, self_excl, usx, usy)
+  epan_list_kernel<F, U, false><<<n_items, kBlock, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale, self_excl, usx, usy)
 #define BK_EL2(F)     \
   if (uniform_scales) \
     BK_EL(F, true);   \
@@ -2493,8 +2574,8 @@ hipError_t launch_epan(const EvalParams &p, const float *qx, const float *qy,
               (unsigned)splits);
     if (b.order) grid = dim3(grid.x * grid.y, 1);
 #define BK_ER(F, B, U)                                                                                          \
-  epan_range_kernel<F, B, U><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, \
-                                                     self_excl, b, usx, usy)
+  epan_range_kernel<F, B, U, false><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, \
+                                                            out_scale, self_excl, b, usx, usy)
 #define BK_ER2(F)                 \
   if (bounded) {                  \
     if (uniform_scales)           \
@@ -2528,7 +2609,7 @@ hipError_t launch_prep_f64(const EvalParams &p, const double *x,
       p, x, y, m, abw, (long long)n, f0, f1);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  prep_f64_tiles_kernel<<<(unsigned)((n + kTile64 - 1) / kTile64), kTile64, 0, s>>>(x, y, f0, (long long)n, tb);
+  prep_f64_tiles_kernel<<<(unsigned)epan_meta_tiles(n), kTile64, 0, s>>>(x, y, f0, (long long)n, p.family, tb);
   return hipGetLastError();
 }
 
@@ -2537,21 +2618,67 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
                              const double4 *f1, const double4 *tb, const WorkItem *items,
                              int n_items, long long r_i, long long r_end,
                              long long q_i, long long nq, int splits,
-                             double *dst, double out_scale, int self_excl,
+                             double *dst, double out_scale, int self_excl, bool uniform_scales,
                              hipStream_t s, const BoundArgs *ba) {
+  if (p.family != FAM_GAUSS) {
+    // scales of every reference when uniform_scales (a = 1): 1/h_d^2, or
+    // 1/h_d for the convolution kernel
+    const double usx = p.family == FAM_EPANCONV ? p.ix : p.gx;
+    const double usy = p.family == FAM_EPANCONV ? p.iy : p.gy;
+    if (items) {
+      if (n_items <= 0) return hipSuccess;
+      const unsigned g = (unsigned)n_items * kEpanBoundParts;
+#define BK_EL(F, U) \
+  epan_list_kernel<F, U, true><<<g, kBlock, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale, self_excl, usx, usy)
+#define BK_EL2(F)     \
+  if (uniform_scales) \
+    BK_EL(F, true);   \
+  else                \
+    BK_EL(F, false)
+      switch (p.family) {
+        case FAM_EPAN: BK_EL2(FAM_EPAN); break;
+        case FAM_EPANPROD: BK_EL2(FAM_EPANPROD); break;
+        default: BK_EL2(FAM_EPANCONV); break;
+      }
+#undef BK_EL2
+#undef BK_EL
+    } else {
+      if (nq <= 0) return hipSuccess;
+      const long long len = split_length(r_i, r_end, splits, kTile);
+      const BoundArgs b = ba ? *ba : BoundArgs{};
+      const bool bounded = ba && ba->mask;
+      dim3 grid((unsigned)((nq + kQueriesPerBlock - 1) / kQueriesPerBlock * kEpanBoundParts), (unsigned)splits);
+      if (b.order) grid = dim3(grid.x * grid.y, 1);
+#define BK_ER(F, B, U)                                                                                      \
+  epan_range_kernel<F, B, U, true><<<grid, kBlock, 0, s>>>(qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, \
+                                                           out_scale, self_excl, b, usx, usy)
+#define BK_ER2(F)                 \
+  if (bounded) {                  \
+    if (uniform_scales)           \
+      BK_ER(F, true, true);       \
+    else                          \
+      BK_ER(F, true, false);      \
+  } else {                        \
+    if (uniform_scales)           \
+      BK_ER(F, false, true);      \
+    else                          \
+      BK_ER(F, false, false);     \
+  }
+      switch (p.family) {
+        case FAM_EPAN: BK_ER2(FAM_EPAN); break;
+        case FAM_EPANPROD: BK_ER2(FAM_EPANPROD); break;
+        default: BK_ER2(FAM_EPANCONV); break;
+      }
+#undef BK_ER2
+#undef BK_ER
+    }
+    return hipGetLastError();
+  }
   const double rho = p.gy / p.gx;
   if (items) {
     if (n_items <= 0) return hipSuccess;
-#define BK_L64(F)                                                             \
-  direct_f64_list_kernel<F><<<n_items, kBlock64, 0, s>>>(                     \
-      qx, qy, f0, f1, tb, items, dst, out_scale, self_excl, rho)
-    switch (p.family) {
-      case FAM_GAUSS: BK_L64(FAM_GAUSS); break;
-      case FAM_EPAN: BK_L64(FAM_EPAN); break;
-      case FAM_EPANPROD: BK_L64(FAM_EPANPROD); break;
-      default: BK_L64(FAM_EPANCONV); break;
-    }
-#undef BK_L64
+    direct_f64_list_kernel<FAM_GAUSS><<<n_items, kBlock64, 0, s>>>(qx, qy, f0, f1, tb, items, dst, out_scale,
+                                                                   self_excl, rho);
   } else {
     if (nq <= 0) return hipSuccess;
     long long len = (r_end - r_i + splits - 1) / splits;
@@ -2562,21 +2689,10 @@ hipError_t launch_direct_f64(const EvalParams &p, const double *qx,
 #define BK_R64(F, B)                                                          \
   direct_f64_range_kernel<F, B><<<grid, kBlock64, 0, s>>>(                    \
       qx, qy, f0, f1, tb, r_i, r_end, len, q_i, nq, dst, out_scale, self_excl, rho, b)
-    if (ba && ba->mask) {
-      switch (p.family) {
-        case FAM_GAUSS: BK_R64(FAM_GAUSS, true); break;
-        case FAM_EPAN: BK_R64(FAM_EPAN, true); break;
-        case FAM_EPANPROD: BK_R64(FAM_EPANPROD, true); break;
-        default: BK_R64(FAM_EPANCONV, true); break;
-      }
-    } else {
-      switch (p.family) {
-        case FAM_GAUSS: BK_R64(FAM_GAUSS, false); break;
-        case FAM_EPAN: BK_R64(FAM_EPAN, false); break;
-        case FAM_EPANPROD: BK_R64(FAM_EPANPROD, false); break;
-        default: BK_R64(FAM_EPANCONV, false); break;
-      }
-    }
+    if (ba && ba->mask)
+      BK_R64(FAM_GAUSS, true);
+    else
+      BK_R64(FAM_GAUSS, false);
 #undef BK_R64
   }
   return hipGetLastError();
