@@ -73,7 +73,7 @@ struct bk_ctx {
   int feat_path = -1;  // Path of the cached features
   int query_order = BK_QUERY_ORDER_GIVEN;
   int count_terms = 0;  // BK_OPT_COUNT_TERMS
-  DevBuf term_count;    // two counters of the instrumented Gaussian kernel
+  DevBuf term_count;    // two counters of the counting FP32 Gaussian / Epanechnikov launches
   DevBuf ord_list, ord_keys_a, ord_keys_b, ord_tmp, ord_x, ord_y, ord_out;  // spatial query order
   // bounded evaluation: tile summaries of the reference set (per version),
   // per-block thresholds, pruning counters, bounds planes
@@ -719,7 +719,8 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
     float *ox = ctx->ord_x.as<float>(), *oy = ctx->ord_y.as<float>();
     hip_check(bk::launch_gather_xy(qx, qy, list, nq, ox, oy, s), "gather launch");
     bk::BoundArgs ord = live_order(ctx, path, p, ox, oy, nullptr, nullptr, nq, (long long)r_i, r_end, splits);
-    if (ctx->count_terms && path == Path::GAUSS32) ord.count = ctx->term_count.as<unsigned long long>();
+    if (ctx->count_terms && (path == Path::GAUSS32 || path == Path::EPAN64))
+      ord.count = ctx->term_count.as<unsigned long long>();
     const bk::BoundArgs *op = (ord.order || ord.count) ? &ord : nullptr;
     if (path == Path::GAUSS32)
       e = bk::launch_gauss(p, ox, oy, ctx->f0.as<float4>(), ctx->f1.as<float4>(), ctx->tiles.as<float4>(), nullptr,
@@ -748,7 +749,8 @@ void eval_range_dev(bk_ctx *ctx, const bk_kernel_desc *desc, size_t r_i, size_t 
                      f64 ? qs.xd.as<double>() + q_i : nullptr, f64 ? qs.yd.as<double>() + q_i : nullptr, nq,
                      (long long)r_i, r_end, splits);
   }
-  if (ctx->count_terms && path == Path::GAUSS32) ord.count = ctx->term_count.as<unsigned long long>();
+  if (ctx->count_terms && (path == Path::GAUSS32 || path == Path::EPAN64))
+    ord.count = ctx->term_count.as<unsigned long long>();
   hip_check(launch_path(ctx, path, p, ctx_feats(ctx), nullptr, 0, (long long)r_i, r_end, (long long)q_i, nq, splits,
                         dst, scale, self_excl, (ord.order || ord.count) ? &ord : nullptr),
             "direct kernel launch");
@@ -1036,7 +1038,8 @@ void eval_queries_dev(bk_ctx *ctx, const bk_kernel_desc *desc, const float *qx, 
     const double scale = path_scale(ctx, path, p, 1);
     // the callers pass queries in Z-curve order: longest workgroups first
     BoundArgs ord = live_order(ctx, path, p, qx, qy, nullptr, nullptr, (long long)nq, 0, nr, splits);
-    if (ctx->count_terms && path == Path::GAUSS32) ord.count = ctx->term_count.as<unsigned long long>();
+    if (ctx->count_terms && (path == Path::GAUSS32 || path == Path::EPAN64))
+      ord.count = ctx->term_count.as<unsigned long long>();
     const BoundArgs *op = (ord.order || ord.count) ? &ord : nullptr;
     hipError_t e;
     if (path == Path::GAUSS32)

@@ -113,9 +113,10 @@ struct BoundArgs {
   // listed_order: most listed tiles first); nullptr = the 2-D grid's order
   const int *order = nullptr;
   int nblocks = 0;
-  // FP32 Gaussian range launches (not BOUNDED): when set, an instrumented
-  // variant adds the pair terms its wavefronts evaluate to count[0] (every
-  // lane) and count[1] (lanes holding a real query)
+  // FP32 Gaussian and Epanechnikov range launches (not BOUNDED): when set,
+  // the pair terms their wavefronts evaluate are added to count[0] (every
+  // lane) and count[1] (lanes holding a real query); the Gaussian runs an
+  // instrumented variant
   unsigned long long *count = nullptr;
 };
 // Reference split length of the range kernels (a multiple of `tile`).

@@ -1077,7 +1077,7 @@ __device__ __forceinline__ void epan_item(
     const double4 *__restrict__ f0, const double4 *__restrict__ f1, const double4 *__restrict__ tb,
     long long r_begin, long long r_end, long long q_begin, int q_count,
     double *__restrict__ dst, double out_scale, int self_excl, const unsigned long long *__restrict__ bmask,
-    double usx, double usy) {
+    double usx, double usy, unsigned long long *__restrict__ count = nullptr) {
   constexpr int kNf = EpanNf<FAM>::kNf;
   constexpr int kWaves = kBlock / 64;
   constexpr int kSubs = kTile / kEpanSub;
@@ -1215,6 +1215,7 @@ __device__ __forceinline__ void epan_item(
       return epan_term<FAM>(px[k], py[k], a, b);
   };
   const long long q_lo = q_begin, q_hi = q_begin + q_count;
+  unsigned long long evals = 0;  // count: references this wavefront loops over
   long long cur = next_needed((r_begin / kEpanSub) * kEpanSub);
   if (cur < r_end) load_ref(cur);
   while (cur < r_end) {
@@ -1225,6 +1226,7 @@ __device__ __forceinline__ void epan_item(
     if (nxt < r_end) load_ref(nxt);
     const int cnt = (int)((r_end - cur) < kEpanSub ? (r_end - cur) : kEpanSub);
     const bool diag = self_excl && cur < q_hi && q_lo < cur + cnt;
+    if (count) evals += (unsigned long long)cnt;
     if (!diag) {
 #pragma unroll BK_EPAN_UNROLL
       for (int j = 0; j < cnt; ++j) {
@@ -1251,6 +1253,16 @@ __device__ __forceinline__ void epan_item(
   for (int k = 0; k < kEpanQpt; ++k) {
     const int qi = t + k * kBlock;
     if (qi < q_count) dst[qi] = acc[k] * out_scale;
+  }
+  if (count) {
+    // pair terms evaluated: every lane, and the lanes holding a real query
+    int real = 0;
+#pragma unroll
+    for (int k = 0; k < kEpanQpt; ++k) real += __popcll(__ballot(t + k * kBlock < q_count));
+    if (lane == 0 && evals > 0) {
+      atomicAdd(count, evals * 64ULL * (unsigned long long)kEpanQpt);
+      atomicAdd(count + 1, evals * (unsigned long long)real);
+    }
   }
 }
 
@@ -1281,7 +1293,7 @@ __global__ __launch_bounds__(kBlock) void epan_range_kernel(
   const int q_count = (int)(left < kEpanQueriesPerBlock ? left : kEpanQueriesPerBlock);
   const long long off = by * nq + qb;
   epan_item<FAM, BOUNDED, UNI, F64>(qx, qy, f0, f1, tb, rb, re, q_i + qb, q_count, dst + off, out_scale,
-                                    self_excl, BOUNDED ? ba.mask + bx * ba.nwords : nullptr, usx, usy);
+                                    self_excl, BOUNDED ? ba.mask + bx * ba.nwords : nullptr, usx, usy, ba.count);
 }
 
 // Work items of up to kParts * kEpanQueriesPerBlock queries, kParts

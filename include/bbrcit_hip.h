@@ -86,10 +86,11 @@ extern "C" {
                                   /*   (D <= 2), so each workgroup                  */
                                   /*   holds nearby queries and skips the reference */
                                   /*   tiles beyond the kernel's reach              */
-#define BK_OPT_COUNT_TERMS     4  /* 1: FP32 Gaussian range evaluations (D <= 2)   */
-                                  /* run an instrumented variant of their kernel  */
-                                  /* that counts the pair terms it evaluates      */
-                                  /* (read with bk_term_count); slower, for       */
+#define BK_OPT_COUNT_TERMS     4  /* 1: FP32-point Gaussian and Epanechnikov range */
+                                  /* evaluations (D <= 2) count the pair terms    */
+                                  /* their kernels evaluate (read with            */
+                                  /* bk_term_count; the Gaussian runs an          */
+                                  /* instrumented variant, slower); for           */
                                   /* measurement only.  0 (default): off          */
 
 /* ---- POD types -------------------------------------------------------- */
@@ -168,9 +169,10 @@ int bk_lexico_order(int device, int dim, size_t n, const double *coords, long lo
 /* Pair terms evaluated since counting was switched on or last read
  * (BK_OPT_COUNT_TERMS): terms_out = wavefront lanes x references they loop
  * over (the vector work), query_terms_out = the same for lanes that hold a
- * real query.  Tiles that a whole wavefront or workgroup passes over (all its
- * terms below 2^-49, resp. 2^-100, of their mass) are not counted.  Resets
- * both counters.  No counterpart in the reference, whose kernel evaluates
+ * real query.  (Sub-)tiles that a whole wavefront or workgroup passes over
+ * are not counted: Gaussian, all its terms below 2^-56 (workgroup: 2^-100)
+ * of their mass; Epanechnikov, all its terms exactly zero.  Resets both
+ * counters.  No counterpart in the reference, whose kernel evaluates
  * every pair (include/CudaDirectKde.h:147-164). */
 int bk_term_count(bk_ctx *ctx, unsigned long long *terms_out,
                   unsigned long long *query_terms_out);
