@@ -63,7 +63,7 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wavefronts)
 #define BK_EPAN_QPT 1
 #endif
 #ifndef BK_EPAN_UNROLL
-#define BK_EPAN_UNROLL 2
+#define BK_EPAN_UNROLL 4
 #endif
 constexpr int kQpt = BK_QPT;       // queries held in registers per thread (FP64-context kernels)
 // queries per thread of the Epanechnikov kernels on FP32 points: one, so a

@@ -1227,8 +1227,16 @@ __device__ __forceinline__ void epan_item(
     const int cnt = (int)((r_end - cur) < kEpanSub ? (r_end - cur) : kEpanSub);
     const bool diag = self_excl && cur < q_hi && q_lo < cur + cnt;
     if (count) evals += (unsigned long long)cnt;
-    if (!diag) {
+    if (!diag && cnt == kEpanSub) {
+      // a whole sub-tile: a fixed trip count
 #pragma unroll BK_EPAN_UNROLL
+      for (int j = 0; j < kEpanSub; ++j) {
+        double4 a, b;
+        staged(j, a, b);
+#pragma unroll
+        for (int k = 0; k < kEpanQpt; ++k) acc[k] += term(k, a, b);
+      }
+    } else if (!diag) {
       for (int j = 0; j < cnt; ++j) {
         double4 a, b;
         staged(j, a, b);
