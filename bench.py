@@ -47,7 +47,7 @@ PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 peak (vector and matrix)
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one run each) and scaled by a
 # 256 MiB copy kernel's readings (fetch x2.00, write x1.00):
 # 360,996 KiB fetched + 257,812 KiB written.
-GAUSS_HBM_BYTES_PER_LAUNCH = (360_996 + 257_812) * 1024
+GAUSS_HBM_BYTES_PER_LAUNCH = (363_188 + 257_812) * 1024
 # FP32 VALU issue peak of MI355X: 157.3 TFLOP/s / 2 FLOP per FMA = 256 CUs x
 # 4 SIMDs x 32 lanes per cycle x 2.4 GHz (lane-instructions per second).
 # Unpacked v_fma_f32 reaches it (a microbenchmark sustained 84%); v_exp_f32
