@@ -53,6 +53,7 @@ using bk::hip_check;
 
 constexpr int kCvBlock = 256;   // threads per workgroup = queries per work item
 constexpr int kCvTile = 256;    // references per LDS stage
+constexpr int kCvChunk = 64;    // references per wavefront-level test (one wavefront's stage writes)
 constexpr int kMaxGridAxis = 64;
 constexpr float kMaskedLog2 = -1.0e30f;  // base-2 log mass of a masked reference
 constexpr double kLog2e = 1.4426950408889634074;
@@ -96,11 +97,12 @@ struct CvArgs {
 // ---- Gaussian product kernel: 8 or 4 x-factors, 8 y-factors per reference ----
 // U: unroll of the stage loop over references (a tuned parameter; it does
 // not change the order of any sum).
+// One chunk of kCvChunk references of the stage, starting at k0.
 template <int A, int B, bool ADAPT, bool DIAG, int U>
-__device__ __forceinline__ void gauss_stage(const float4 *__restrict__ tl, float qx, float qy, int self,
+__device__ __forceinline__ void gauss_stage(const float4 *__restrict__ tl, int k0, float qx, float qy, int self,
                                             const float (&kx)[A], const float (&ky)[B], float (&acc)[A][B]) {
 #pragma unroll U
-  for (int k = 0; k < kCvTile; ++k) {
+  for (int k = k0; k < k0 + kCvChunk; ++k) {
     const float4 r = tl[k];
     const float dx = qx - r.x, dy = qy - r.y;
     float tx = dx * dx, ty = dy * dy;
@@ -130,11 +132,11 @@ __device__ __forceinline__ void gauss_stage(const float4 *__restrict__ tl, float
 // (2-u)^3 (u^2 + 6u + 4) for u < 2 carries 32 times the reference's k(u);
 // the 1/1024 of both axes is applied in the score reduction.
 template <int A, int B, bool ADAPT, bool DIAG, int U>
-__device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, double qx, double qy, int self,
+__device__ __forceinline__ void epan_stage(const double4 *__restrict__ tl, int k0, double qx, double qy, int self,
                                            const double (&kx)[A], const double (&ky)[B], double (&accK)[A][B],
                                            double (&accT)[A][B]) {
 #pragma unroll U
-  for (int k = 0; k < kCvTile; ++k) {
+  for (int k = k0; k < k0 + kCvChunk; ++k) {
     const double4 r = tl[k];
     double ax = fabs(qx - r.x), ay = fabs(qy - r.y);
     if (ADAPT) {
@@ -176,8 +178,10 @@ __device__ __forceinline__ double wave_min_d(double v) {
   return v;
 }
 
-// Minima over the workgroup of the 5 values v (one set per thread) through
-// box[5][waves]: written before a barrier, read after it.
+// Per-wavefront minima of the 5 values v (one set per thread) into
+// box[5][wavefront]: written before a barrier, read after it.  Wavefront c of
+// a stage's writes holds the box of the stage's c-th chunk of kCvChunk
+// references; a wavefront's query box is its own entry.
 __device__ __forceinline__ void box_write(double (&box)[5][kCvBlock / 64], const double (&v)[5]) {
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
@@ -185,27 +189,34 @@ __device__ __forceinline__ void box_write(double (&box)[5][kCvBlock / 64], const
     if ((threadIdx.x & 63) == 0) box[i][threadIdx.x >> 6] = m;
   }
 }
-__device__ __forceinline__ void box_read(const double (&box)[5][kCvBlock / 64], double (&v)[5]) {
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    v[i] = box[i][0];
-#pragma unroll
-    for (int w = 1; w < kCvBlock / 64; ++w) v[i] = fmin(v[i], box[i][w]);
-  }
+
+// The (query, reference, bandwidth pair) units that wavefront wv of item w
+// evaluates over the chunk of references starting at c0 (its lanes that
+// hold a real query), added to g.count.
+template <int A, int B>
+__device__ __forceinline__ void count_units(const CvArgs &g, const CvItem &w, int c0, int wv) {
+  const int la = g.n_hx - w.sub_x < A ? g.n_hx - w.sub_x : A;
+  const int lb = g.n_hy - w.sub_y < B ? g.n_hy - w.sub_y : B;
+  const int nr = w.r_end - c0 < kCvChunk ? w.r_end - c0 : kCvChunk;
+  const int left = w.q_count - wv * 64;
+  const int nq = left < 0 ? 0 : (left < 64 ? left : 64);
+  if (nr > 0 && nq > 0)
+    atomicAdd(g.count, (unsigned long long)nr * (unsigned long long)nq * (unsigned long long)(la * lb));
 }
 
 // Persistent pair kernels.  Every workgroup pulls items until the queue is
 // exhausted; all branches on the item are uniform (the item index is read
 // from LDS and made scalar).
 //
-// The Gaussian pair kernel skips a stage when every product ex ey of it is
-// below 2^-152 for every query of the item: the per-axis gaps between the
-// query box and the stage's box give W_min (gx^2 kx_min + gy^2 ky_min) >=
+// The Gaussian pair kernel: a wavefront passes over a chunk of kCvChunk
+// references of a stage when every product ex ey of it is below 2^-152 for
+// every query of the wavefront: the per-axis gaps between the wavefront's
+// query box and the chunk's box give W_min (gx^2 kx_min + gy^2 ky_min) >=
 // kCvGaussCullExp, with kx_min, ky_min the sub-grid's smallest exponent
-// factors, W_min the stage's smallest 1/abw^2 and log2(m / m_scale) <= 0.
+// factors, W_min the chunk's smallest 1/abw^2 and log2(m / m_scale) <= 0.
 // Such a product is below half of FP32's smallest denormal, so the FMA that
-// would add it leaves its accumulator unchanged: skipping changes no result
-// bit.  Small bandwidth sub-grids skip most stages of Z-ordered data.
+// would add it leaves its accumulator unchanged: passing over changes no
+// result bit.  Small bandwidth sub-grids skip most chunks of Z-ordered data.
 template <int A, int B, bool ADAPT, int U>
 __global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
   // the 8 x 8 tiling holds every bandwidth of a typical grid (nothing to
@@ -274,34 +285,34 @@ __global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
       const bool nxt_valid = rn + t < w.r_end;
       float4 nxt = make_float4(0.f, 0.f, kMaskedLog2, 0.f);
       if (nxt_valid) nxt = g.pts[rn + t];
-      bool live = true;
-      if constexpr (kCull) {
-        // both boxes re-read from LDS at every stage (no registers held across it)
-        double qb[5], tb[5];
-        box_read(qbox, qb);
-        box_read(tbox[buf], tb);
-        const double gx = fmax(0.0, fmax(tb[0] + qb[1], qb[0] + tb[1]));  // x0_t - x1_q, x0_q - x1_t
-        const double gy = fmax(0.0, fmax(tb[2] + qb[3], qb[2] + tb[3]));
-        live = !(tb[4] * (gx * gx * (double)kx_min + gy * gy * (double)ky_min) >= kCvGaussCullExp);
-      }
-      if (live) {
-        if (g.count && t == 0) {
-          // (query, reference, bandwidth pair) units of this stage
-          const int la = g.n_hx - w.sub_x < A ? g.n_hx - w.sub_x : A;
-          const int lb = g.n_hy - w.sub_y < B ? g.n_hy - w.sub_y : B;
-          const int nr = w.r_end - r0 < kCvTile ? w.r_end - r0 : kCvTile;
-          atomicAdd(g.count, (unsigned long long)nr * (unsigned long long)w.q_count * (unsigned long long)(la * lb));
+      // per chunk of kCvChunk references: this wavefront's query box against
+      // the chunk's box (both re-read from LDS at every stage, no registers
+      // held across it); a chunk out of reach of all its queries is passed over
+      const bool diag = r0 < q_end && w.q_begin < rn;
+      float acc[A][B];
+#pragma unroll
+      for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[a][b] = 0.0f;
+      bool any = false;
+      for (int c = 0; c < kCvTile / kCvChunk; ++c) {
+        if (r0 + c * kCvChunk >= w.r_end) break;
+        bool live = true;
+        if constexpr (kCull) {
+          const int wv = t >> 6;
+          const double gx = fmax(0.0, fmax(tbox[buf][0][c] + qbox[1][wv], qbox[0][wv] + tbox[buf][1][c]));
+          const double gy = fmax(0.0, fmax(tbox[buf][2][c] + qbox[3][wv], qbox[2][wv] + tbox[buf][3][c]));
+          live = !(tbox[buf][4][c] * (gx * gx * (double)kx_min + gy * gy * (double)ky_min) >= kCvGaussCullExp);
         }
-        const bool diag = r0 < q_end && w.q_begin < rn;
-        float acc[A][B];
-#pragma unroll
-        for (int a = 0; a < A; ++a)
-#pragma unroll
-          for (int b = 0; b < B; ++b) acc[a][b] = 0.0f;
+        if (!live) continue;
+        any = true;
+        if (g.count && (t & 63) == 0) count_units<A, B>(g, w, r0 + c * kCvChunk, t >> 6);
         if (diag)
-          gauss_stage<A, B, ADAPT, true, U>(tile[buf], q.x, q.y, qi - r0, kx, ky, acc);
+          gauss_stage<A, B, ADAPT, true, U>(tile[buf], c * kCvChunk, q.x, q.y, qi - r0, kx, ky, acc);
         else
-          gauss_stage<A, B, ADAPT, false, U>(tile[buf], q.x, q.y, 0, kx, ky, acc);
+          gauss_stage<A, B, ADAPT, false, U>(tile[buf], c * kCvChunk, q.x, q.y, 0, kx, ky, acc);
+      }
+      if (any) {
 #pragma unroll
         for (int a = 0; a < A; ++a)
 #pragma unroll
@@ -330,12 +341,12 @@ __global__ __launch_bounds__(kCvBlock) void cv_gauss_kernel(CvArgs g) {
   }
 }
 
-// The Epanechnikov pair kernel skips a stage whose references are all out of
-// the support of every bandwidth of the item's sub-grid for every query of
-// the item: the per-axis gap between the query box and the stage's box,
-// times the stage's smallest 1/abw and the sub-grid's smallest 1/h, is at
-// least 2 (the convolution kernel's support, wider than the kernel's) on one
-// axis.  The test repeats the pair term's own operations on smaller inputs,
+// The Epanechnikov pair kernel: a wavefront passes over a chunk of kCvChunk
+// references whose references are all out of the support of every bandwidth
+// of the item's sub-grid for every query of the wavefront: the per-axis gap
+// between its query box and the chunk's box, times the chunk's smallest
+// 1/abw and the sub-grid's smallest 1/h, is at least 2 (the convolution
+// kernel's support, wider than the kernel's) on one axis.  The test repeats the pair term's own operations on smaller inputs,
 // so every skipped term is exactly zero as computed and no result changes.
 template <int A, int B, bool ADAPT, int U>
 __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
@@ -395,31 +406,29 @@ __global__ __launch_bounds__(kCvBlock) void cv_epan_kernel(CvArgs g) {
       box_write(tbox[0], v);
     }
     __syncthreads();
-    double qb[5];
-    box_read(qbox, qb);
     int buf = 0;
     const int q_end = w.q_begin + w.q_count;
+    const int wv = t >> 6;
     for (int r0 = w.r_begin; r0 < w.r_end; r0 += kCvTile) {
       const int rn = r0 + kCvTile;
       double4 nxt = zero;
       if (rn + t < w.r_end) nxt = g.pts64[rn + t];
-      double tb[5];
-      box_read(tbox[buf], tb);
-      const double gx = fmax(0.0, fmax(tb[0] + qb[1], qb[0] + tb[1]));  // x0_t - x1_q, x0_q - x1_t
-      const double gy = fmax(0.0, fmax(tb[2] + qb[3], qb[2] + tb[3]));
-      const double ux = (ADAPT ? gx * tb[4] : gx) * kx_min, uy = (ADAPT ? gy * tb[4] : gy) * ky_min;
-      if (!(ux >= 2.0 || uy >= 2.0)) {
-        if (g.count && t == 0) {
-          // (query, reference, bandwidth pair) units of this stage
-          const int la = g.n_hx - w.sub_x < A ? g.n_hx - w.sub_x : A;
-          const int lb = g.n_hy - w.sub_y < B ? g.n_hy - w.sub_y : B;
-          const int nr = w.r_end - r0 < kCvTile ? w.r_end - r0 : kCvTile;
-          atomicAdd(g.count, (unsigned long long)nr * (unsigned long long)w.q_count * (unsigned long long)(la * lb));
-        }
-        if (r0 < q_end && w.q_begin < rn)
-          epan_stage<A, B, ADAPT, true, U>(tile[buf], q.x, q.y, qi - r0, kx, ky, accK, accT);
+      const bool diag = r0 < q_end && w.q_begin < rn;
+      // per chunk of kCvChunk references: this wavefront's query box against
+      // the chunk's box; a chunk out of the support for all its queries is
+      // passed over
+      for (int c = 0; c < kCvTile / kCvChunk; ++c) {
+        if (r0 + c * kCvChunk >= w.r_end) break;
+        const double gx = fmax(0.0, fmax(tbox[buf][0][c] + qbox[1][wv], qbox[0][wv] + tbox[buf][1][c]));
+        const double gy = fmax(0.0, fmax(tbox[buf][2][c] + qbox[3][wv], qbox[2][wv] + tbox[buf][3][c]));
+        const double ux = (ADAPT ? gx * tbox[buf][4][c] : gx) * kx_min;
+        const double uy = (ADAPT ? gy * tbox[buf][4][c] : gy) * ky_min;
+        if (ux >= 2.0 || uy >= 2.0) continue;
+        if (g.count && (t & 63) == 0) count_units<A, B>(g, w, r0 + c * kCvChunk, wv);
+        if (diag)
+          epan_stage<A, B, ADAPT, true, U>(tile[buf], c * kCvChunk, q.x, q.y, qi - r0, kx, ky, accK, accT);
         else
-          epan_stage<A, B, ADAPT, false, U>(tile[buf], q.x, q.y, 0, kx, ky, accK, accT);
+          epan_stage<A, B, ADAPT, false, U>(tile[buf], c * kCvChunk, q.x, q.y, 0, kx, ky, accK, accT);
       }
       if (rn < w.r_end) {
         tile[buf ^ 1][t] = nxt;
