@@ -308,8 +308,8 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float, tune: 
                                           "(quarter-rate transcendental)",
                      "note": f"over the evaluated units; separable product kernel: {A + B} exp per reference "
                              f"serve {A * B} bandwidth pairs, so the unfactored 9 FLOP/pair accounting exceeds "
-                             "the FP32 peak; stages whose products are all below 2^-152 (no bit of any sum) "
-                             "are skipped",
+                             "the FP32 peak; a wavefront passes over each 64-reference chunk whose products "
+                             "are all below 2^-152 for its queries (no bit of any sum)",
                      "unfactored_tflops": rate_main * 9 / 1e12},
         "best_lcv": [list(map(int, np.unravel_index(np.argmax(np.where(np.isfinite(lcv[d]), lcv[d], -np.inf)),
                                                     lcv[d].shape))) for d in range(len(CV_SIZES))],
@@ -345,8 +345,9 @@ def bench_cv_grid(bk, steps: int, warmup: int, cpu: bool, seconds: float, tune: 
         "evaluated_units": e_units, "evaluated_frac": e_units / units,
         "evaluated_units_per_s": e_units / (e_avg * 1e-3),
         "tiling_variant": int(e_variant), "schedule": e_schedule, "autotune": tune_table(e_tuned),
-        "note": "the pair kernel skips the 256-reference stages out of the support of every bandwidth of its "
-                "sub-grid (exactly zero terms); evaluated_units counts the units of the stages it runs",
+        "note": "a wavefront passes over each 64-reference chunk out of the support of every bandwidth of its "
+                "sub-grid for all its queries (exactly zero terms); evaluated_units counts the units of the "
+                "chunks it runs",
         "best_lscv": [list(map(int, np.unravel_index(np.argmin(elscv[d]), elscv[d].shape)))
                       for d in range(len(CV_SIZES))]}
     # independent check: the single-bandwidth C-ABI path on the 50k dataset

@@ -418,12 +418,13 @@ int bk_cv_grid_last_timing(const bk_cv_grid *plan, double *main_ms,
                            double *total_ms, long long *rescued);
 
 /* Units of the last run evaluated by the pair kernel, when it ran with
- * BK_CVG_COUNT_UNITS set: the units of the 256-reference stages it did not
- * skip.  An Epanechnikov stage is skipped when it lies beyond the support of
- * every bandwidth of its sub-grid (its terms exactly zero), a Gaussian stage
- * when every product of its x and y factors is below 2^-152 for every query
- * of the work item (each would add nothing to its FP32 stage sum).  Self
- * pairs are counted as evaluated.  No
+ * BK_CVG_COUNT_UNITS set: per wavefront (64 queries of a work item), the
+ * units of its lanes holding a query over the 64-reference chunks it did not
+ * pass over.  A wavefront passes over an Epanechnikov chunk beyond the
+ * support of every bandwidth of its sub-grid for all its queries (its terms
+ * exactly zero), a Gaussian chunk when every product of its x and y factors
+ * is below 2^-152 for all its queries (each would add nothing to its FP32
+ * stage sum).  Self pairs are counted as evaluated.  No
  * counterpart in the reference, whose cross-validation evaluates every pair
  * a tree traversal does not prune. */
 int bk_cv_grid_unit_count(const bk_cv_grid *plan, unsigned long long *evaluated);
