@@ -181,15 +181,17 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
 // |offsets| (sx, sy, ox, oy) -- both for the culling of a query box -- then,
 // from [2] on, 1 + kTile / kEpanSub float4: the FP32 reach boxes
 // (L_x, H_x, L_y, H_y) of the tile and of its sub-tiles, for the
-// wavefront-level tests.
+// wavefront-level tests; then the FP32 coordinate boxes (x0, x1, y0, y1) of
+// the sub-tiles, for the radial kernel's own test.
 // The tiles carry metadata up to a multiple of kEpanScanBatch (tiles past the
 // points hold empty boxes, out of reach of every query).
 constexpr int kEpanSub = 64;  // references per wavefront stage
-constexpr int kEpanTileMeta = 2 + (1 + kTile / kEpanSub + 1) / 2;
+constexpr int kEpanReachF4 = 1 + kTile / kEpanSub;  // reach boxes per tile
+constexpr int kEpanTileMeta = 2 + (kEpanReachF4 + 1) / 2 + kTile / kEpanSub / 2;
 constexpr int kEpanScanBatch = 8;  // tile-level reach tests whose loads are issued together
-// FP64 path: the same layout ([1] holds that path's scale summary; the reach
-// boxes are written for the Epanechnikov families).
-constexpr int kF64TileMeta = kEpanTileMeta;
+// FP64 path: the same layout up to the reach boxes ([1] holds that path's
+// scale summary; the reach boxes are written for the Epanechnikov families).
+constexpr int kF64TileMeta = 2 + (kEpanReachF4 + 1) / 2;
 inline long long epan_meta_tiles(long long n) {
   return (n + (long long)kTile * kEpanScanBatch - 1) / ((long long)kTile * kEpanScanBatch) * kEpanScanBatch;
 }

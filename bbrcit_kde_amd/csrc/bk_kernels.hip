@@ -1014,9 +1014,15 @@ __device__ __forceinline__ float4 epan_reach_box(float x0, float x1, float y0, f
   return reach_box_f32(x0, x1, y0, y1, rx, ry);
 }
 
-// Reach boxes of tile ti: [0] the tile's, [1 + s] that of its sub-tile s.
+// Reach boxes of tile ti: [0] the tile's, [1 + s] that of its sub-tile s
+// (stride kEpanTileMeta on FP32 points, kF64TileMeta in FP64 contexts).
+template <bool F64>
 __device__ __forceinline__ const float4 *epan_reach(const double4 *__restrict__ tb, long long ti) {
-  return reinterpret_cast<const float4 *>(tb + kEpanTileMeta * ti + 2);
+  return reinterpret_cast<const float4 *>(tb + (F64 ? kF64TileMeta : kEpanTileMeta) * ti + 2);
+}
+// FP32 coordinate boxes (x0, x1, y0, y1) of tile ti's sub-tiles (FP32 points).
+__device__ __forceinline__ const float4 *epan_sub_boxes(const double4 *__restrict__ tb, long long ti) {
+  return reinterpret_cast<const float4 *>(tb + kEpanTileMeta * ti + 2 + (kEpanReachF4 + 1) / 2);
 }
 
 // True when no query of the wavefront is inside the reach box e on both
@@ -1131,7 +1137,7 @@ __device__ __forceinline__ void epan_item(
         batch = b;
         float4 e[kEpanScanBatch];
 #pragma unroll
-        for (int i = 0; i < kEpanScanBatch; ++i) e[i] = epan_reach(tb, b + i)[0];
+        for (int i = 0; i < kEpanScanBatch; ++i) e[i] = epan_reach<F64>(tb, b + i)[0];
         unsigned m = 0;
 #pragma unroll
         for (int i = 0; i < kEpanScanBatch; ++i) m |= epan_wave_far<kEpanQpt>(e[i], xu, xd, yu, yd) ? 0u : 1u << i;
@@ -1149,13 +1155,33 @@ __device__ __forceinline__ void epan_item(
       }
       if (ti != stile) {
         stile = ti;
-        const float4 *er = epan_reach(tb, ti) + 1;
+        const float4 *er = epan_reach<F64>(tb, ti) + 1;
         float4 e[kSubs];
 #pragma unroll
         for (int i = 0; i < kSubs; ++i) e[i] = er[i];
         unsigned m = 0;
 #pragma unroll
         for (int i = 0; i < kSubs; ++i) m |= epan_wave_far<kEpanQpt>(e[i], xu, xd, yu, yd) ? 0u : 1u << i;
+        if constexpr (FAM == FAM_EPAN && !F64) {
+          // the radial kernel's own test on the sub-tiles left: its support
+          // is the disc inside the reach box (epan_tile_far's bound, per
+          // query, with the tile's smallest scale and largest offsets)
+          if (m) {
+            const double4 sc = tb[kEpanTileMeta * ti + 1];
+            const float4 *sb = epan_sub_boxes(tb, ti);
+#pragma unroll
+            for (int i = 0; i < kSubs; ++i) {
+              if (!((m >> i) & 1u)) continue;
+              const float4 bx = sb[i];
+              bool near = false;
+#pragma unroll
+              for (int k = 0; k < kEpanQpt; ++k)
+                near |= !epan_beyond_support<FAM_EPAN>(u_lower(bx.x, bx.y, px[k], px[k], sc.x, sc.z),
+                                                       u_lower(bx.z, bx.w, py[k], py[k], sc.y, sc.w));
+              if (!__any(near)) m &= ~(1u << i);
+            }
+          }
+        }
         near_s = m;
       }
       const unsigned srest = near_s >> (int)((sub - ti * kTile) / kEpanSub);
@@ -1400,8 +1426,11 @@ __global__ __launch_bounds__(kTile) void prep_epan_tiles_kernel(const float *__r
   const float w_sy = wave_reduce_f(valid ? down(syv) : big, false);
   const float w_ox = wave_reduce_f(valid ? up(oxv) : 0.0f, true);
   const float w_oy = wave_reduce_f(valid ? up(oyv) : 0.0f, true);
-  if ((threadIdx.x & 63) == 0)
+  if ((threadIdx.x & 63) == 0) {
     reach[1 + (threadIdx.x >> 6)] = epan_reach_box(w_x0, w_x1, w_y0, w_y1, w_sx, w_sy, w_ox, w_oy, c);
+    reinterpret_cast<float4 *>(m + 2 + (kEpanReachF4 + 1) / 2)[threadIdx.x >> 6] =
+        make_float4(w_x0, w_x1, w_y0, w_y1);
+  }
 }
 
 // ---- FP64 context path ---------------------------------------------------
