@@ -231,8 +231,9 @@ int bk_eval_range_device(bk_ctx *ctx, const bk_kernel_desc *desc,
  * 64 MiB device array per GPU that the library holds until the process
  * ends (its address is looked up in bk_create).  Limit: the features of the
  * context's whole reference set must fit in it (FP32 Gaussian kinds
- * 32.4 bytes per reference, about 2.07M references; Epanechnikov kinds and
- * FP64 contexts 64.25 bytes, about 1.04M; D >= 3 16 or 32 bytes); larger
+ * 32.4 bytes per reference, about 2.07M references; Epanechnikov kinds
+ * 64.9 bytes, about 1.03M; FP64 contexts 64.6 bytes, about 1.04M; D >= 3
+ * 16 or 32 bytes); larger
  * sets fail with BK_ERR_INVALID_ARG before any work is queued.  The split
  * count is lowered until the partial sums fit beside the features, so
  * results agree with bk_eval_range_device up to the FP64 order of the split

@@ -129,7 +129,7 @@ def test_noalloc_other_paths_and_smaller_second_call(bk, oracle):
 
 def test_noalloc_rejects_reference_sets_beyond_its_scratch(bk):
     torch = pytest.importorskip("torch")
-    # FP64 features take 64.25 bytes per reference: 1.1M references exceed 64 MiB
+    # FP64 features take 64.6 bytes per reference: 1.1M references exceed 64 MiB
     n = 1_100_000
     pts = np.random.default_rng(3).uniform(-1.0, 1.0, (n, 2))
     q = pts[:1000]
