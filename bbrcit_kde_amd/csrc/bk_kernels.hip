@@ -1438,33 +1438,25 @@ constexpr int kBlock64 = 256;
 constexpr int kTile64 = 256;
 
 // 2^e in FP64 for e <= 0: e = n + f with n = rint(e) and |f| <= 1/2 (for
-// every finite e), 2^f = exp(f ln 2) by its Taylor series to degree 12 in
-// Horner form (truncation below 2e-16 relative for |f ln 2| <= 0.347;
-// coefficients (ln 2)^k / k! rounded once), scaled by ldexp.  The scale is
-// raised to 2^-2000 at least, which underflows to 0 as the true value does;
-// NaN stays NaN (f is NaN).
-constexpr double kLn2 = 0.6931471805599453;
-constexpr double exp2_coef(int k) {
-  double c = 1.0;
-  for (int i = 1; i <= k; ++i) c = c * kLn2 / i;
-  return c;
-}
-
+// every finite e), 2^f by its Chebyshev interpolant of degree 10 on
+// [-1/2, 1/2] in Horner form (coefficients from exp2_f64_coef.py, each
+// rounded once to FP64: 2.2e-16 relative interpolation error, 3.8e-16 with
+// the FMA roundings), scaled by ldexp.  The scale is raised to 2^-2000 at
+// least, which underflows to 0 as the true value does; NaN stays NaN (f is
+// NaN).
 __device__ __forceinline__ double exp2_f64(double e) {
   const double n = __builtin_rint(e);
   const double f = e - n;
-  double p = exp2_coef(12);
-  p = __builtin_fma(p, f, exp2_coef(11));
-  p = __builtin_fma(p, f, exp2_coef(10));
-  p = __builtin_fma(p, f, exp2_coef(9));
-  p = __builtin_fma(p, f, exp2_coef(8));
-  p = __builtin_fma(p, f, exp2_coef(7));
-  p = __builtin_fma(p, f, exp2_coef(6));
-  p = __builtin_fma(p, f, exp2_coef(5));
-  p = __builtin_fma(p, f, exp2_coef(4));
-  p = __builtin_fma(p, f, exp2_coef(3));
-  p = __builtin_fma(p, f, exp2_coef(2));
-  p = __builtin_fma(p, f, exp2_coef(1));
+  double p = 7.072585949269223e-09;
+  p = __builtin_fma(p, f, 1.0208690299958306e-07);
+  p = __builtin_fma(p, f, 1.321544258792169e-06);
+  p = __builtin_fma(p, f, 1.5252657260200837e-05);
+  p = __builtin_fma(p, f, 0.0001540353044173605);
+  p = __builtin_fma(p, f, 0.0013333558230164974);
+  p = __builtin_fma(p, f, 0.009618129107606888);
+  p = __builtin_fma(p, f, 0.05550410866444772);
+  p = __builtin_fma(p, f, 0.24022650695910097);
+  p = __builtin_fma(p, f, 0.69314718055995);
   p = __builtin_fma(p, f, 1.0);
   return __builtin_ldexp(p, (int)__builtin_fmax(n, -2000.0));
 }
