@@ -29,7 +29,10 @@ def mixture(n, seed):
     return pts, w / w.sum()
 
 
-def test_leaf_pairs_match_individual_ranges(bk, oracle):
+@pytest.mark.parametrize("kind,h", [(GAUSSPROD, (0.09, 0.13)), (EPANPROD, (0.09, 0.13)), (EPAN, (0.1, 0.1))])
+def test_leaf_pairs_match_individual_ranges(bk, oracle, kind, h):
+    # unaligned reference and query ranges: the Epanechnikov work items start
+    # inside sub-tiles and end in partial ones
     pts, mass = mixture(6000, 31)
     q = ds.uniform_box(4000, -3, 3, 32)
     rng = np.random.default_rng(1)
@@ -42,14 +45,14 @@ def test_leaf_pairs_match_individual_ranges(bk, oracle):
         pairs.append((r_i, r_j, q_i, q_j))
     pairs.append((0, 5999, 0, 3999))
     pairs.append((17, 17, 3999, 3999))
-    desc = bk.KernelDesc(GAUSSPROD, 2, 0.09, 0.13)
+    desc = bk.KernelDesc(kind, 2, *h)
     P, Q, M = f32(pts), f32(q), f32(mass)
     with bk.DirectKde(pts, q, mass) as kde:
         outs = kde.eval_leaf_pairs(desc, pairs)
         single = kde.unnormalized_eval(desc, *pairs[5])
     for (r_i, r_j, q_i, q_j), got in zip(pairs, outs):
-        want = oracle.direct(GAUSSPROD, 0.09, 0.13, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
-        assert_parity(got, want, M[r_i:r_j + 1].sum(), what=f"pair {(r_i, r_j, q_i, q_j)}")
+        want = oracle.direct(kind, *h, P[r_i:r_j + 1], Q[q_i:q_j + 1], mass=M[r_i:r_j + 1])
+        assert_parity(got, want, M[r_i:r_j + 1].sum(), what=f"kind {kind} pair {(r_i, r_j, q_i, q_j)}")
     np.testing.assert_allclose(outs[5], single, rtol=1e-6)
 
 
