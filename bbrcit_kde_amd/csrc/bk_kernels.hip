@@ -1493,6 +1493,18 @@ __device__ __forceinline__ double pair_term_f64(double qx, double qy,
   }
 }
 
+// The FP64 Gaussian term in a compact tile's centred form: with u = q - c
+// (c the tile's centre) and the reference's r' = r - c,
+//   e = log2 m - W |q - r|^2_rho = C_r + A0 ux + A1 uy - W |u|^2_rho,
+// A0 = 2 W r'x, A1 = 2 W rho r'y, C_r = log2 m - W |r'|^2_rho, a = (A0, A1,
+// C_r, -W): three FMAs in place of the differences, their squares and the
+// mass product.  Every part is at most (sqrt|e| + 10)^2 in a compact tile
+// (W |r'|^2_rho <= 100), so the FP64 rounding of e stays below ~2e-16 of
+// that, far inside the 1e-12 bar for every term above underflow.
+__device__ __forceinline__ double gauss_centred_f64(double ux, double uy, double u2, double4 a) {
+  return exp2_f64(__builtin_fma(a.w, u2, __builtin_fma(a.x, ux, __builtin_fma(a.y, uy, a.z))));
+}
+
 // Per aligned tile of kTile64 FP64 references, two double4: the box (x0, x1,
 // y0, y1) and (min a.z, min a.w, max a.z, sum |a.w|) of pair_term_f64's
 // layout (FAM_GAUSS: smallest and largest exponent scale, smallest mass, sum
@@ -1652,6 +1664,44 @@ __device__ __forceinline__ void direct_f64_item(
        tile = live_from(tile + kTile64)) {
     const long long r = tile + t;
     const bool in = r >= r_begin && r < r_end;
+    if constexpr (FAM == FAM_GAUSS) {
+      // a compact tile is summed in its centred form (f1; a masked slot
+      // has C = -1e300, a zero term)
+      const double4 cm = tb[kF64TileMeta * (tile / kTile64) + 2];
+      if (cm.z != 0.0) {
+        s0[t] = in ? f1[r] : make_double4(0.0, 0.0, -1.0e300, 0.0);
+        __syncthreads();
+        const int cnt = (int)((r_end - tile) < kTile64 ? (r_end - tile) : kTile64);
+        const bool diag = self_excl && tile < q_hi && q_lo < tile + cnt;
+        double ux[kQpt], uy[kQpt], u2[kQpt];
+#pragma unroll
+        for (int k = 0; k < kQpt; ++k) {
+          ux[k] = px[k] - cm.x;
+          uy[k] = py[k] - cm.y;
+          u2[k] = __builtin_fma(rho * uy[k], uy[k], ux[k] * ux[k]);
+        }
+        if (!diag) {
+#pragma unroll 2
+          for (int j = 0; j < cnt; ++j) {
+            const double4 a = s0[j];
+#pragma unroll
+            for (int k = 0; k < kQpt; ++k) acc[k] += gauss_centred_f64(ux[k], uy[k], u2[k], a);
+          }
+        } else {
+          for (int j = 0; j < cnt; ++j) {
+            const double4 a = s0[j];
+            const long long rj = tile + j;
+#pragma unroll
+            for (int k = 0; k < kQpt; ++k) {
+              const double v = gauss_centred_f64(ux[k], uy[k], u2[k], a);
+              acc[k] += (rj == q_begin + t + k * kBlock64) ? 0.0 : v;
+            }
+          }
+        }
+        __syncthreads();
+        continue;
+      }
+    }
     s0[t] = in ? f0[r] : zero;
     if constexpr (kNf == 2) s1[t] = in ? f1[r] : zero;
     __syncthreads();
@@ -1761,10 +1811,15 @@ __global__ void prep_f64_kernel(EvalParams p, const double *__restrict__ x,
 // 2 (1 + 2^-20) / s_min with s_min the smallest scale (convolution kernel):
 // the FP64 products and sums that form the term lose at most a few 2^-53 of
 // its parts.
+// For the Gaussian: [2] = (cx, cy, compact, 0), the box centre and whether
+// every reference has W |r - c|^2_rho <= 100, and for a compact tile the
+// centred features of gauss_centred_f64 in f1.
 __global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *__restrict__ x,
                                                                   const double *__restrict__ y,
+                                                                  const double *__restrict__ m,
                                                                   const double4 *__restrict__ f0, long long n,
-                                                                  int family, double4 *__restrict__ tb) {
+                                                                  int family, double rho, double4 *__restrict__ tb,
+                                                                  double4 *__restrict__ f1) {
   static_assert(kTile64 == kTile && kEpanSub == 64, "one sub-tile per wavefront");
   __shared__ double sh[kTile64];
   const long long i = (long long)blockIdx.x * kTile64 + threadIdx.x;
@@ -1805,7 +1860,20 @@ __global__ __launch_bounds__(kTile64) void prep_f64_tiles_kernel(const double *_
     mt[0] = make_double4(x0, x1, y0, y1);
     mt[1] = make_double4(wz, ww, wz_max, sh[0]);
   }
-  if (family == FAM_GAUSS) return;
+  if (family == FAM_GAUSS) {
+    const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+    const double rx = xi - cx, ry = yi - cy;
+    const double W = a.z;
+    const double r2 = valid ? W * __builtin_fma(rho * ry, ry, rx * rx) : 0.0;
+    const bool compact = !(reduce(r2, true) > 100.0);
+    if (threadIdx.x == 0) mt[2] = make_double4(cx, cy, compact ? 1.0 : 0.0, 0.0);
+    if (compact && valid) {
+      const double mi = m[i];
+      const double c = mi > 0.0 ? log2(mi) - W * __builtin_fma(rho * ry, ry, rx * rx) : -1.0e300;
+      f1[i] = make_double4(2.0 * W * rx, 2.0 * W * rho * ry, c, -W);
+    }
+    return;
+  }
   auto reach = [&](double s_min) {
     return family == FAM_EPANCONV ? 2.0 * (1.0 + 0x1p-20) / s_min : (1.0 + 0x1p-20) / sqrt(s_min);
   };
@@ -2650,7 +2718,8 @@ hipError_t launch_prep_f64(const EvalParams &p, const double *x,
       p, x, y, m, abw, (long long)n, f0, f1);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  prep_f64_tiles_kernel<<<(unsigned)epan_meta_tiles(n), kTile64, 0, s>>>(x, y, f0, (long long)n, p.family, tb);
+  prep_f64_tiles_kernel<<<(unsigned)epan_meta_tiles(n), kTile64, 0, s>>>(x, y, m, f0, (long long)n, p.family,
+                                                                          p.gy / p.gx, tb, f1);
   return hipGetLastError();
 }
 

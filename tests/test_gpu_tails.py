@@ -12,6 +12,9 @@ magnitude below an upper bound of the sum.
     base-2 exponent units): queries whose sums sit just above the parity
     floor (1e-10 of C * sum m) are held to 1e-5 relative.  The worst
     relative error is printed.
+  * FP64 Gaussian, compact tiles in their centred form and wide ones from
+    the differences: a mixture 5000 units from the origin, queries near the
+    data and beyond it, held to 1e-12 relative down to the FP64 floor.
   * D = 3 Gaussian (FP64 distances of the FP32 points): queries 10 to 16
     from the centre of a standard normal sample, sums far below the FP32
     range (down to ~1e-150), held to 1e-5 relative with no floor (the
@@ -197,3 +200,24 @@ def test_d3_far_queries_without_floor(bk, oracle):
     assert want.min() > 1e-290 and want.min() < 1e-100
     assert (want < 1e-45).mean() > 0.5  # below the smallest FP32 denormal
     assert rel_err(got, want).max() <= 1e-5
+
+
+@pytest.mark.parametrize("h", [(0.03, 0.03), (0.02, 0.035)])
+def test_f64_gaussian_centred_and_direct_tiles_far_from_origin(bk, oracle, h):
+    # FP64 Gaussian tiles compact enough (W |r - c|^2_rho <= 100) are summed in
+    # their centred form, the others from the differences; a mixture 5000
+    # units from the origin has both (its sparse tails are wide), and the
+    # centred parts cancel nothing of the offset.  Held to 1e-12 relative
+    # down to the FP64 tail floor, queries near the data and beyond it.
+    kind = GAUSS if h[0] == h[1] else GAUSSPROD
+    pts, w = ds.gaussian_mixture(30_000, seed=1300)
+    pts = pts[ds.kd_order(pts, 64)] + np.array([5000.0, -3000.0])
+    mass = w / w.sum()
+    rng = np.random.default_rng(1301)
+    q = np.vstack([pts[rng.choice(len(pts), 1500, replace=False)] + rng.normal(0.0, 0.05, (1500, 2)),
+                   ds.uniform_box(1500, -4.0, 4.0, 1302) + np.array([5000.0, -3000.0])])
+    desc = bk.KernelDesc(kind, 2, *h)
+    with bk.DirectKde(pts, q, mass, precision=bk.BK_PREC_F64) as kde:
+        got = kde.unnormalized_eval(desc, 0, len(pts) - 1, 0, len(q) - 1)
+    want = oracle.direct(kind, h[0], h[1], pts, q, mass=mass)
+    assert_parity(got, want, mass.sum(), rtol=1e-12, what=f"f64 offset kind {kind}", kind=kind)
