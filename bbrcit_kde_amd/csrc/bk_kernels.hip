@@ -5,14 +5,15 @@
 // sum += m * U(q, r; h*abw) in FP32 (include/CudaDirectKde.h:126-173).
 // Here the same sum is organised for CDNA4:
 //   * queries are read once, coalesced, from SoA arrays and kept in
-//     registers, kQpt per thread;
+//     registers (one per thread on FP32 points, kQpt in the FP64-context
+//     Gaussian kernel);
 //   * references are pre-transformed once per kernel descriptor into
 //     per-point feature vectors, so the reference's per-pair division by
 //     h^2 a^2 becomes a multiply;
 //   * a workgroup stages kTile feature vectors in LDS (double-buffered, one
-//     barrier per stage, the next stage prefetched into registers); every
-//     lane of a wavefront reads the same entry, so each LDS read is a
-//     broadcast;
+//     barrier per stage, the next stage prefetched into registers; the
+//     Epanechnikov kernels stage per wavefront, below); every lane of a
+//     wavefront reads the same entry, so each LDS read is a broadcast;
 //   * Gaussian kinds (FP32): the term m*exp(-c d2 / (h a)^2) is one
 //     v_exp_f32 of a base-2 exponent.  On spatially compact tiles the
 //     exponent is formed from tile-centred coordinates as
