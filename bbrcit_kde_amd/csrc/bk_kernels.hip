@@ -395,7 +395,7 @@ __device__ __forceinline__ void gauss_packed_loop(const float4 *__restrict__ sf,
     // the common case: all kGaussTile references, a fixed trip count (the
     // compiler hoists the LDS reads of later groups across the unrolled body)
     constexpr int G = kGaussTile / 4;
-#pragma unroll 4
+#pragma unroll 8
     for (int g = 0; g < G; ++g) {
       const float4 a = sf[3 * g], b = sf[3 * g + 1], c = sf[3 * g + 2];
       term(a.x, a.y, a.z, 0);
