@@ -459,8 +459,9 @@ __global__ void cv_reset_kernel(int *queue) {
 }
 
 // Sum the reference splits of one dataset into slot 0 (fixed order).
-__global__ void cv_combine_kernel(double *__restrict__ P, long long slot_stride, long long plane_stride,
-                                  long long n_points, int planes, int G, int splits, int begin, int n) {
+// (P is [slot][plane][g][point], so plane p, bandwidth g sit at (p G + g) n_points.)
+__global__ void cv_combine_kernel(double *__restrict__ P, long long slot_stride, long long n_points, int planes,
+                                  int G, int splits, int begin, int n) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long total = (long long)planes * G * n;
   if (k >= total) return;
@@ -807,7 +808,6 @@ std::vector<CvItem> make_items(const bk_cv_grid *g, int A, int B, int nsx, int n
   std::vector<CvItem> items;
   for (int d = 0; d < g->n_sets; ++d) {
     const int b = g->begin[d], e = g->begin[d + 1];
-    const long long n = e - b;
     const int s = g->splits[d];
     const long long len = g->split_len[d];
     for (int q0 = b; q0 < e; q0 += kCvBlock) {
@@ -959,8 +959,7 @@ void enqueue_pipeline(bk_cv_grid *g, bool time_main) {
     const int n = g->begin[d + 1] - g->begin[d];
     const long long total = (long long)planes * G * n;
     hipLaunchKernelGGL(cv_combine_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g->P.as<double>(),
-                       args.slot_stride, args.plane_stride, g->n_points, planes, (int)G, g->splits[d], g->begin[d],
-                       n);
+                       args.slot_stride, g->n_points, planes, (int)G, g->splits[d], g->begin[d], n);
     hip_check(hipGetLastError(), "combine launch");
   }
   if (g->fam == CV_GAUSS) {

@@ -78,10 +78,6 @@ constexpr int kSubTile = kGaussTile / 4;
 constexpr int kGaussPackedF4 = kGaussTile * 3 / 4;
 static_assert(kGaussTile % 4 == 0 && kGaussPackedF4 <= kGaussTile, "packed features fit the tile's block");
 
-__device__ __forceinline__ float clamp01(float v) {
-  return __builtin_fminf(__builtin_fmaxf(v, 0.0f), 1.0f);
-}
-
 __device__ __forceinline__ void write_zeros(double *dst, int q_count) {
   for (int qi = threadIdx.x; qi < q_count; qi += kBlock) dst[qi] = 0.0;
 }

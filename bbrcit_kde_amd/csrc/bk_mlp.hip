@@ -538,17 +538,6 @@ __global__ void zero_u32_kernel(unsigned int *p, int n) {
 
 unsigned grid_for(long long n, int threads) { return (unsigned)std::max<long long>(1, (n + threads - 1) / threads); }
 
-// Host-side BF16 round-to-nearest-even (for the initial working copy).
-float host_bf16(float x) {
-  uint32_t u;
-  std::memcpy(&u, &x, 4);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  u &= 0xFFFF0000u;
-  float r;
-  std::memcpy(&r, &u, 4);
-  return r;
-}
-
 }  // namespace
 
 struct bk_mlp {
