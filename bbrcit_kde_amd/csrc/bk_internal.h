@@ -188,7 +188,7 @@ hipError_t launch_gauss(const EvalParams &p, const float *qx, const float *qy,
 constexpr int kEpanSub = 64;  // references per wavefront stage
 constexpr int kEpanReachF4 = 1 + kTile / kEpanSub;  // reach boxes per tile
 constexpr int kEpanTileMeta = 2 + (kEpanReachF4 + 1) / 2 + kTile / kEpanSub / 2;
-constexpr int kEpanScanBatch = 8;  // tile-level reach tests whose loads are issued together
+constexpr int kEpanScanBatch = 4;  // tile-level reach tests whose loads are issued together
 // FP64 path: the same layout up to the reach boxes ([1] holds that path's
 // scale summary; the reach boxes are written for the Epanechnikov families).
 constexpr int kF64TileMeta = 2 + (kEpanReachF4 + 1) / 2;
